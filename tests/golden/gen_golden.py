@@ -1,0 +1,123 @@
+"""Write the golden vectors under tests/golden/ from the reference itself.
+
+Runs only in a container that holds the reference sources: it builds
+oracle/_ref/ref_driver with oracle/Makefile.ref (the reference's serial CPU
+FMM compiled from its own sources, compile-time mp=9 as shipped in
+include/constants.h:12) and feeds it seeded inputs.  Each case is saved as
+ref_<name>.npz holding the inputs and every array the driver recorded:
+sort permutations, box keys/ranges, near and far interaction lists, precalc
+tables, multipoles after P2M and after the upward pass, local expansions per
+level, P2P-only and final potentials/gradients (sorted and caller order), the
+direct sum.  (The shipped fmm() itself is not run: with its level-2 neib=3
+list it reads past the end of nfj at kernel/m2l.cxx:43, a heap overflow
+under AddressSanitizer.)
+
+Usage: python tests/golden/gen_golden.py [--ref /root/reference]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+ORACLE = os.path.join(ROOT, "oracle")
+
+PI_F = np.float32(np.pi)
+
+# name: (ni, nj, lmax, neq, same_points, asis, seed)
+CASES = {
+    "g3": (1200, 1100, 3, 0, False, False, 11),
+    "gni3": (900, 1000, 3, 1, False, False, 12),
+    "gnj3": (1000, 900, 3, 2, False, False, 13),
+    "g2": (400, 450, 2, 0, False, False, 14),
+    "g4same": (2500, 2500, 4, 0, True, False, 15),
+}
+# arrays kept in every case; expansion dumps only where listed (size)
+FULL = {"g3", "gnj3", "g2"}
+
+
+def make_inputs(ni, nj, same, seed):
+    """test/test.cxx:72-92 style: points uniform in [-pi,pi)^3 (float),
+    normals/charges uniform in [0,1) -- drawn with numpy, not rand()."""
+    rng = np.random.default_rng(seed)
+    xi = (rng.random((ni, 3), dtype=np.float32) * np.float32(2) * PI_F - PI_F).astype(np.float32)
+    ni_n = rng.random((ni, 3), dtype=np.float32)
+    if same:
+        xj = xi.copy()
+    else:
+        xj = (rng.random((nj, 3), dtype=np.float32) * np.float32(2) * PI_F - PI_F).astype(np.float32)
+    nj_n = rng.random((nj, 3), dtype=np.float32)
+    q = rng.random(nj, dtype=np.float32)
+    return xi, ni_n, xj, nj_n, q
+
+
+def write_input(path, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q):
+    box = np.array([-PI_F, -PI_F, -PI_F, np.float32(2) * PI_F], np.float32)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<5i", ni, nj, lmax, neq, int(asis)))
+        f.write(box.tobytes())
+        for k in range(3):
+            f.write(np.ascontiguousarray(xi[:, k]).tobytes())
+        for k in range(3):
+            f.write(np.ascontiguousarray(ni_n[:, k]).tobytes())
+        for k in range(3):
+            f.write(np.ascontiguousarray(xj[:, k]).tobytes())
+        for k in range(3):
+            f.write(np.ascontiguousarray(nj_n[:, k]).tobytes())
+        f.write(q.tobytes())
+    return box
+
+
+def read_dump(path):
+    out = {}
+    data = open(path, "rb").read()
+    p = 0
+    while p < len(data):
+        name = data[p:p + 32].split(b"\0", 1)[0].decode()
+        typ, n = struct.unpack_from("<2i", data, p + 32)
+        p += 40
+        dt = {0: np.int32, 1: np.float32, 2: np.float64}[typ]
+        es = np.dtype(dt).itemsize
+        out[name] = np.frombuffer(data[p:p + es * n], dtype=dt).copy()
+        p += es * n
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--only", default=None)
+    a = ap.parse_args()
+    subprocess.run(["make", "-s", "-C", ORACLE, "-f", "Makefile.ref", f"REF={a.ref}", "-j8"], check=True)
+    exe = os.path.join(ORACLE, "_ref", "ref_driver")
+    tmp = os.path.join(ORACLE, "_ref", "tmp")
+    os.makedirs(tmp, exist_ok=True)
+    for name, (ni, nj, lmax, neq, same, asis, seed) in CASES.items():
+        if a.only and name != a.only:
+            continue
+        xi, ni_n, xj, nj_n, q = make_inputs(ni, nj, same, seed)
+        fin = os.path.join(tmp, f"{name}.in")
+        fout = os.path.join(tmp, f"{name}.out")
+        box = write_input(fin, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q)
+        subprocess.run([exe, fin, fout], check=True, stdout=subprocess.DEVNULL)
+        d = read_dump(fout)
+        keep = {}
+        for k, v in d.items():
+            if name not in FULL and (k.startswith("ax_") or k.endswith("bx") or k == "dnm_sample"):
+                continue
+            keep[k] = v
+        keep.update(in_xi=xi, in_ni=ni_n, in_xj=xj, in_nj=nj_n, in_q=q, in_box=box,
+                    in_cfg=np.array([ni, nj, lmax, neq, int(same), int(asis), seed], np.int32))
+        dst = os.path.join(HERE, f"ref_{name}.npz")
+        np.savez_compressed(dst, **keep)
+        print(f"{name}: {len(keep)} arrays -> {os.path.relpath(dst, ROOT)} ({os.path.getsize(dst)//1024} KiB)")
+
+
+if __name__ == "__main__":
+    sys.exit(main())
