@@ -1,0 +1,168 @@
+/*
+ * fmmx.h -- C ABI of the MI355X FMM engine (libfmmx.so).
+ *
+ * The engine evaluates the reference's Laplace FMM matvec (fmm/fmm.cxx:29,
+ * "void fmm(int mi, int mj, int neq)") on one GPU: the tree (Morton keys,
+ * stable sort, box lists, interaction lists), P2P, P2M, M2M, M2L, L2L, L2P
+ * and the direct sums, with the reference's kernels G (neq 0), Gni (neq 1)
+ * and Gnj (neq 2).
+ *
+ * The reference's interface is a set of C++ symbols over global arrays
+ * (include/arrays.h, include/constants.h) that callers sequence themselves
+ * (test/test.cxx:107-116, bio/main.cxx:327-336).  This ABI replaces it with
+ * an explicit handle: plain pointers and sizes, no global state, integer
+ * status codes instead of exit().  libfmmx_ref.so (fmmx_refshim.h) layers
+ * the reference's own C++ symbols on top of it for unchanged callers.
+ *
+ * Threading: one handle per host thread; calls on one handle are not
+ * reentrant.  All work runs on the handle's HIP stream.
+ */
+#ifndef FMMX_H
+#define FMMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMMX_ABI_VERSION 1
+
+/* status codes */
+enum {
+  FMMX_OK = 0,
+  FMMX_EINVAL = 1,   /* bad argument (sizes, order, NaN coordinates, ...) */
+  FMMX_ESTATE = 2,   /* call out of sequence (e.g. evaluate before set_points) */
+  FMMX_EDEVICE = 3,  /* HIP runtime error */
+  FMMX_ENOMEM = 4,   /* device or host allocation failed */
+  FMMX_EUNSUPPORTED = 5
+};
+
+/* kernels: the reference's neq values (fmm/fmm.cxx:47-65) */
+enum { FMMX_G = 0, FMMX_GNI = 1, FMMX_GNJ = 2 };
+
+/* arithmetic of the evaluation */
+enum {
+  FMMX_MIXED = 0, /* FP32 pair/expansion arithmetic, FP64 tables and per-target sums */
+  FMMX_FP64 = 1   /* FP64 throughout ("plain") */
+};
+
+/* conventions */
+enum {
+  FMMX_CONV_REF = 0,  /* reference arithmetic: r^2+eps softening (self pair
+                         included, include/constants.h:34), eps added to radii
+                         in expansions and rotations, float pi, float tables */
+  FMMX_CONV_EXACT = 1 /* unsoftened 1/r kernel, r=0 pairs (self) excluded,
+                         exact geometry and tables */
+};
+
+/* flags */
+enum {
+  FMMX_DEVICE_PTRS = 1, /* all array arguments of this call are device pointers */
+  FMMX_KEEP_LISTS = 2   /* set_points: also build the far-field (M2L)
+                           interaction lists in the reference's order for
+                           fmmx_get_list (M2L itself does not need them) */
+};
+
+typedef struct fmmx_config {
+  int p;          /* expansion degrees n = 0..p-1 (the reference's mp), 2..FMMX_MAX_P */
+  int lmax;       /* leaf level >= 2; 0 = fmm/nlevel.cxx thresholds on nj */
+  int precision;  /* FMMX_MIXED or FMMX_FP64 */
+  int convention; /* FMMX_CONV_REF or FMMX_CONV_EXACT */
+  float xmin, ymin, zmin, rd; /* root cube (include/constants.h:56-59);
+                                 rd <= 0: computed from the points as
+                                 bio/main.cxx:364-380 does */
+} fmmx_config;
+
+#define FMMX_MAX_P 12
+
+typedef struct fmmx_handle fmmx_handle;
+
+/* Create a handle on HIP device `device`; builds the FP64 tables and the
+ * dense translation operators for (p, convention).  stream may be NULL
+ * (the default stream). */
+int fmmx_create(const fmmx_config* cfg, int device, void* hip_stream, fmmx_handle** out);
+int fmmx_destroy(fmmx_handle* h);
+
+/* Tree set-up for targets (xi..) and sources (xj..), caller order: the
+ * reference's nlevel/boxpart/sorti/sortj/boxallocate/alloc chain
+ * (test/test.cxx:107-112) plus fmm.cxx steps boxdatai/boxdataj and the
+ * interaction lists.  nx*: target normals (needed for FMMX_GNI) and source
+ * normals (FMMX_GNJ); NULL when unused.  Reused by every evaluate until the
+ * next set_points. */
+int fmmx_set_points(fmmx_handle* h, int ni, const float* xi, const float* yi, const float* zi,
+                    const float* nxi, const float* nyi, const float* nzi, int nj, const float* xj,
+                    const float* yj, const float* zj, const float* nxj, const float* nyj,
+                    const float* nzj, int flags);
+
+/* One FMM matvec (fmm/fmm.cxx:29-152) for source strengths qj (caller
+ * order).  Outputs in caller order: vi (potential, or the Gni/Gnj layer
+ * potential) and, for FMMX_G only, the gradient gxi,gyi,gzi (may be NULL).
+ * Unlike the reference, outputs are overwritten (not accumulated) and are
+ * double.  Factor 1/(4 pi) included as in kernel/G_p2p.cxx:25. */
+int fmmx_evaluate(fmmx_handle* h, int neq, const float* qj, double* vi, double* gxi, double* gyi,
+                  double* gzi, int flags);
+
+/* Direct sums (kernel/G_dir.cxx:6, Gni_dir.cxx:6, Gnj_dir.cxx:6) on the GPU
+ * for ni targets against nj sources, FP64 accumulation, in the handle's
+ * convention.  Stateless apart from the handle's device and stream. */
+int fmmx_direct(fmmx_handle* h, int neq, int ni, const float* xi, const float* yi, const float* zi,
+                const float* nxi, const float* nyi, const float* nzi, int nj, const float* xj,
+                const float* yj, const float* zj, const float* nxj, const float* nyj,
+                const float* nzj, const float* qj, double* vi, double* gxi, double* gyi,
+                double* gzi, int flags);
+
+/* Wait for the handle's stream. */
+int fmmx_synchronize(fmmx_handle* h);
+
+/* Timing and sizes of the last set_points/evaluate (device time from HIP
+ * events per phase, milliseconds). */
+typedef struct fmmx_stats {
+  double ms_tree, ms_lists, ms_p2p, ms_p2m, ms_m2m, ms_m2l, ms_l2l, ms_l2p, ms_total;
+  int lmax, nbi_leaf, nbj_leaf, nbi_total, nbj_total;
+  int64_t p2p_pairs;     /* sum over near box pairs of n_i * n_j */
+  int64_t m2l_pairs;     /* number of (target box, source box) M2L pairs */
+  int64_t m2l_tiles;     /* M2L work tiles launched */
+} fmmx_stats;
+int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st);
+/* Enable/disable per-phase event timing (default off: one event pair). */
+int fmmx_set_timing(fmmx_handle* h, int per_phase);
+
+/* ---- introspection of the tree for parity checks (host outputs) ---- */
+/* lmax in use and box counts: what = 0 lmax, 1 boxes of side at level lev
+ * (side 0 = targets i, 1 = sources j) */
+int fmmx_get_count(fmmx_handle* h, int what, int side, int lev);
+/* sort permutation: sorted position -> caller index (sort/sorti.cxx:39-41) */
+int fmmx_get_perm(fmmx_handle* h, int side, int* perm);
+/* box keys at a level; particle ranges [first,last] (inclusive, as ndi/ndj
+ * in fmm/boxdatai.cxx:22-29) at the leaf level, else NULL */
+int fmmx_get_boxes(fmmx_handle* h, int side, int lev, int* keys, int* first, int* last);
+/* interaction list: kind 0 near (leaf level, fmm/ijbox.cxx neib 0), kind 1
+ * far at level lev (neib 1 at level 2, neib 2 above; needs
+ * FMMX_KEEP_LISTS).  nij[nb_i(lev)], neij[nb_i(lev) * 189] row-padded as
+ * neij in include/arrays.h:34. */
+int fmmx_get_list(fmmx_handle* h, int lev, int kind, int* nij, int* neij);
+/* Expansions of the last evaluate in the reference's layout (complex
+ * interleaved, p(p+1)/2 per box, unscaled): multipoles after the upward pass
+ * for all source boxes stacked leaf level first (the reference's bx rows),
+ * and local expansions at level lev after its M2L. */
+int fmmx_get_multipoles(fmmx_handle* h, double* bx_reim);
+int fmmx_get_locals(fmmx_handle* h, int lev, double* ax_reim);
+
+/* Host only (no GPU needed): the dense real operator the engine applies,
+ * (p*p) x (p*p) row-major, row = output slot, in the scaled basis of
+ * fmmx_get_multipoles' slots: multipoles M_n / rb^n, locals L_n * rb^(n+1)
+ * with real slots n(n+1)/2+m (Re) and p(p+1)/2 + n(n-1)/2 + m-1 (Im, m>0).
+ * kind 0: M2M from child octant a0 (kernel/m2m.cxx:126-154); 1: L2L to
+ * child octant a0 (kernel/l2l.cxx:203-237); 2: M2L for target-minus-source
+ * offset (a0,a1,a2) at box size rb (kernel/m2l.cxx:43-81). */
+int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, double rb, double* A);
+
+/* Last error message of the calling thread. */
+const char* fmmx_last_error(void);
+int fmmx_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

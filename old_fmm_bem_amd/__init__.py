@@ -1,0 +1,279 @@
+"""MI355X FMM engine: Python host layer over the C ABI of libfmmx.so.
+
+Mirrors the reference's FMM matvec interface (fmm/fmm.cxx:29,
+``fmm(mi, mj, neq)`` with neq 0 = G, 1 = Gni, 2 = Gnj) as an explicit object:
+
+    f = FMM(p=10, lmax=4, precision="mixed", convention="exact")
+    f.set_points(xi, xj)               # tree + lists (sorti/sortj/boxallocate/...)
+    v, g = f.evaluate(q, neq=0)        # one matvec
+
+Arrays may be numpy (host) or torch CUDA tensors (device pointers; then the
+outputs are torch tensors on the same device).  The compute always runs in
+libfmmx.so's HIP kernels; there is no CPU fallback -- a missing or unloadable
+library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["FMM", "FMMError", "lib", "operator", "G", "GNI", "GNJ", "MIXED", "FP64", "REF", "EXACT"]
+
+G, GNI, GNJ = 0, 1, 2
+MIXED, FP64 = 0, 1
+REF, EXACT = 0, 1
+NEBM = 189
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_PREC = {"mixed": MIXED, "fp32": MIXED, "fp64": FP64, MIXED: MIXED, FP64: FP64}
+_CONV = {"ref": REF, "exact": EXACT, REF: REF, EXACT: EXACT}
+
+
+class FMMError(RuntimeError):
+    pass
+
+
+class _Config(C.Structure):
+    _fields_ = [("p", C.c_int), ("lmax", C.c_int), ("precision", C.c_int), ("convention", C.c_int),
+                ("xmin", C.c_float), ("ymin", C.c_float), ("zmin", C.c_float), ("rd", C.c_float)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("ms_tree", "ms_lists", "ms_p2p", "ms_p2m", "ms_m2m", "ms_m2l",
+                                          "ms_l2l", "ms_l2p", "ms_total")] + \
+               [(n, C.c_int) for n in ("lmax", "nbi_leaf", "nbj_leaf", "nbi_total", "nbj_total")] + \
+               [(n, C.c_int64) for n in ("p2p_pairs", "m2l_pairs", "m2l_tiles")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_SYMBOLS = {
+    "fmmx_create": (C.c_int, [C.POINTER(_Config), C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "fmmx_destroy": (C.c_int, [C.c_void_p]),
+    "fmmx_set_points": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 6 + [C.c_int] + [C.c_void_p] * 6 + [C.c_int]),
+    "fmmx_evaluate": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
+    "fmmx_direct": (C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_int] + [C.c_void_p] * 11
+                    + [C.c_int]),
+    "fmmx_synchronize": (C.c_int, [C.c_void_p]),
+    "fmmx_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "fmmx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmmx_get_count": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "fmmx_get_perm": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "fmmx_get_boxes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fmmx_get_list": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "fmmx_get_multipoles": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fmmx_get_locals": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "fmmx_operator": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]),
+    "fmmx_last_error": (C.c_char_p, []),
+    "fmmx_abi_version": (C.c_int, []),
+}
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "libfmmx.so")
+
+
+def lib():
+    """Load libfmmx.so (built by old_fmm_bem_amd/build.py); raises if absent."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise FMMError(f"{path} is missing: build it with `python -m old_fmm_bem_amd.build` "
+                           "(there is no CPU fallback)")
+        L = C.CDLL(path)
+        for name, (res, args) in _SYMBOLS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise FMMError(f"libfmmx error {rc}: {lib().fmmx_last_error().decode()}")
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+class _Arr:
+    """Normalises an input column: numpy -> contiguous float32 host array,
+    torch CUDA tensor -> contiguous float32 device tensor."""
+
+    def __init__(self, a, device):
+        self.dev = device
+        if a is None:
+            self.a, self.ptr = None, None
+        elif device:
+            import torch
+            self.a = a.detach().to(torch.float32).contiguous()
+            self.ptr = self.a.data_ptr()
+        else:
+            self.a = np.ascontiguousarray(a, dtype=np.float32)
+            self.ptr = self.a.ctypes.data
+
+
+def operator(p, convention, kind, a0, a1=0, a2=0, rb=1.0):
+    """Dense real operator the engine applies (host only, no GPU): kind 0 M2M
+    (octant a0), 1 L2L (octant a0), 2 M2L (offset a0,a1,a2 at box size rb)."""
+    A = np.zeros((p * p, p * p))
+    _check(lib().fmmx_operator(p, _CONV[convention], kind, a0, a1, a2, float(rb), A.ctypes.data))
+    return A
+
+
+class FMM:
+    def __init__(self, p=10, lmax=0, precision="mixed", convention="exact", box=None, device=0, stream=None):
+        cfg = _Config(p, lmax, _PREC[precision], _CONV[convention], 0.0, 0.0, 0.0, 0.0)
+        if box is not None:
+            cfg.xmin, cfg.ymin, cfg.zmin, cfg.rd = [float(np.float32(b)) for b in box]
+        self.p, self.lmax_cfg = p, lmax
+        self.precision, self.convention = _PREC[precision], _CONV[convention]
+        h = C.c_void_p()
+        _check(lib().fmmx_create(C.byref(cfg), device, stream, C.byref(h)))
+        self.h = h
+        self.ni = self.nj = 0
+        self._device = False
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fmmx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up ---------------------------------------------------------
+    def set_points(self, xi, xj=None, ni_normals=None, nj_normals=None, keep_lists=False):
+        """Targets xi (n,3), sources xj (m,3) (default: the same points)."""
+        if xj is None:
+            xj = xi
+        dev = _is_torch(xi) and xi.is_cuda
+        self._device = dev
+        cols = [_Arr(xi[:, k], dev) for k in range(3)]
+        cols += [_Arr(None if ni_normals is None else ni_normals[:, k], dev) for k in range(3)]
+        cols += [_Arr(xj[:, k], dev) for k in range(3)]
+        cols += [_Arr(None if nj_normals is None else nj_normals[:, k], dev) for k in range(3)]
+        self._cols = cols
+        self.ni, self.nj = int(xi.shape[0]), int(xj.shape[0])
+        flags = (1 if dev else 0) | (2 if keep_lists else 0)
+        _check(lib().fmmx_set_points(self.h, self.ni, *[c.ptr for c in cols[:6]], self.nj,
+                                     *[c.ptr for c in cols[6:]], flags))
+        return self
+
+    # -- matvec ---------------------------------------------------------
+    def evaluate(self, q, neq=G, gradient=True):
+        """Potential v (ni,) and, for neq=G, gradient g (ni,3), caller order."""
+        dev = self._device
+        qa = _Arr(q, dev)
+        want_g = gradient and neq == G
+        if dev:
+            import torch
+            v = torch.empty(self.ni, dtype=torch.float64, device=q.device)
+            g = torch.empty((3, self.ni), dtype=torch.float64, device=q.device) if want_g else None
+            ptr = lambda t: t.data_ptr()  # noqa: E731
+        else:
+            v = np.empty(self.ni)
+            g = np.empty((3, self.ni)) if want_g else None
+            ptr = lambda t: t.ctypes.data  # noqa: E731
+        gp = [ptr(g[k]) for k in range(3)] if want_g else [None] * 3
+        _check(lib().fmmx_evaluate(self.h, neq, qa.ptr, ptr(v), *gp, 1 if dev else 0))
+        if want_g:
+            return v, g.T
+        return v, None
+
+    def direct(self, xi, xj, q, neq=G, ni_normals=None, nj_normals=None):
+        """GPU direct sum (FP64 accumulation) in this handle's convention."""
+        dev = _is_torch(xi) and xi.is_cuda
+        a = [_Arr(xi[:, k], dev) for k in range(3)]
+        a += [_Arr(None if ni_normals is None else ni_normals[:, k], dev) for k in range(3)]
+        b = [_Arr(xj[:, k], dev) for k in range(3)]
+        b += [_Arr(None if nj_normals is None else nj_normals[:, k], dev) for k in range(3)]
+        qa = _Arr(q, dev)
+        n = int(xi.shape[0])
+        if dev:
+            import torch
+            v = torch.zeros(n, dtype=torch.float64, device=xi.device)
+            g = torch.zeros((3, n), dtype=torch.float64, device=xi.device)
+            ptr = lambda t: t.data_ptr()  # noqa: E731
+        else:
+            v = np.zeros(n)
+            g = np.zeros((3, n))
+            ptr = lambda t: t.ctypes.data  # noqa: E731
+        _check(lib().fmmx_direct(self.h, neq, n, *[c.ptr for c in a], int(xj.shape[0]), *[c.ptr for c in b], qa.ptr,
+                                 ptr(v), ptr(g[0]), ptr(g[1]), ptr(g[2]), 1 if dev else 0))
+        return v, (g.T if neq == G else None)
+
+    def synchronize(self):
+        _check(lib().fmmx_synchronize(self.h))
+
+    def set_timing(self, per_phase=True):
+        _check(lib().fmmx_set_timing(self.h, int(per_phase)))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().fmmx_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    # -- introspection ----------------------------------------------------
+    @property
+    def lmax(self):
+        return lib().fmmx_get_count(self.h, 0, 0, 0)
+
+    def nboxes(self, side, lev):
+        return lib().fmmx_get_count(self.h, 1, 0 if side == "i" else 1, lev)
+
+    def perm(self, side):
+        n = self.ni if side == "i" else self.nj
+        a = np.zeros(n, np.int32)
+        _check(lib().fmmx_get_perm(self.h, 0 if side == "i" else 1, a.ctypes.data))
+        return a
+
+    def boxes(self, side, lev):
+        n = self.nboxes(side, lev)
+        k, f, l = (np.zeros(n, np.int32) for _ in range(3))
+        _check(lib().fmmx_get_boxes(self.h, 0 if side == "i" else 1, lev, k.ctypes.data, f.ctypes.data,
+                                    l.ctypes.data))
+        return k, f, l
+
+    def interaction_list(self, lev, kind):
+        n = self.nboxes("i", lev)
+        nij = np.zeros(n, np.int32)
+        neij = np.zeros(n * NEBM, np.int32)
+        _check(lib().fmmx_get_list(self.h, lev, kind, nij.ctypes.data, neij.ctypes.data))
+        neij = neij.reshape(n, NEBM)
+        flat = np.concatenate([neij[i, :nij[i]] for i in range(n)]) if n else np.zeros(0, np.int32)
+        return nij, flat
+
+    def multipoles(self):
+        nr = self.p * (self.p + 1) // 2
+        rows = sum(self.nboxes("j", l) for l in range(2, self.lmax + 1))
+        a = np.zeros(rows * nr * 2)
+        _check(lib().fmmx_get_multipoles(self.h, a.ctypes.data))
+        return a.view(np.complex128).reshape(rows, nr)
+
+    def locals(self, lev):
+        nr = self.p * (self.p + 1) // 2
+        a = np.zeros(self.nboxes("i", lev) * nr * 2)
+        _check(lib().fmmx_get_locals(self.h, lev, a.ctypes.data))
+        return a.view(np.complex128).reshape(-1, nr)
+
+
+def matvec(xi, q, xj=None, neq=G, **kw):
+    """One-shot: tree set-up and one evaluation."""
+    f = FMM(**kw)
+    try:
+        f.set_points(xi, xj)
+        return f.evaluate(q, neq=neq)
+    finally:
+        f.close()

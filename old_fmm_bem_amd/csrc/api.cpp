@@ -1,0 +1,664 @@
+// api.cpp -- handle lifetime, set-up, the matvec sequence of fmm/fmm.cxx:29-152
+// on device-resident data, and the extern "C" entry points of fmmx.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <tuple>
+
+#include "engine.h"
+#include "tree_util.h"
+
+namespace fmmx {
+
+namespace {
+
+constexpr int TPB = 256;
+inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
+
+__global__ void k_gather_q(int n, const uint32_t* __restrict__ perm, const float* __restrict__ q,
+                           float* __restrict__ qs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) qs[i] = q[perm[i]];
+}
+
+__global__ void k_scatter(int n, const uint32_t* __restrict__ perm, const double* __restrict__ s,
+                          double* __restrict__ d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[perm[i]] = s[i];
+}
+
+__global__ void k_zero_f64(long n, double* __restrict__ a) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// order-preserving float <-> int map for atomic min/max
+__device__ inline int f2o(float f) {
+  int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__global__ void k_init_mm(int* mm) {
+  if (threadIdx.x < 3) mm[threadIdx.x] = 0x7fffffff;
+  else if (threadIdx.x < 6) mm[threadIdx.x] = (int)0x80000000;
+}
+__global__ void k_minmax(int n, const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                         int* mm) {
+  __shared__ int s[6];
+  if (threadIdx.x < 6) s[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : (int)0x80000000;
+  __syncthreads();
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    int a = f2o(x[i]), b = f2o(y[i]), c = f2o(z[i]);
+    atomicMin(&s[0], a); atomicMin(&s[1], b); atomicMin(&s[2], c);
+    atomicMax(&s[3], a); atomicMax(&s[4], b); atomicMax(&s[5], c);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) atomicMin(&mm[threadIdx.x], s[threadIdx.x]);
+  else if (threadIdx.x < 6) atomicMax(&mm[threadIdx.x], s[threadIdx.x]);
+}
+inline float o2f(int o) {
+  int i = o >= 0 ? o : o ^ 0x7fffffff;
+  float f;
+  std::memcpy(&f, &i, 4);
+  return f;
+}
+
+// fmm/nlevel.cxx:8-26 ("gpu p=5" switch points)
+int nlevel(int nmax) {
+  const float sw[6] = {1e5f, 5e5f, 3e6f, 2e7f, 1.5e8f, 1.3e9f};
+  int l = 1;
+  for (int k = 0; k < 6; k++)
+    if ((float)nmax < sw[k]) return l + k + 1;
+  return l + 7;
+}
+
+// process-wide cache of host operators keyed by (p, conv, lmax, rd)
+struct OpSet {
+  std::vector<double> m2m, l2l, m2l;  // transposed [op][k][row]
+  int nlev = 1;
+};
+std::mutex g_cache_mu;
+std::map<std::tuple<int, int, int, float>, std::shared_ptr<OpSet>> g_cache;
+std::map<std::pair<int, int>, std::shared_ptr<Tables>> g_tables;
+
+}  // namespace
+
+std::shared_ptr<Tables> get_tables(int p, int conv) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto& t = g_tables[{p, conv}];
+  if (!t) {
+    t = std::make_shared<Tables>();
+    build_tables(*t, p, conv);
+  }
+  return t;
+}
+
+namespace {
+
+void transpose_into(const double* A, int P2, double* dst) {
+  for (int r = 0; r < P2; r++)
+    for (int k = 0; k < P2; k++) dst[(size_t)k * P2 + r] = A[(size_t)r * P2 + k];
+}
+
+std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
+  const int P2 = T.p * T.p;
+  const bool per_level = T.conv == FMMX_CONV_REF;
+  auto key = std::make_tuple(T.p, T.conv, per_level ? lmax : 0, per_level ? rd : 0.f);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) return it->second;
+  }
+  auto S = std::make_shared<OpSet>();
+  const size_t sq = (size_t)P2 * P2;
+  S->m2m.resize(8 * sq);
+  S->l2l.resize(8 * sq);
+  std::vector<double> A(sq);
+  for (int o = 0; o < 8; o++) {
+    op_m2m(T, o, A.data());
+    transpose_into(A.data(), P2, &S->m2m[o * sq]);
+    op_l2l(T, o, A.data());
+    transpose_into(A.data(), P2, &S->l2l[o * sq]);
+  }
+  S->nlev = per_level ? lmax - 1 : 1;
+  S->m2l.resize((size_t)S->nlev * NOFF * sq);
+  const int njobs = S->nlev * NOFF;
+  // parallel over (level, offset)
+  std::vector<std::thread> th;
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      std::vector<double> B(sq);
+      for (int j = (int)t; j < njobs; j += (int)nt) {
+        int li = j / NOFF, oi = j % NOFF, d[3];
+        offset_vector(oi, d);
+        double rbv = per_level ? (double)rd / (double)(1 << (li + 2)) : 1.0;
+        op_m2l(T, d[0], d[1], d[2], rbv, B.data());
+        transpose_into(B.data(), P2, &S->m2l[((size_t)li * NOFF + oi) * sq]);
+      }
+    });
+  for (auto& x : th) x.join();
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_cache[key] = S;
+  return S;
+}
+
+template <typename T>
+int upload(DevBuf& dst, const std::vector<double>& src, hipStream_t s) {
+  int rc = dst.ensure(src.size() * sizeof(T));
+  if (rc) return rc;
+  std::vector<T> tmp(src.begin(), src.end());
+  FMMX_HIP(hipMemcpyAsync(dst.p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  FMMX_HIP(hipStreamSynchronize(s));
+  return FMMX_OK;
+}
+
+}  // namespace
+
+Engine::~Engine() {
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+int Engine::init(const fmmx_config* c, int dev, void* strm) {
+  if (!c) return fail(FMMX_EINVAL, "config is NULL");
+  if (c->p < 2 || c->p > MAXP) return fail(FMMX_EINVAL, "p out of range [2, FMMX_MAX_P]");
+  if (c->lmax != 0 && (c->lmax < 2 || c->lmax > MAXL)) return fail(FMMX_EINVAL, "lmax out of range [2, 8]");
+  if (c->precision != FMMX_MIXED && c->precision != FMMX_FP64) return fail(FMMX_EINVAL, "bad precision");
+  if (c->convention != FMMX_CONV_REF && c->convention != FMMX_CONV_EXACT) return fail(FMMX_EINVAL, "bad convention");
+  if (c->rd > 0 && !(std::isfinite(c->xmin) && std::isfinite(c->ymin) && std::isfinite(c->zmin) && std::isfinite(c->rd)))
+    return fail(FMMX_EINVAL, "root cube not finite");
+  cfg = *c;
+  device = dev;
+  FMMX_HIP(hipSetDevice(dev));
+  stream = (hipStream_t)strm;
+  p = c->p;
+  P2 = p * p;
+  fp64 = c->precision == FMMX_FP64;
+  user_box = c->rd > 0;
+  for (auto& e : ev) FMMX_HIP(hipEventCreate(&e));
+  tab = *get_tables(p, c->convention);
+  // device tables: table-ratio corrections and the M2L offset table
+  std::vector<double> corr(tab.corr);
+  int rc = corr_d.ensure(corr.size() * 8);
+  if (rc) return rc;
+  FMMX_HIP(hipMemcpy(corr_d.p, corr.data(), corr.size() * 8, hipMemcpyHostToDevice));
+  std::vector<int> ot(8 * NEBM * 4);
+  for (int cls = 0; cls < 8; cls++) {
+    int px = cls & 1, py = (cls >> 1) & 1, pz = (cls >> 2) & 1, o = 0;
+    for (int dx = px - 3; dx <= px + 2; dx++)
+      for (int dy = py - 3; dy <= py + 2; dy++)
+        for (int dz = pz - 3; dz <= pz + 2; dz++) {
+          if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1) continue;
+          int* e = &ot[(cls * NEBM + o) * 4];
+          e[0] = dx; e[1] = dy; e[2] = dz; e[3] = offset_index(dx, dy, dz);
+          o++;
+        }
+  }
+  if ((rc = offtab.ensure(ot.size() * 4))) return rc;
+  FMMX_HIP(hipMemcpy(offtab.p, ot.data(), ot.size() * 4, hipMemcpyHostToDevice));
+  return FMMX_OK;
+}
+
+int Engine::upload_operators() {
+  if (ops_lmax == lmax && ops_rd == rd) return FMMX_OK;
+  auto S = get_ops(tab, lmax, rd);
+  m2l_per_level = S->nlev > 1;
+  int rc;
+  if (fp64) {
+    if ((rc = upload<double>(op_m2m, S->m2m, stream)) || (rc = upload<double>(op_l2l, S->l2l, stream)) ||
+        (rc = upload<double>(op_m2l, S->m2l, stream)))
+      return rc;
+  } else {
+    if ((rc = upload<float>(op_m2m, S->m2m, stream)) || (rc = upload<float>(op_l2l, S->l2l, stream)) ||
+        (rc = upload<float>(op_m2l, S->m2l, stream)))
+      return rc;
+  }
+  ops_lmax = lmax;
+  ops_rd = rd;
+  return FMMX_OK;
+}
+
+static const float* stage(DevBuf& b, const float* src, int n, bool dev, hipStream_t s, int& rc) {
+  if (!src || dev) return src;
+  rc = b.ensure((size_t)(n > 0 ? n : 1) * 4);
+  if (rc) return nullptr;
+  if (n > 0 && hipMemcpyAsync(b.p, src, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess) {
+    rc = fail(FMMX_EDEVICE, "hipMemcpyAsync H2D failed");
+    return nullptr;
+  }
+  return b.as<float>();
+}
+
+int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi, const float* nxi, const float* nyi,
+                       const float* nzi, int nj, const float* xj, const float* yj, const float* zj, const float* nxj,
+                       const float* nyj, const float* nzj, int flags) {
+  if (ni < 0 || nj < 0) return fail(FMMX_EINVAL, "negative particle count");
+  if ((ni > 0 && (!xi || !yi || !zi)) || (nj > 0 && (!xj || !yj || !zj)))
+    return fail(FMMX_EINVAL, "coordinate pointer is NULL");
+  FMMX_HIP(hipSetDevice(device));
+  const bool dev = flags & FMMX_DEVICE_PTRS;
+  keep_lists = flags & FMMX_KEEP_LISTS;
+  int rc = FMMX_OK;
+  const float* a[14] = {xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj, nullptr, nullptr};
+  const float* d[14];
+  for (int k = 0; k < 12; k++) {
+    d[k] = stage(in[k], a[k], k < 6 ? ni : nj, dev, stream, rc);
+    if (rc) return rc;
+  }
+  have_points = false;
+  FMMX_HIP(hipEventRecord(ev[0], stream));
+  // root cube
+  if (user_box) {
+    xmin = cfg.xmin; ymin = cfg.ymin; zmin = cfg.zmin; rd = cfg.rd;
+  } else {
+    DevBuf& mm = si.tmp;
+    if ((rc = mm.ensure(64))) return rc;
+    k_init_mm<<<1, 64, 0, stream>>>(mm.as<int>());
+    if (ni > 0) k_minmax<<<nblk(ni), TPB, 0, stream>>>(ni, d[0], d[1], d[2], mm.as<int>());
+    if (nj > 0) k_minmax<<<nblk(nj), TPB, 0, stream>>>(nj, d[6], d[7], d[8], mm.as<int>());
+    int h[6];
+    FMMX_HIP(hipMemcpyAsync(h, mm.p, 24, hipMemcpyDeviceToHost, stream));
+    FMMX_HIP(hipStreamSynchronize(stream));
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+      lo[k] = o2f(h[k]);
+      hi[k] = o2f(h[3 + k]);
+    }
+    if (ni + nj == 0) lo[0] = lo[1] = lo[2] = 0, hi[0] = hi[1] = hi[2] = 1;
+    // bio/main.cxx:373-380: largest extent times (1+1e-5), in float
+    float r = 0;
+    for (int k = 0; k < 3; k++) {
+      if (!std::isfinite(lo[k]) || !std::isfinite(hi[k])) return fail(FMMX_EINVAL, "non-finite coordinates");
+      r = std::max(r, (float)((hi[k] - lo[k]) * (float)(1 + 1e-5)));
+    }
+    if (!(r > 0)) r = 1.0f;
+    xmin = lo[0]; ymin = lo[1]; zmin = lo[2]; rd = r;
+  }
+  lmax = cfg.lmax > 0 ? cfg.lmax : std::min(MAXL, nlevel(nj));
+  if ((rc = upload_operators())) return rc;
+  // tree of both sides (keys, sort, gathers, per-level boxes)
+  if ((rc = build_side(si, ni, d[0], d[1], d[2], d[3], d[4], d[5], false))) return rc;
+  if ((rc = build_side(sj, nj, d[6], d[7], d[8], d[9], d[10], d[11], true))) return rc;
+  int hc[2][MAXL + 1] = {};
+  if (ni > 0) FMMX_HIP(hipMemcpyAsync(hc[0], si.cnt.p, (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
+  if (nj > 0) FMMX_HIP(hipMemcpyAsync(hc[1], sj.cnt.p, (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
+  FMMX_HIP(hipStreamSynchronize(stream));
+  for (int l = 2; l <= lmax; l++) {
+    si.nb[l] = ni > 0 ? hc[0][l] : 0;
+    sj.nb[l] = nj > 0 ? hc[1][l] : 0;
+  }
+  if ((rc = build_lists())) return rc;
+  if ((rc = build_m2l_order())) return rc;
+  if (keep_lists && (rc = build_far_lists())) return rc;
+  FMMX_HIP(hipEventRecord(ev[1], stream));
+  unsigned long long pc = 0;
+  if (si.nb[lmax] > 0) FMMX_HIP(hipMemcpyAsync(&pc, pair_ctr.p, 8, hipMemcpyDeviceToHost, stream));
+  FMMX_HIP(hipStreamSynchronize(stream));
+  p2p_pairs = (int64_t)pc;
+  float ms = 0;
+  FMMX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  stats.ms_tree = ms;
+  have_points = true;
+  last_neq = -1;
+  return FMMX_OK;
+}
+
+int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags) {
+  if (!have_points) return fail(FMMX_ESTATE, "evaluate before set_points");
+  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ) return fail(FMMX_EINVAL, "neq must be 0, 1 or 2");
+  if (neq == FMMX_GNI && si.n > 0 && !si.has_normals) return fail(FMMX_EINVAL, "Gni needs target normals");
+  if (neq == FMMX_GNJ && sj.n > 0 && !sj.has_normals) return fail(FMMX_EINVAL, "Gnj needs source normals");
+  if ((sj.n > 0 && !qj) || (si.n > 0 && !vi)) return fail(FMMX_EINVAL, "q or v pointer is NULL");
+  FMMX_HIP(hipSetDevice(device));
+  const bool dev = flags & FMMX_DEVICE_PTRS;
+  const int ni = si.n, nj = sj.n;
+  const size_t es = fp64 ? 8 : 4;
+  int rc = FMMX_OK;
+  size_t ni1 = (size_t)(ni > 0 ? ni : 1);
+  if ((rc = qs.ensure((size_t)(nj > 0 ? nj : 1) * 4)) || (rc = vs.ensure(ni1 * 8)) || (rc = gxs.ensure(ni1 * 8)) ||
+      (rc = gys.ensure(ni1 * 8)) || (rc = gzs.ensure(ni1 * 8)))
+    return rc;
+  for (int l = 2; l <= lmax; l++) {
+    if ((rc = mexp[l].ensure((size_t)std::max(1, sj.nb[l]) * P2 * es)) ||
+        (rc = lexp[l].ensure((size_t)std::max(1, si.nb[l]) * P2 * es)))
+      return rc;
+  }
+  const float* q = stage(qin, qj, nj, dev, stream, rc);
+  if (rc) return rc;
+  int e = 2;
+  auto mark = [&]() -> int {
+    if (per_phase) FMMX_HIP(hipEventRecord(ev[e++], stream));
+    return FMMX_OK;
+  };
+  FMMX_HIP(hipEventRecord(ev[2], stream));
+  e = 3;
+  k_zero_f64<<<1, 64, 0, stream>>>(1, pair_ctr.as<double>() + 1);  // M2L pair counter
+  if (nj > 0) k_gather_q<<<nblk(nj), TPB, 0, stream>>>(nj, sj.perm.as<uint32_t>(), q, qs.as<float>());
+  if (ni > 0) {
+    if ((rc = run_p2p(neq))) return rc;
+    if ((rc = mark())) return rc;  // ev3: after P2P
+    if ((rc = run_p2m(neq))) return rc;
+    if ((rc = mark())) return rc;  // ev4: after P2M
+    for (int l = lmax - 1; l >= 2; l--)
+      if ((rc = run_m2m(l))) return rc;
+    if ((rc = mark())) return rc;  // ev5: after M2M
+    if ((rc = run_m2l(2, true))) return rc;
+    for (int l = 3; l <= lmax; l++) {
+      if ((rc = run_l2l(l))) return rc;
+      if ((rc = run_m2l(l, false))) return rc;
+    }
+    if ((rc = mark())) return rc;  // ev6: after M2L+L2L
+    if ((rc = run_l2p(neq))) return rc;
+    if ((rc = mark())) return rc;  // ev7: after L2P
+  }
+  // outputs in caller order
+  double* outs[4] = {vi, neq == FMMX_G ? gxi : nullptr, neq == FMMX_G ? gyi : nullptr, neq == FMMX_G ? gzi : nullptr};
+  double* srcs[4] = {vs.as<double>(), gxs.as<double>(), gys.as<double>(), gzs.as<double>()};
+  for (int k = 0; k < 4; k++) {
+    if (!outs[k] || ni == 0) continue;
+    double* dst = outs[k];
+    if (!dev) {
+      if ((rc = out[k].ensure(ni1 * 8))) return rc;
+      dst = out[k].as<double>();
+    }
+    k_scatter<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), srcs[k], dst);
+    if (!dev) FMMX_HIP(hipMemcpyAsync(outs[k], dst, (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
+  }
+  FMMX_HIP(hipEventRecord(ev[15], stream));
+  if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
+  last_neq = neq;
+  return FMMX_OK;
+}
+
+int Engine::direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
+                   const float* nyi, const float* nzi, int nj, const float* xj, const float* yj, const float* zj,
+                   const float* nxj, const float* nyj, const float* nzj, const float* qj, double* vi, double* gxi,
+                   double* gyi, double* gzi, int flags) {
+  if (ni < 0 || nj < 0) return fail(FMMX_EINVAL, "negative particle count");
+  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ) return fail(FMMX_EINVAL, "neq must be 0, 1 or 2");
+  if (neq == FMMX_GNI && ni > 0 && (!nxi || !nyi || !nzi)) return fail(FMMX_EINVAL, "Gni needs target normals");
+  if (neq == FMMX_GNJ && nj > 0 && (!nxj || !nyj || !nzj)) return fail(FMMX_EINVAL, "Gnj needs source normals");
+  if (ni > 0 && (!xi || !yi || !zi || !vi)) return fail(FMMX_EINVAL, "NULL target pointer");
+  if (nj > 0 && (!xj || !yj || !zj || !qj)) return fail(FMMX_EINVAL, "NULL source pointer");
+  if (neq == FMMX_G && ni > 0 && (!gxi || !gyi || !gzi)) return fail(FMMX_EINVAL, "G needs gradient outputs");
+  FMMX_HIP(hipSetDevice(device));
+  const bool dev = flags & FMMX_DEVICE_PTRS;
+  int rc = FMMX_OK;
+  const float* a[13] = {xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj, qj};
+  const float* d[13];
+  for (int k = 0; k < 13; k++) {
+    d[k] = stage(k < 12 ? in[k] : qin, a[k], k < 6 ? ni : nj, dev, stream, rc);
+    if (rc) return rc;
+  }
+  double* o[4] = {vi, gxi, gyi, gzi};
+  double* od[4];
+  for (int k = 0; k < 4; k++) {
+    od[k] = o[k];
+    if (!dev && o[k]) {
+      if ((rc = out[k].ensure((size_t)std::max(1, ni) * 8))) return rc;
+      od[k] = out[k].as<double>();
+    }
+  }
+  if ((rc = run_direct(neq, ni, d[0], d[1], d[2], d[3], d[4], d[5], nj, d[6], d[7], d[8], d[9], d[10], d[11], d[12],
+                       od[0], od[1], od[2], od[3])))
+    return rc;
+  if (!dev) {
+    for (int k = 0; k < 4; k++)
+      if (o[k] && ni > 0 && (k == 0 || neq == FMMX_G))
+        FMMX_HIP(hipMemcpyAsync(o[k], od[k], (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
+    FMMX_HIP(hipStreamSynchronize(stream));
+  }
+  return FMMX_OK;
+}
+
+}  // namespace fmmx
+
+using fmmx::Engine;
+
+struct fmmx_handle {
+  Engine e;
+};
+
+#define FMMX_GUARD(body)                                              \
+  try {                                                               \
+    body                                                              \
+  } catch (const std::bad_alloc&) {                                   \
+    return fmmx::fail(FMMX_ENOMEM, "host allocation failed");         \
+  } catch (const std::exception& ex) {                                \
+    return fmmx::fail(FMMX_EDEVICE, ex.what());                       \
+  }
+
+extern "C" {
+
+int fmmx_abi_version(void) { return FMMX_ABI_VERSION; }
+
+int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, double rb, double* A) {
+  FMMX_GUARD({
+    if (p < 2 || p > fmmx::MAXP || !A) return fmmx::fail(FMMX_EINVAL, "bad p or NULL output");
+    if (convention != FMMX_CONV_REF && convention != FMMX_CONV_EXACT) return fmmx::fail(FMMX_EINVAL, "bad convention");
+    auto T = fmmx::get_tables(p, convention);
+    if (kind == 0 || kind == 1) {
+      if (a0 < 0 || a0 > 7) return fmmx::fail(FMMX_EINVAL, "octant out of range");
+      if (kind == 0) fmmx::op_m2m(*T, a0, A);
+      else fmmx::op_l2l(*T, a0, A);
+      return FMMX_OK;
+    }
+    if (kind == 2) {
+      if (fmmx::offset_index(a0, a1, a2) < 0 || !(rb > 0)) return fmmx::fail(FMMX_EINVAL, "bad M2L offset or rb");
+      fmmx::op_m2l(*T, a0, a1, a2, rb, A);
+      return FMMX_OK;
+    }
+    return fmmx::fail(FMMX_EINVAL, "bad kind");
+  })
+}
+
+const char* fmmx_last_error(void) { return fmmx::last_error().c_str(); }
+
+int fmmx_create(const fmmx_config* cfg, int device, void* stream, fmmx_handle** out) {
+  FMMX_GUARD({
+    if (!out) return fmmx::fail(FMMX_EINVAL, "out is NULL");
+    *out = nullptr;
+    auto* h = new fmmx_handle();
+    int rc = h->e.init(cfg, device, stream);
+    if (rc) {
+      delete h;
+      return rc;
+    }
+    *out = h;
+    return FMMX_OK;
+  })
+}
+
+int fmmx_destroy(fmmx_handle* h) {
+  FMMX_GUARD({
+    if (!h) return FMMX_OK;
+    (void)hipSetDevice(h->e.device);
+    (void)hipStreamSynchronize(h->e.stream);
+    delete h;
+    return FMMX_OK;
+  })
+}
+
+int fmmx_set_points(fmmx_handle* h, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
+                    const float* nyi, const float* nzi, int nj, const float* xj, const float* yj, const float* zj,
+                    const float* nxj, const float* nyj, const float* nzj, int flags) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.set_points(ni, xi, yi, zi, nxi, nyi, nzi, nj, xj, yj, zj, nxj, nyj, nzj, flags);
+  })
+}
+
+int fmmx_evaluate(fmmx_handle* h, int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi,
+                  int flags) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.evaluate(neq, qj, vi, gxi, gyi, gzi, flags);
+  })
+}
+
+int fmmx_direct(fmmx_handle* h, int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
+                const float* nyi, const float* nzi, int nj, const float* xj, const float* yj, const float* zj,
+                const float* nxj, const float* nyj, const float* nzj, const float* qj, double* vi, double* gxi,
+                double* gyi, double* gzi, int flags) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.direct(neq, ni, xi, yi, zi, nxi, nyi, nzi, nj, xj, yj, zj, nxj, nyj, nzj, qj, vi, gxi, gyi, gzi,
+                       flags);
+  })
+}
+
+int fmmx_synchronize(fmmx_handle* h) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+  if (hipStreamSynchronize(h->e.stream) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "stream synchronize failed");
+  return FMMX_OK;
+}
+
+int fmmx_set_timing(fmmx_handle* h, int per_phase) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+  h->e.per_phase = per_phase != 0;
+  return FMMX_OK;
+}
+
+int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st) {
+  if (!h || !st) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  Engine& e = h->e;
+  if (hipEventSynchronize(e.ev[15]) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "event synchronize failed");
+  fmmx_stats s = e.stats;
+  float ms = 0;
+  if (e.last_neq >= 0 && hipEventElapsedTime(&ms, e.ev[2], e.ev[15]) == hipSuccess) s.ms_total = ms;
+  if (e.per_phase && e.last_neq >= 0 && e.si.n > 0) {
+    double* dst[5] = {&s.ms_p2p, &s.ms_p2m, &s.ms_m2m, &s.ms_m2l, &s.ms_l2p};
+    for (int k = 0; k < 5; k++)
+      if (hipEventElapsedTime(&ms, e.ev[2 + k], e.ev[3 + k]) == hipSuccess) *dst[k] = ms;
+    s.ms_l2l = 0;  // interleaved with M2L, counted there
+  }
+  s.lmax = e.lmax;
+  s.nbi_leaf = e.si.nb[e.lmax];
+  s.nbj_leaf = e.sj.nb[e.lmax];
+  s.nbi_total = s.nbj_total = 0;
+  for (int l = 2; l <= e.lmax; l++) {
+    s.nbi_total += e.si.nb[l];
+    s.nbj_total += e.sj.nb[l];
+  }
+  s.p2p_pairs = e.p2p_pairs;
+  unsigned long long mp = 0;
+  if (e.pair_ctr.p) (void)hipMemcpy(&mp, e.pair_ctr.as<unsigned long long>() + 1, 8, hipMemcpyDeviceToHost);
+  s.m2l_pairs = (int64_t)mp;
+  *st = s;
+  return FMMX_OK;
+}
+
+int fmmx_get_count(fmmx_handle* h, int what, int side, int lev) {
+  if (!h) return -1;
+  Engine& e = h->e;
+  if (what == 0) return e.lmax;
+  if (lev < 2 || lev > e.lmax) return -1;
+  return side == 0 ? e.si.nb[lev] : e.sj.nb[lev];
+}
+
+int fmmx_get_perm(fmmx_handle* h, int side, int* perm) {
+  if (!h || !perm) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  fmmx::Side& s = side == 0 ? h->e.si : h->e.sj;
+  if (s.n > 0 && hipMemcpy(perm, s.perm.p, (size_t)s.n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return fmmx::fail(FMMX_EDEVICE, "copy failed");
+  return FMMX_OK;
+}
+
+int fmmx_get_boxes(fmmx_handle* h, int side, int lev, int* keys, int* first, int* last) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+  Engine& e = h->e;
+  if (lev < 2 || lev > e.lmax) return fmmx::fail(FMMX_EINVAL, "level out of range");
+  fmmx::Side& s = side == 0 ? e.si : e.sj;
+  size_t nb = (size_t)s.nb[lev];
+  if (nb == 0) return FMMX_OK;
+  if (keys && hipMemcpy(keys, s.bkey[lev].p, nb * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return fmmx::fail(FMMX_EDEVICE, "copy failed");
+  if (lev == e.lmax) {
+    if (first && hipMemcpy(first, s.bfirst[lev].p, nb * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return fmmx::fail(FMMX_EDEVICE, "copy failed");
+    if (last && hipMemcpy(last, s.blast.p, nb * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return fmmx::fail(FMMX_EDEVICE, "copy failed");
+  }
+  return FMMX_OK;
+}
+
+int fmmx_get_list(fmmx_handle* h, int lev, int kind, int* nij_out, int* neij_out) {
+  if (!h || !nij_out || !neij_out) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  Engine& e = h->e;
+  if (lev < 2 || lev > e.lmax) return fmmx::fail(FMMX_EINVAL, "level out of range");
+  size_t nb = (size_t)e.si.nb[lev];
+  if (kind == 0) {
+    if (lev != e.lmax) return fmmx::fail(FMMX_EINVAL, "near list exists at the leaf level only");
+    if (nb == 0) return FMMX_OK;
+    std::vector<int> rows(nb * fmmx::NNEAR);
+    if (hipMemcpy(nij_out, e.nij.p, nb * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(rows.data(), e.neij.p, rows.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return fmmx::fail(FMMX_EDEVICE, "copy failed");
+    for (size_t b = 0; b < nb; b++)
+      for (int c = 0; c < fmmx::NEBM; c++) neij_out[b * fmmx::NEBM + c] = c < nij_out[b] ? rows[b * fmmx::NNEAR + c] : -1;
+    return FMMX_OK;
+  }
+  if (!e.keep_lists) return fmmx::fail(FMMX_ESTATE, "far lists need FMMX_KEEP_LISTS at set_points");
+  if (nb == 0) return FMMX_OK;
+  if (hipMemcpy(nij_out, e.fnij[lev].p, nb * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(neij_out, e.fneij[lev].p, nb * fmmx::NEBM * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return fmmx::fail(FMMX_EDEVICE, "copy failed");
+  return FMMX_OK;
+}
+
+static int copy_expansion(Engine& e, fmmx::DevBuf& b, int nb, int lev, bool local, double* out) {
+  const int p = e.p, P2 = e.P2, nr = fmmx::nre(p);
+  std::vector<double> v((size_t)nb * P2);
+  if (nb == 0) return FMMX_OK;
+  if (e.fp64) {
+    if (hipMemcpy(v.data(), b.p, v.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      return fmmx::fail(FMMX_EDEVICE, "copy failed");
+  } else {
+    std::vector<float> f(v.size());
+    if (hipMemcpy(f.data(), b.p, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return fmmx::fail(FMMX_EDEVICE, "copy failed");
+    std::copy(f.begin(), f.end(), v.begin());
+  }
+  const double rb = e.rb(lev);
+  for (int i = 0; i < nb; i++)
+    for (int n = 0; n < p; n++) {
+      const double s = local ? 1.0 / std::pow(rb, n + 1) : std::pow(rb, n);
+      for (int m = 0; m <= n; m++) {
+        const double re = v[(size_t)i * P2 + fmmx::slot_re(n, m)];
+        const double im = m ? v[(size_t)i * P2 + fmmx::slot_im(p, n, m)] : 0.0;
+        out[2 * ((size_t)i * nr + fmmx::slot_re(n, m))] = s * re;
+        out[2 * ((size_t)i * nr + fmmx::slot_re(n, m)) + 1] = s * im;
+      }
+    }
+  return FMMX_OK;
+}
+
+int fmmx_get_multipoles(fmmx_handle* h, double* bx) {
+  if (!h || !bx) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  Engine& e = h->e;
+  if (e.last_neq < 0) return fmmx::fail(FMMX_ESTATE, "no evaluation yet");
+  if (hipStreamSynchronize(e.stream) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "sync failed");
+  size_t row = 0;
+  const int nr = fmmx::nre(e.p);
+  for (int l = e.lmax; l >= 2; l--) {
+    int rc = copy_expansion(e, e.mexp[l], e.sj.nb[l], l, false, bx + 2 * row * nr);
+    if (rc) return rc;
+    row += e.sj.nb[l];
+  }
+  return FMMX_OK;
+}
+
+int fmmx_get_locals(fmmx_handle* h, int lev, double* ax) {
+  if (!h || !ax) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  Engine& e = h->e;
+  if (e.last_neq < 0) return fmmx::fail(FMMX_ESTATE, "no evaluation yet");
+  if (lev < 2 || lev > e.lmax) return fmmx::fail(FMMX_EINVAL, "level out of range");
+  if (hipStreamSynchronize(e.stream) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "sync failed");
+  return copy_expansion(e, e.lexp[lev], e.si.nb[lev], lev, true, ax);
+}
+
+}  // extern "C"

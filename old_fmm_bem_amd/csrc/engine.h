@@ -1,0 +1,131 @@
+// engine.h -- device-resident state of one FMM handle and the launchers
+// that operate on it.
+#pragma once
+
+#include <array>
+#include <memory>
+
+#include "fmmx_internal.h"
+
+namespace fmmx {
+
+constexpr int MAXL = 8;  // deepest supported leaf level (keys of 3*8 bits)
+
+#define FMMX_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(FMMX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+// Grow-only device allocation owned by a handle.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int ensure(size_t need) {
+    if (need <= bytes && p) return FMMX_OK;
+    if (p) {
+      // growing: earlier asynchronous work may still read the old block
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
+    p = nullptr;
+    bytes = 0;
+    size_t b = need ? need : 16;
+    if (hipMalloc(&p, b) != hipSuccess) return fail(FMMX_ENOMEM, "hipMalloc failed");
+    bytes = b;
+    return FMMX_OK;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// One particle set (targets i or sources j) in Morton order, with its boxes
+// at every level 2..lmax.
+struct Side {
+  int n = 0;
+  bool has_normals = false;
+  DevBuf keys_in, idx_in, keys, perm, tmp;
+  DevBuf x, y, z, nx, ny, nz;
+  std::array<DevBuf, MAXL + 1> flag, pos, bkey, bfirst, parent, cfirst, dense;
+  DevBuf blast, cnt;
+  std::array<int, MAXL + 1> nb{};  // host copy of box counts per level
+};
+
+struct Engine {
+  fmmx_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int p = 0, P2 = 0, lmax = 0;
+  bool fp64 = false;
+  float xmin = 0, ymin = 0, zmin = 0, rd = 0;
+  bool have_points = false, keep_lists = false, user_box = false;
+  Tables tab;
+
+  Side si, sj;
+  DevBuf scratch;  // rocPRIM temporary storage
+  // near list at the leaf level (targets): counts and rows of NNEAR
+  DevBuf nij, neij, pair_ctr;
+  // far lists per level (only with FMMX_KEEP_LISTS): rows of NEBM
+  std::array<DevBuf, MAXL + 1> fnij, fneij;
+  // M2L target ordering by parity class and class counts per level
+  std::array<DevBuf, MAXL + 1> cls_key, cls_key2, cls_ord, cls_ord2, cls_cnt;
+  // operators (precision of the evaluation): M2M[8], L2L[8], M2L[levels][NOFF]
+  DevBuf op_m2m, op_l2l, op_m2l;
+  bool m2l_per_level = false;
+  DevBuf offtab;  // int4 (dx,dy,dz,opidx) per (class, 189)
+  DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)
+  int ops_lmax = -1;
+  float ops_rd = -1;
+  // caller-order staging for host pointers
+  DevBuf in[14], qin, out[4];
+  // evaluation buffers
+  DevBuf qs, vs, gxs, gys, gzs;
+  std::array<DevBuf, MAXL + 1> mexp, lexp;
+  int last_neq = -1;
+  // timing
+  bool per_phase = false;
+  hipEvent_t ev[16] = {};
+  fmmx_stats stats{};
+  int64_t p2p_pairs = 0, m2l_pairs = 0;
+
+  ~Engine();
+  int init(const fmmx_config* c, int dev, void* stream);
+  int set_points(int ni, const float* xi, const float* yi, const float* zi, const float* nxi, const float* nyi,
+                 const float* nzi, int nj, const float* xj, const float* yj, const float* zj, const float* nxj,
+                 const float* nyj, const float* nzj, int flags);
+  int evaluate(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags);
+  int direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
+             const float* nyi, const float* nzi, int nj, const float* xj, const float* yj, const float* zj,
+             const float* nxj, const float* nyj, const float* nzj, const float* qj, double* vi, double* gxi,
+             double* gyi, double* gzi, int flags);
+
+  // tree.hip
+  int build_side(Side& s, int n, const float* x, const float* y, const float* z, const float* nx,
+                 const float* ny, const float* nz, bool dense);
+  int build_lists();
+  int build_far_lists();
+  int build_m2l_order();
+  // kernels.hip
+  int run_p2p(int neq);
+  int run_p2m(int neq);
+  int run_m2m(int lev);
+  int run_m2l(int lev, bool init);
+  int run_l2l(int lev);
+  int run_l2p(int neq);
+  int run_direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
+                 const float* nyi, const float* nzi, int nj, const float* xj, const float* yj,
+                 const float* zj, const float* nxj, const float* nyj, const float* nzj, const float* qj,
+                 double* vi, double* gxi, double* gyi, double* gzi);
+  int upload_operators();
+  double c4() const { return cfg.convention == FMMX_CONV_REF ? 0.25 / (double)PIF : 0.25 / M_PI; }
+  double rb(int lev) const { return (double)rd / (double)(1 << lev); }
+};
+
+}  // namespace fmmx

@@ -1,0 +1,836 @@
+// kernels.hip -- FMM phase kernels for CDNA4 (gfx950), wave64.
+//
+//   P2P  kernel/G_p2p.cxx, Gni_p2p.cxx, Gnj_p2p.cxx: one workgroup per target
+//        leaf box, source particles of the near boxes staged through LDS,
+//        FP32 (or FP64) pair math, FP64 per-target sums.
+//   P2M  kernel/G_p2m.cxx, Gn_p2m.cxx; L2P kernel/G_l2p.cxx, Gn_l2p.cxx: one
+//        wave per leaf box; solid harmonics from a Cartesian recurrence (no
+//        acos/atan per term), wave reduction through LDS.
+//   M2M/L2L kernel/m2m.cxx, l2l.cxx: dense p^2 x p^2 operators per child
+//        octant (level independent in the scaled basis).
+//   M2L  kernel/m2l.cxx: targets of one parity class share the same 189
+//        lattice offsets, so a tile of targets runs as 189 GEMMs
+//        (operator [p^2 x p^2] x gathered source multipoles [p^2 x tile]).
+//   direct sums kernel/G_dir.cxx, Gni_dir.cxx, Gnj_dir.cxx.
+#include "engine.h"
+#include "tree_util.h"
+
+namespace fmmx {
+
+namespace {
+
+// ---------------------------------------------------------------------
+// Coefficients of the solid-harmonic recurrences, S_n^m = rho^n fac P_n^m e^{im beta}
+// with fac = sqrt((n-m)!/(n+m)!) and the Condon-Shortley phase
+// (kernel/multipole.cxx:7-35):
+//   S_m^m     = -a_m (x+iy) S_{m-1}^{m-1},      a_m = sqrt((2m-1)/(2m))
+//   S_{m+1}^m = b_m z S_m^m,                    b_m = sqrt(2m+1)
+//   S_{n+1}^m = c_nm z S_n^m - d_nm r^2 S_{n-1}^m
+// and of their derivatives (d/dx + i d/dy) S_n^m = gp_nm S_{n-1}^{m+1},
+// (d/dx - i d/dy) S_n^m = -gm_nm S_{n-1}^{m-1}, d/dz S_n^m = gz_nm S_{n-1}^m.
+constexpr double csqrt(double x) {
+  if (x <= 0) return 0;
+  double r = x > 1 ? x : 1;
+  for (int i = 0; i < 200; i++) r = 0.5 * (r + x / r);
+  return r;
+}
+struct HarmCoef {
+  double a[MAXP + 2], b[MAXP + 2], c[MAXP + 2][MAXP + 2], d[MAXP + 2][MAXP + 2];
+  double gp[MAXP + 2][MAXP + 2], gm[MAXP + 2][MAXP + 2], gz[MAXP + 2][MAXP + 2];
+};
+constexpr HarmCoef make_coef() {
+  HarmCoef h{};
+  for (int m = 0; m <= MAXP + 1; m++) {
+    h.a[m] = m > 0 ? csqrt((2.0 * m - 1) / (2.0 * m)) : 0;
+    h.b[m] = csqrt(2.0 * m + 1);
+  }
+  for (int n = 0; n <= MAXP + 1; n++)
+    for (int m = 0; m <= MAXP + 1; m++) {
+      double den = (double)(n + 1 - m) * (n + 1 + m);
+      h.c[n][m] = den > 0 ? (2.0 * n + 1) / csqrt(den) : 0;
+      h.d[n][m] = den > 0 ? csqrt((double)(n - m) * (n + m) / den) : 0;
+      h.gp[n][m] = (n - m) * (n - m - 1) > 0 ? csqrt((double)(n - m) * (n - m - 1)) : 0;
+      h.gm[n][m] = (n + m) * (n + m - 1) > 0 ? csqrt((double)(n + m) * (n + m - 1)) : 0;
+      h.gz[n][m] = (n - m) * (n + m) > 0 ? csqrt((double)(n - m) * (n + m)) : 0;
+    }
+  return h;
+}
+__constant__ HarmCoef c_hc = make_coef();
+
+__host__ __device__ constexpr int sl(int n, int m) { return n * (n + 1) / 2 + m; }
+
+template <int P, typename T>
+__device__ __forceinline__ void solid_harmonics(T x, T y, T z, T* Sr, T* Si) {
+  const T r2 = x * x + y * y + z * z;
+  Sr[0] = T(1);
+  Si[0] = T(0);
+#pragma unroll
+  for (int m = 0; m < P; m++) {
+    if (m > 0) {
+      const T a = -T(c_hc.a[m]);
+      const int pm = sl(m - 1, m - 1);
+      Sr[sl(m, m)] = a * (x * Sr[pm] - y * Si[pm]);
+      Si[sl(m, m)] = a * (x * Si[pm] + y * Sr[pm]);
+    }
+    if (m + 1 < P) {
+      const T b = T(c_hc.b[m]) * z;
+      Sr[sl(m + 1, m)] = b * Sr[sl(m, m)];
+      Si[sl(m + 1, m)] = b * Si[sl(m, m)];
+    }
+#pragma unroll
+    for (int n = m + 1; n + 1 < P; n++) {
+      const T c = T(c_hc.c[n][m]) * z, d = T(c_hc.d[n][m]) * r2;
+      Sr[sl(n + 1, m)] = c * Sr[sl(n, m)] - d * Sr[sl(n - 1, m)];
+      Si[sl(n + 1, m)] = c * Si[sl(n, m)] - d * Si[sl(n - 1, m)];
+    }
+  }
+}
+
+// Gradient of S_n^m (n >= 1) in terms of degree n-1 values:
+// dS/dx = (Ap + Am)/2, dS/dy = -i (Ap - Am)/2, dS/dz = gz S_{n-1}^m
+template <int P, typename T>
+__device__ __forceinline__ void harm_grad(const T* Sr, const T* Si, int n, int m, T& pr, T& pi, T& mr, T& mi,
+                                          T& zr, T& zi) {
+  pr = pi = mr = mi = zr = zi = T(0);
+  if (m + 1 <= n - 1) {
+    const T g = T(c_hc.gp[n][m]);
+    pr = g * Sr[sl(n - 1, m + 1)];
+    pi = g * Si[sl(n - 1, m + 1)];
+  }
+  if (m >= 1) {
+    const T g = -T(c_hc.gm[n][m]);
+    mr = g * Sr[sl(n - 1, m - 1)];
+    mi = g * Si[sl(n - 1, m - 1)];
+  } else if (n >= 2) {
+    const T g = T(c_hc.gp[n][0]);  // m = 0: conj(S_{n-1}^1) term
+    mr = g * Sr[sl(n - 1, 1)];
+    mi = -g * Si[sl(n - 1, 1)];
+  }
+  if (m <= n - 1) {
+    const T g = T(c_hc.gz[n][m]);
+    zr = g * Sr[sl(n - 1, m)];
+    zi = g * Si[sl(n - 1, m)];
+  }
+}
+
+// Evaluation point of an expansion in units of the box size.  REF: the
+// reference's angles (kernel/G_p2m.cxx:31-41) with eps added to the radius
+// and float pi in the azimuth branches; the harmonics at these angles equal
+// the solid harmonics at the point below.
+__device__ __forceinline__ void eff_point(int conv, double dx, double dy, double dz, double rbinv, double& ux,
+                                          double& uy, double& uz) {
+  if (conv == FMMX_CONV_EXACT) {
+    ux = dx * rbinv;
+    uy = dy * rbinv;
+    uz = dz * rbinv;
+    return;
+  }
+  const double eps = (double)EPSF, pif = (double)PIF;
+  double rh = sqrt(dx * dx + dy * dy + dz * dz) + eps;
+  double ca = cos(acos(dz / rh));
+  double sa = sqrt((1 - ca) * (1 + ca));
+  double be;
+  if (fabs(dx) + fabs(dy) < eps) be = 0;
+  else if (fabs(dx) < eps) be = dy / fabs(dy) * pif * 0.5;
+  else if (dx > 0) be = atan(dy / dx);
+  else be = atan(dy / dx) + pif;
+  double sb, cb;
+  sincos(be, &sb, &cb);
+  ux = rh * sa * cb * rbinv;
+  uy = rh * sa * sb * rbinv;
+  uz = rh * ca * rbinv;
+}
+
+__device__ __forceinline__ void box_center(uint32_t key, int lev, double xmin, double ymin, double zmin, double rb,
+                                           double& cx, double& cy, double& cz) {
+  int ix, iy, iz;
+  morton_decode(key, lev, ix, iy, iz);
+  cx = xmin + (ix + 0.5) * rb;
+  cy = ymin + (iy + 0.5) * rb;
+  cz = zmin + (iz + 0.5) * rb;
+}
+
+// ---------------------------------------------------------------------
+// P2M: one wave per source leaf box
+template <int P, typename T>
+__global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
+                                            const int* __restrict__ blast, const float* __restrict__ x,
+                                            const float* __restrict__ y, const float* __restrict__ z,
+                                            const float* __restrict__ nx, const float* __restrict__ ny,
+                                            const float* __restrict__ nz, const float* __restrict__ q, int gn,
+                                            int conv, int lev, double xmin, double ymin, double zmin, double rb,
+                                            const double* __restrict__ corr, T* __restrict__ M) {
+  constexpr int NR = P * (P + 1) / 2, P2 = P * P, NC = (P2 + 63) / 64;
+  __shared__ T red[64][P2 + 1];
+  const int b = blockIdx.x, lane = threadIdx.x;
+  if (b >= nb) return;
+  double cx, cy, cz;
+  box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
+  const double rbinv = 1.0 / rb;
+  T acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) acc[c] = T(0);
+  const int i0 = bfirst[b], i1 = blast[b];
+  for (int base = i0; base <= i1; base += 64) {
+    const int i = base + lane;
+    T* row = red[lane];
+    if (i <= i1) {
+      double ux, uy, uz;
+      eff_point(conv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
+      T Sr[NR], Si[NR];
+      solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
+      if (!gn) {
+        const T qi = T(q[i]);
+#pragma unroll
+        for (int n = 0; n < P; n++)
+#pragma unroll
+          for (int m = 0; m <= n; m++) {
+            const T w = qi * T(corr[sl(n, m)]);
+            row[sl(n, m)] = w * Sr[sl(n, m)];
+            if (m > 0) row[NR + n * (n - 1) / 2 + m - 1] = -w * Si[sl(n, m)];
+          }
+      } else {
+        // dipole q*n (the float product vj*gxj of kernel/Gn_p2m.cxx:113)
+        const T qx = T(q[i] * nx[i]), qy = T(q[i] * ny[i]), qz = T(q[i] * nz[i]);
+        row[0] = T(0);
+#pragma unroll
+        for (int n = 1; n < P; n++)
+#pragma unroll
+          for (int m = 0; m <= n; m++) {
+            T pr, pi, mr, mi, zr, zi;
+            harm_grad<P, T>(Sr, Si, n, m, pr, pi, mr, mi, zr, zi);
+            // n . grad S = qx (Ap+Am)/2 + qy (-i)(Ap-Am)/2 + qz dS/dz
+            const T dr = T(0.5) * qx * (pr + mr) + T(0.5) * qy * (pi - mi) + qz * zr;
+            const T di = T(0.5) * qx * (pi + mi) - T(0.5) * qy * (pr - mr) + qz * zi;
+            const T w = T(corr[sl(n, m)] * rbinv);
+            row[sl(n, m)] = w * dr;
+            if (m > 0) row[NR + n * (n - 1) / 2 + m - 1] = -w * di;
+          }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < P2; c++) row[c] = T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const int col = lane + 64 * c;
+      if (col < P2) {
+        T s = T(0);
+        for (int r = 0; r < 64; r++) s += red[r][col];
+        acc[c] += s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int col = lane + 64 * c;
+    if (col < P2) M[(size_t)b * P2 + col] = acc[c];
+  }
+}
+
+// ---------------------------------------------------------------------
+// L2P: one wave per target leaf box; adds to the P2P results (sorted order)
+template <int P, typename T>
+__global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
+                                            const int* __restrict__ blast, const float* __restrict__ x,
+                                            const float* __restrict__ y, const float* __restrict__ z,
+                                            const float* __restrict__ nx, const float* __restrict__ ny,
+                                            const float* __restrict__ nz, const T* __restrict__ L, int gn, int grad,
+                                            int conv, int lev, double xmin, double ymin, double zmin, double rb,
+                                            const double* __restrict__ corr, double c4, double* __restrict__ vs,
+                                            double* __restrict__ gxs, double* __restrict__ gys,
+                                            double* __restrict__ gzs) {
+  constexpr int NR = P * (P + 1) / 2, P2 = P * P;
+  __shared__ T Lr[NR], Li[NR];
+  const int b = blockIdx.x, lane = threadIdx.x;
+  if (b >= nb) return;
+  // weights: 1 for m = 0, 2 for m > 0 (kernel/G_l2p.cxx:246-260); table ratio in REF
+  for (int s = lane; s < NR; s += 64) {
+    int n = 0;
+    while (sl(n + 1, 0) <= s) n++;
+    const int m = s - sl(n, 0);
+    const double w = (m == 0 ? 1.0 : 2.0) * corr[s];
+    Lr[s] = T(w * (double)L[(size_t)b * P2 + s]);
+    Li[s] = m == 0 ? T(0) : T(w * (double)L[(size_t)b * P2 + NR + n * (n - 1) / 2 + m - 1]);
+  }
+  __syncthreads();
+  double cx, cy, cz;
+  box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
+  const double rbinv = 1.0 / rb;
+  const int i0 = bfirst[b], i1 = blast[b];
+  for (int i = i0 + lane; i <= i1; i += 64) {
+    double ux, uy, uz;
+    eff_point(conv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
+    T Sr[NR], Si[NR];
+    solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
+    T phi = T(0), gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+    for (int n = 0; n < P; n++)
+#pragma unroll
+      for (int m = 0; m <= n; m++) {
+        const int s = sl(n, m);
+        const T lr = Lr[s], li = Li[s];
+        phi += Sr[s] * lr - Si[s] * li;
+        if (n >= 1) {
+          T pr, pi, mr, mi, zr, zi;
+          harm_grad<P, T>(Sr, Si, n, m, pr, pi, mr, mi, zr, zi);
+          // Re(Ap L), Im(Ap L), Re(Am L), Im(Am L)
+          const T apr = pr * lr - pi * li, api = pr * li + pi * lr;
+          const T amr = mr * lr - mi * li, ami = mr * li + mi * lr;
+          gx += T(0.5) * (apr + amr);
+          gy += T(0.5) * (api - ami);
+          gz += zr * lr - zi * li;
+        }
+      }
+    const double s1 = c4 * rbinv, s2 = c4 * rbinv * rbinv;
+    if (gn) {
+      vs[i] -= s2 * ((double)nx[i] * (double)gx + (double)ny[i] * (double)gy + (double)nz[i] * (double)gz);
+    } else {
+      vs[i] += s1 * (double)phi;
+      if (grad) {
+        gxs[i] += s2 * (double)gx;
+        gys[i] += s2 * (double)gy;
+        gzs[i] += s2 * (double)gz;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// M2M: one workgroup per parent; operators stored transposed [oct][k][row]
+template <typename T>
+__global__ __launch_bounds__(128) void k_m2m(int nbp, int P2, const int* __restrict__ cfirst,
+                                             const uint32_t* __restrict__ ckey, const T* __restrict__ Mc,
+                                             T* __restrict__ Mp, const T* __restrict__ opT) {
+  extern __shared__ unsigned char smem_raw[];
+  T* v = reinterpret_cast<T*>(smem_raw);
+  const int b = blockIdx.x;
+  if (b >= nbp) return;
+  T acc0 = T(0), acc1 = T(0);
+  const int r0 = threadIdx.x, r1 = threadIdx.x + 128;
+  for (int c = cfirst[b]; c < cfirst[b + 1]; c++) {
+    for (int k = threadIdx.x; k < P2; k += blockDim.x) v[k] = Mc[(size_t)c * P2 + k];
+    __syncthreads();
+    const T* A = opT + (size_t)(ckey[c] & 7) * P2 * P2;
+    for (int k = 0; k < P2; k++) {
+      const T vk = v[k];
+      if (r0 < P2) acc0 += A[(size_t)k * P2 + r0] * vk;
+      if (r1 < P2) acc1 += A[(size_t)k * P2 + r1] * vk;
+    }
+    __syncthreads();
+  }
+  if (r0 < P2) Mp[(size_t)b * P2 + r0] = acc0;
+  if (r1 < P2) Mp[(size_t)b * P2 + r1] = acc1;
+}
+
+// L2L: one workgroup per child box; writes (M2L then accumulates)
+template <typename T>
+__global__ __launch_bounds__(128) void k_l2l(int nbc, int P2, const int* __restrict__ parent,
+                                             const uint32_t* __restrict__ ckey, const T* __restrict__ Lp,
+                                             T* __restrict__ Lc, const T* __restrict__ opT) {
+  extern __shared__ unsigned char smem_raw[];
+  T* v = reinterpret_cast<T*>(smem_raw);
+  const int b = blockIdx.x;
+  if (b >= nbc) return;
+  const int pb = parent[b];
+  for (int k = threadIdx.x; k < P2; k += blockDim.x) v[k] = Lp[(size_t)pb * P2 + k];
+  __syncthreads();
+  const T* A = opT + (size_t)(ckey[b] & 7) * P2 * P2;
+  for (int r = threadIdx.x; r < P2; r += blockDim.x) {
+    T acc = T(0);
+    for (int k = 0; k < P2; k++) acc += A[(size_t)k * P2 + r] * v[k];
+    Lc[(size_t)b * P2 + r] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------
+// M2L: a workgroup owns TN targets of one parity class at a level and loops
+// over the class's 189 offsets; per offset the operator (K-chunked) and the
+// gathered source multipoles go through LDS and every thread accumulates a
+// RPT x 4 register block of (rows x targets).
+constexpr int M2L_TN = 64, M2L_KC = 32, M2L_BST = M2L_TN + 4;
+
+template <int P2, typename T>
+__global__ __launch_bounds__(256) void k_m2l(int lev, const int* __restrict__ cls_cnt, const uint32_t* __restrict__ ord,
+                                             const uint32_t* __restrict__ ikey, const int* __restrict__ dense_j,
+                                             const T* __restrict__ M, T* __restrict__ L, const T* __restrict__ opT,
+                                             const int4* __restrict__ offtab, int init,
+                                             unsigned long long* __restrict__ npairs) {
+  constexpr int RPT = (P2 + 15) / 16;
+  __shared__ T As[M2L_KC][P2];
+  __shared__ T Bs[M2L_KC][M2L_BST];
+  __shared__ int tpos[M2L_TN][3];
+  __shared__ int tord[M2L_TN];
+  __shared__ int src[M2L_TN];
+  __shared__ int s_class, s_start, s_n;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    // find this workgroup's (class, tile)
+    int w = blockIdx.x, start = 0, cls = -1, n = 0;
+    for (int c = 0; c < 8; c++) {
+      const int cnt = cls_cnt[c], nt = (cnt + M2L_TN - 1) / M2L_TN;
+      if (cls < 0 && w < nt) {
+        cls = c;
+        n = min(M2L_TN, cnt - w * M2L_TN);
+        start += w * M2L_TN;
+      } else if (cls < 0) {
+        w -= nt;
+        start += cnt;
+      }
+    }
+    s_class = cls;
+    s_start = start;
+    s_n = n;
+  }
+  __syncthreads();
+  const int cls = s_class, ntile = s_n;
+  if (cls < 0) return;
+  const int g = 1 << lev;
+  if (tid < M2L_TN) {
+    int ix = -1000, iy = -1000, iz = -1000, o = -1;
+    if (tid < ntile) {
+      o = (int)ord[s_start + tid];
+      morton_decode(ikey[o], lev, ix, iy, iz);
+    }
+    tpos[tid][0] = ix;
+    tpos[tid][1] = iy;
+    tpos[tid][2] = iz;
+    tord[tid] = o;
+  }
+  const int tx = tid & 15, ty = tid >> 4;
+  T acc[RPT][4];
+#pragma unroll
+  for (int r = 0; r < RPT; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[r][c] = T(0);
+  unsigned long long mypairs = 0;
+  for (int oi = 0; oi < NEBM; oi++) {
+    const int4 off = offtab[cls * NEBM + oi];
+    __syncthreads();
+    if (tid < M2L_TN) {
+      int s = -1;
+      if (tid < ntile) {
+        const int jx = tpos[tid][0] - off.x, jy = tpos[tid][1] - off.y, jz = tpos[tid][2] - off.z;
+        if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g) s = dense_j[morton_encode(jx, jy, jz, lev)];
+      }
+      src[tid] = s;
+      if (s >= 0) mypairs++;
+    }
+    const T* A = opT + (size_t)off.w * P2 * P2;
+    for (int k0 = 0; k0 < P2; k0 += M2L_KC) {
+      const int kc = min(M2L_KC, P2 - k0);
+      __syncthreads();
+      for (int e = tid; e < kc * P2; e += 256) As[e / P2][e % P2] = A[(size_t)k0 * P2 + e];
+      for (int e = tid; e < kc * M2L_TN; e += 256) {
+        const int t = e / kc, k = e % kc, s = src[t];
+        Bs[k][t] = s >= 0 ? M[(size_t)s * P2 + k0 + k] : T(0);
+      }
+      __syncthreads();
+      for (int k = 0; k < kc; k++) {
+        T bv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) bv[c] = Bs[k][tx * 4 + c];
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+          const int row = ty + 16 * r;
+          const T av = row < P2 ? As[k][row] : T(0);
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[r][c] += av * bv[c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int t = tx * 4 + c;
+    if (t >= ntile) continue;
+    T* dst = L + (size_t)tord[t] * P2;
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int row = ty + 16 * r;
+      if (row < P2) dst[row] = init ? acc[r][c] : dst[row] + acc[r][c];
+    }
+  }
+  if (tid < M2L_TN && mypairs) atomicAdd(npairs, mypairs);
+}
+
+// ---------------------------------------------------------------------
+// P2P: one workgroup per target leaf box (chunks of BS targets), sources of
+// each near box staged through LDS in tiles of BS.
+template <typename T, int NEQ, int CONV>
+__device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, float nyi, float nzi, float4 s,
+                                     float4 sn, T& p, T& fx, T& fy, T& fz) {
+  T dx, dy, dz;
+  if (CONV == FMMX_CONV_REF || sizeof(T) == 4) {
+    dx = T(xi - s.x);  // float subtraction as kernel/G_p2p.cxx:21-23
+    dy = T(yi - s.y);
+    dz = T(zi - s.z);
+  } else {
+    dx = T(xi) - T(s.x);
+    dy = T(yi) - T(s.y);
+    dz = T(zi) - T(s.z);
+  }
+  T r2 = dx * dx + dy * dy + dz * dz;
+  T inv;
+  if (CONV == FMMX_CONV_REF) {
+    inv = rsqrt(r2 + T(EPSF));
+  } else {
+    inv = r2 > T(0) ? rsqrt(r2) : T(0);
+  }
+  const T qi = T(s.w) * inv;
+  if (NEQ == FMMX_G) {
+    p += qi;
+    const T rs = qi * inv * inv;
+    fx -= dx * rs;
+    fy -= dy * rs;
+    fz -= dz * rs;
+  } else if (NEQ == FMMX_GNI) {
+    p += (dx * T(nxi) + dy * T(nyi) + dz * T(nzi)) * (qi * inv * inv);
+  } else {
+    p += (dx * T(sn.x) + dy * T(sn.y) + dz * T(sn.z)) * (qi * inv * inv);
+  }
+}
+
+template <typename T, int NEQ, int CONV, int BS>
+__global__ __launch_bounds__(BS) void k_p2p(int nbi, const int* __restrict__ ifirst, const int* __restrict__ ilast,
+                                            const int* __restrict__ nij, const int* __restrict__ neij,
+                                            const int* __restrict__ jfirst, const int* __restrict__ jlast,
+                                            const float* __restrict__ xi, const float* __restrict__ yi,
+                                            const float* __restrict__ zi, const float* __restrict__ nxi,
+                                            const float* __restrict__ nyi, const float* __restrict__ nzi,
+                                            const float* __restrict__ xj, const float* __restrict__ yj,
+                                            const float* __restrict__ zj, const float* __restrict__ q,
+                                            const float* __restrict__ nxj, const float* __restrict__ nyj,
+                                            const float* __restrict__ nzj, double c4, double* __restrict__ vs,
+                                            double* __restrict__ gxs, double* __restrict__ gys,
+                                            double* __restrict__ gzs) {
+  __shared__ float4 sp[BS];
+  __shared__ float4 sn[NEQ == FMMX_GNJ ? BS : 1];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  if (b >= nbi) return;
+  const int i0 = ifirst[b], i1 = ilast[b], nn = nij[b];
+  for (int base = i0; base <= i1; base += BS) {
+    const int i = base + tid;
+    const bool act = i <= i1;
+    float px = 0, py = 0, pz = 0, qx = 0, qy = 0, qz = 0;
+    if (act) {
+      px = xi[i];
+      py = yi[i];
+      pz = zi[i];
+      if (NEQ == FMMX_GNI) {
+        qx = nxi[i];
+        qy = nyi[i];
+        qz = nzi[i];
+      }
+    }
+    double ap = 0, ax = 0, ay = 0, az = 0;
+    for (int e = 0; e < nn; e++) {
+      const int jj = neij[b * NNEAR + e];
+      const int j0 = jfirst[jj], j1 = jlast[jj];
+      for (int jb = j0; jb <= j1; jb += BS) {
+        const int js = jb + tid;
+        __syncthreads();
+        if (js <= j1) {
+          sp[tid] = make_float4(xj[js], yj[js], zj[js], q[js]);
+          if (NEQ == FMMX_GNJ) sn[tid] = make_float4(nxj[js], nyj[js], nzj[js], 0.f);
+        }
+        __syncthreads();
+        const int cnt = min(BS, j1 - jb + 1);
+        T p = T(0), fx = T(0), fy = T(0), fz = T(0);
+        int s = 0;
+        for (; s + 4 <= cnt; s += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx, fy, fz);
+        }
+        for (; s < cnt; s++)
+          pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz);
+        ap += (double)p;
+        if (NEQ == FMMX_G) {
+          ax += (double)fx;
+          ay += (double)fy;
+          az += (double)fz;
+        }
+      }
+    }
+    if (act) {
+      vs[i] = c4 * ap;
+      if (NEQ == FMMX_G) {
+        gxs[i] = c4 * ax;
+        gys[i] = c4 * ay;
+        gzs[i] = c4 * az;
+      }
+    }
+  }
+}
+
+// direct sums: block of 64 targets x a slice of the sources, FP64 pair math,
+// partial sums combined with atomics (outputs zeroed first)
+template <int NEQ, int CONV>
+__global__ __launch_bounds__(64) void k_direct(int ni, const float* __restrict__ xi, const float* __restrict__ yi,
+                                               const float* __restrict__ zi, const float* __restrict__ nxi,
+                                               const float* __restrict__ nyi, const float* __restrict__ nzi, int nj,
+                                               const float* __restrict__ xj, const float* __restrict__ yj,
+                                               const float* __restrict__ zj, const float* __restrict__ q,
+                                               const float* __restrict__ nxj, const float* __restrict__ nyj,
+                                               const float* __restrict__ nzj, int slice, double c4,
+                                               double* __restrict__ v, double* __restrict__ gx,
+                                               double* __restrict__ gy, double* __restrict__ gz) {
+  __shared__ float4 sp[64];
+  __shared__ float4 sn[64];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const bool act = i < ni;
+  float px = 0, py = 0, pz = 0, qx = 0, qy = 0, qz = 0;
+  if (act) {
+    px = xi[i];
+    py = yi[i];
+    pz = zi[i];
+    if (NEQ == FMMX_GNI) {
+      qx = nxi[i];
+      qy = nyi[i];
+      qz = nzi[i];
+    }
+  }
+  const int j0 = blockIdx.y * slice, j1 = min(nj, j0 + slice);
+  double p = 0, fx = 0, fy = 0, fz = 0;
+  for (int jb = j0; jb < j1; jb += 64) {
+    const int js = jb + threadIdx.x;
+    __syncthreads();
+    if (js < j1) {
+      sp[threadIdx.x] = make_float4(xj[js], yj[js], zj[js], q[js]);
+      if (NEQ == FMMX_GNJ) sn[threadIdx.x] = make_float4(nxj[js], nyj[js], nzj[js], 0.f);
+    }
+    __syncthreads();
+    const int cnt = min(64, j1 - jb);
+    for (int s = 0; s < cnt; s++) pair<double, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], sn[s], p, fx, fy, fz);
+  }
+  if (act) {
+    atomicAdd(&v[i], c4 * p);
+    if (NEQ == FMMX_G) {
+      atomicAdd(&gx[i], c4 * fx);
+      atomicAdd(&gy[i], c4 * fy);
+      atomicAdd(&gz[i], c4 * fz);
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_zero_t(long n, T* __restrict__ a) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = T(0);
+}
+__global__ void k_zero_d(long n, double* __restrict__ a) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+template <class T>
+struct tag {
+  using type = T;
+};
+
+template <typename F>
+int dispatch_p(int p, F&& f) {
+  switch (p) {
+#define FMMX_CASE(P) \
+  case P:            \
+    return f(std::integral_constant<int, P>{});
+    FMMX_CASE(2) FMMX_CASE(3) FMMX_CASE(4) FMMX_CASE(5) FMMX_CASE(6) FMMX_CASE(7) FMMX_CASE(8) FMMX_CASE(9)
+    FMMX_CASE(10) FMMX_CASE(11) FMMX_CASE(12)
+#undef FMMX_CASE
+  }
+  return fail(FMMX_EUNSUPPORTED, "unsupported p");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// launchers
+
+int Engine::run_p2p(int neq) {
+  const int nbi = si.nb[lmax];
+  if (nbi == 0) return FMMX_OK;
+  const double cc = c4();
+  const bool ref = cfg.convention == FMMX_CONV_REF;
+  const double mean = (double)si.n / nbi;
+  auto go = [&](auto tag_t, auto neq_c, auto conv_c, auto bs_c) -> int {
+    using T = typename decltype(tag_t)::type;
+    constexpr int NEQ = decltype(neq_c)::value, CONV = decltype(conv_c)::value, BS = decltype(bs_c)::value;
+    k_p2p<T, NEQ, CONV, BS><<<nbi, BS, 0, stream>>>(
+        nbi, si.bfirst[lmax].as<int>(), si.blast.as<int>(), nij.as<int>(), neij.as<int>(), sj.bfirst[lmax].as<int>(),
+        sj.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(),
+        si.nz.as<float>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), qs.as<float>(), sj.nx.as<float>(),
+        sj.ny.as<float>(), sj.nz.as<float>(), cc, vs.as<double>(), gxs.as<double>(), gys.as<double>(),
+        gzs.as<double>());
+    return FMMX_OK;
+  };
+  auto by_bs = [&](auto t, auto nq, auto cv) -> int {
+    if (mean > 96) return go(t, nq, cv, std::integral_constant<int, 256>{});
+    if (mean > 40) return go(t, nq, cv, std::integral_constant<int, 128>{});
+    return go(t, nq, cv, std::integral_constant<int, 64>{});
+  };
+  auto by_conv = [&](auto t, auto nq) -> int {
+    return ref ? by_bs(t, nq, std::integral_constant<int, FMMX_CONV_REF>{})
+               : by_bs(t, nq, std::integral_constant<int, FMMX_CONV_EXACT>{});
+  };
+  auto by_neq = [&](auto t) -> int {
+    if (neq == FMMX_G) return by_conv(t, std::integral_constant<int, FMMX_G>{});
+    if (neq == FMMX_GNI) return by_conv(t, std::integral_constant<int, FMMX_GNI>{});
+    return by_conv(t, std::integral_constant<int, FMMX_GNJ>{});
+  };
+  int rc = fp64 ? by_neq(tag<double>{}) : by_neq(tag<float>{});
+  if (rc) return rc;
+  FMMX_HIP(hipGetLastError());
+  return FMMX_OK;
+}
+
+int Engine::run_p2m(int neq) {
+  const int nb = sj.nb[lmax];
+  if (nb == 0) return FMMX_OK;
+  const double* corr = corr_d.as<double>();
+  return dispatch_p(p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    if (fp64)
+      k_p2m<P, double><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),
+                                              sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(),
+                                              sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), qs.as<float>(),
+                                              neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax),
+                                              corr, mexp[lmax].as<double>());
+    else
+      k_p2m<P, float><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),
+                                             sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(),
+                                             sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), qs.as<float>(),
+                                             neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr,
+                                             mexp[lmax].as<float>());
+    FMMX_HIP(hipGetLastError());
+    return FMMX_OK;
+  });
+}
+
+int Engine::run_l2p(int neq) {
+  const int nb = si.nb[lmax];
+  if (nb == 0) return FMMX_OK;
+  const double* corr = corr_d.as<double>();
+  return dispatch_p(p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    if (fp64)
+      k_l2p<P, double><<<nb, 64, 0, stream>>>(nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(),
+                                              si.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(),
+                                              si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),
+                                              lexp[lmax].as<double>(), neq == FMMX_GNI, neq == FMMX_G, cfg.convention,
+                                              lmax, xmin, ymin, zmin, rb(lmax), corr, c4(), vs.as<double>(),
+                                              gxs.as<double>(), gys.as<double>(), gzs.as<double>());
+    else
+      k_l2p<P, float><<<nb, 64, 0, stream>>>(nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(),
+                                             si.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(),
+                                             si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),
+                                             lexp[lmax].as<float>(), neq == FMMX_GNI, neq == FMMX_G, cfg.convention,
+                                             lmax, xmin, ymin, zmin, rb(lmax), corr, c4(), vs.as<double>(),
+                                             gxs.as<double>(), gys.as<double>(), gzs.as<double>());
+    FMMX_HIP(hipGetLastError());
+    return FMMX_OK;
+  });
+}
+
+int Engine::run_m2m(int lev) {
+  const int nbp = sj.nb[lev];
+  if (nbp == 0) return FMMX_OK;
+  const size_t sh = (size_t)P2 * (fp64 ? 8 : 4);
+  if (fp64)
+    k_m2m<double><<<nbp, 128, sh, stream>>>(nbp, P2, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(),
+                                            mexp[lev + 1].as<double>(), mexp[lev].as<double>(), op_m2m.as<double>());
+  else
+    k_m2m<float><<<nbp, 128, sh, stream>>>(nbp, P2, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(),
+                                           mexp[lev + 1].as<float>(), mexp[lev].as<float>(), op_m2m.as<float>());
+  FMMX_HIP(hipGetLastError());
+  return FMMX_OK;
+}
+
+int Engine::run_l2l(int lev) {
+  const int nbc = si.nb[lev];
+  if (nbc == 0) return FMMX_OK;
+  const size_t sh = (size_t)P2 * (fp64 ? 8 : 4);
+  if (fp64)
+    k_l2l<double><<<nbc, 128, sh, stream>>>(nbc, P2, si.parent[lev].as<int>(), si.bkey[lev].as<uint32_t>(),
+                                            lexp[lev - 1].as<double>(), lexp[lev].as<double>(), op_l2l.as<double>());
+  else
+    k_l2l<float><<<nbc, 128, sh, stream>>>(nbc, P2, si.parent[lev].as<int>(), si.bkey[lev].as<uint32_t>(),
+                                           lexp[lev - 1].as<float>(), lexp[lev].as<float>(), op_l2l.as<float>());
+  FMMX_HIP(hipGetLastError());
+  return FMMX_OK;
+}
+
+int Engine::run_m2l(int lev, bool init) {
+  const int nb = si.nb[lev];
+  if (nb == 0) return FMMX_OK;
+  const size_t opstride = (size_t)NOFF * P2 * P2;
+  const size_t opoff = m2l_per_level ? (size_t)(lev - 2) * opstride : 0;
+  const int grid = (nb + M2L_TN - 1) / M2L_TN + 8;
+  if (sj.nb[lev] == 0) {
+    if (init) {
+      long n = (long)nb * P2;
+      if (fp64) k_zero_t<double><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, lexp[lev].as<double>());
+      else k_zero_t<float><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, lexp[lev].as<float>());
+    }
+    return FMMX_OK;
+  }
+  return dispatch_p(p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value, Q = P * P;
+    if (fp64)
+      k_m2l<Q, double><<<grid, 256, 0, stream>>>(lev, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(),
+                                                 si.bkey[lev].as<uint32_t>(), sj.dense[lev].as<int>(),
+                                                 mexp[lev].as<double>(), lexp[lev].as<double>(),
+                                                 op_m2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0,
+                                                 pair_ctr.as<unsigned long long>() + 1);
+    else
+      k_m2l<Q, float><<<grid, 256, 0, stream>>>(lev, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(),
+                                                si.bkey[lev].as<uint32_t>(), sj.dense[lev].as<int>(),
+                                                mexp[lev].as<float>(), lexp[lev].as<float>(),
+                                                op_m2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0,
+                                                pair_ctr.as<unsigned long long>() + 1);
+    FMMX_HIP(hipGetLastError());
+    return FMMX_OK;
+  });
+}
+
+int Engine::run_direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
+                       const float* nyi, const float* nzi, int nj, const float* xj, const float* yj, const float* zj,
+                       const float* nxj, const float* nyj, const float* nzj, const float* qj, double* v, double* gx,
+                       double* gy, double* gz) {
+  if (ni == 0) return FMMX_OK;
+  long nz4 = ni;
+  k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, v);
+  if (neq == FMMX_G) {
+    k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gx);
+    k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gy);
+    k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gz);
+  }
+  if (nj == 0) return FMMX_OK;
+  const int tb = (ni + 63) / 64;
+  int ns = std::max(1, std::min(1024, 2048 / tb));
+  int slice = (nj + ns - 1) / ns;
+  slice = ((slice + 63) / 64) * 64;
+  ns = (nj + slice - 1) / slice;
+  dim3 grid(tb, ns);
+  const bool ref = cfg.convention == FMMX_CONV_REF;
+  const double cc = c4();
+#define FMMX_DIR(NEQ, CONV)                                                                                      \
+  k_direct<NEQ, CONV><<<grid, 64, 0, stream>>>(ni, xi, yi, zi, nxi, nyi, nzi, nj, xj, yj, zj, qj, nxj, nyj, nzj, \
+                                               slice, cc, v, gx, gy, gz)
+  if (ref) {
+    if (neq == FMMX_G) FMMX_DIR(FMMX_G, FMMX_CONV_REF);
+    else if (neq == FMMX_GNI) FMMX_DIR(FMMX_GNI, FMMX_CONV_REF);
+    else FMMX_DIR(FMMX_GNJ, FMMX_CONV_REF);
+  } else {
+    if (neq == FMMX_G) FMMX_DIR(FMMX_G, FMMX_CONV_EXACT);
+    else if (neq == FMMX_GNI) FMMX_DIR(FMMX_GNI, FMMX_CONV_EXACT);
+    else FMMX_DIR(FMMX_GNJ, FMMX_CONV_EXACT);
+  }
+#undef FMMX_DIR
+  FMMX_HIP(hipGetLastError());
+  return FMMX_OK;
+}
+
+}  // namespace fmmx

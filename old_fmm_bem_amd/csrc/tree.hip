@@ -1,0 +1,315 @@
+// tree.hip -- GPU tree build: Morton keys (fmm/boxn.cxx), stable radix sort
+// (the reference's counting sort, sort/csort.cxx, is stable; so is an LSD
+// radix sort), SoA gathers (sort/sortvar.cxx), per-level box lists
+// (fmm/boxdatai.cxx, boxdataj.cxx, boxm2mi.cxx, boxm2mj.cxx), and the
+// interaction lists of fmm/ijbox.cxx in the reference's loop order.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "engine.h"
+#include "tree_util.h"
+
+namespace fmmx {
+
+namespace {
+
+constexpr int TPB = 256;
+inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
+
+// fmm/boxn.cxx:12-20: (x - xmin) in float, times kl/rd in double, truncate.
+// The reference clamps only the upper face; coordinates outside the cube are
+// clamped to the nearest box here instead of producing invalid keys.
+__global__ void k_keys(int n, const float* __restrict__ x, const float* __restrict__ y,
+                       const float* __restrict__ z, float xmin, float ymin, float zmin, double rbinv,
+                       int lmax, uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int kl = 1 << lmax;
+  int f0 = (int)((double)(x[i] - xmin) * rbinv);
+  int f1 = (int)((double)(y[i] - ymin) * rbinv);
+  int f2 = (int)((double)(z[i] - zmin) * rbinv);
+  f0 = min(max(f0, 0), kl - 1);
+  f1 = min(max(f1, 0), kl - 1);
+  f2 = min(max(f2, 0), kl - 1);
+  key[i] = morton_encode(f0, f1, f2, lmax);
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void k_gather(int n, const uint32_t* __restrict__ perm, const float* __restrict__ a,
+                         float* __restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[perm[i]];
+}
+
+__global__ void k_level_flags(int n, const uint32_t* __restrict__ keys, int shift, int* __restrict__ flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1 : 0;
+}
+
+__global__ void k_level_compact(int n, const uint32_t* __restrict__ keys, int shift, const int* __restrict__ flag,
+                                const int* __restrict__ pos, uint32_t* __restrict__ bkey,
+                                int* __restrict__ bfirst, int* __restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) {
+    bkey[pos[i]] = keys[i] >> shift;
+    bfirst[pos[i]] = i;
+  }
+  if (i == n - 1) *cnt = pos[i] + flag[i];
+}
+
+// parent ordinal of each box at level lev: the level lev-1 box holding its
+// first particle (inclusive count of level lev-1 box starts, minus one)
+__global__ void k_parent(int ub, const int* __restrict__ cnt, const int* __restrict__ bfirst,
+                         const int* __restrict__ pos_up, const int* __restrict__ flag_up,
+                         int* __restrict__ parent) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ub || b >= *cnt) return;
+  const int i = bfirst[b];
+  parent[b] = pos_up[i] + flag_up[i] - 1;
+}
+
+// first child (at level lev) of every level lev-1 box; children of one
+// parent are consecutive (fmm/boxm2mj.cxx:267-275)
+__global__ void k_cfirst(int ub, const int* __restrict__ cnt_c, const int* __restrict__ cnt_p,
+                         const int* __restrict__ parent, int* __restrict__ cfirst) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int nc = *cnt_c;
+  if (b >= ub || b >= nc) return;
+  if (b == 0 || parent[b] != parent[b - 1]) cfirst[parent[b]] = b;
+  if (b == nc - 1) cfirst[*cnt_p] = nc;
+}
+
+__global__ void k_blast(int ub, const int* __restrict__ cnt, const int* __restrict__ bfirst, int n,
+                        int* __restrict__ blast) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int nb = *cnt;
+  if (b >= ub || b >= nb) return;
+  blast[b] = (b + 1 < nb ? bfirst[b + 1] : n) - 1;
+}
+
+__global__ void k_zero_u64(unsigned long long* a, int n) {
+  if ((int)threadIdx.x < n) a[threadIdx.x] = 0;
+}
+
+__global__ void k_fill_int(long n, int v, int* __restrict__ a) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_dense(int ub, const int* __restrict__ cnt, const uint32_t* __restrict__ bkey,
+                        int* __restrict__ dense) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ub || b >= *cnt) return;
+  dense[bkey[b]] = b;
+}
+
+// fmm/ijbox.cxx:322-345 (neib 0): jx -> jy -> jz over the 3x3x3 block, empty
+// and out-of-grid boxes skipped.  The reference clips to the bounding box of
+// the source boxes (:312-321); positions outside it hold no source box, so
+// the lists are identical.  Also counts particle pairs for the statistics.
+__global__ void k_near_list(int nbi, const uint32_t* __restrict__ bkey, int lev, const int* __restrict__ dense_j,
+                            const int* __restrict__ ifirst, const int* __restrict__ ilast,
+                            const int* __restrict__ jfirst, const int* __restrict__ jlast, int* __restrict__ nij,
+                            int* __restrict__ neij, unsigned long long* __restrict__ pairs) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbi) return;
+  int ix, iy, iz, g = 1 << lev, c = 0;
+  morton_decode(bkey[b], lev, ix, iy, iz);
+  long npair = 0, ni = ilast[b] - ifirst[b] + 1;
+  for (int jx = max(ix - 1, 0); jx <= min(ix + 1, g - 1); jx++)
+    for (int jy = max(iy - 1, 0); jy <= min(iy + 1, g - 1); jy++)
+      for (int jz = max(iz - 1, 0); jz <= min(iz + 1, g - 1); jz++) {
+        int jj = dense_j[morton_encode(jx, jy, jz, lev)];
+        if (jj >= 0) {
+          neij[b * NNEAR + c++] = jj;
+          npair += ni * (jlast[jj] - jfirst[jj] + 1);
+        }
+      }
+  nij[b] = c;
+  atomicAdd(pairs, (unsigned long long)npair);
+}
+
+// fmm/ijbox.cxx:346-365 (neib 1, used at level 2) and :366-401 (neib 2)
+__global__ void k_far_list(int nbi, const uint32_t* __restrict__ ikey, int nbj, const uint32_t* __restrict__ jkey,
+                           int lev, const int* __restrict__ dense_j, int* __restrict__ nij,
+                           int* __restrict__ neij) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbi) return;
+  int ix, iy, iz, g = 1 << lev, c = 0;
+  morton_decode(ikey[b], lev, ix, iy, iz);
+  int* row = neij + (long)b * NEBM;
+  if (lev == 2) {
+    for (int jj = 0; jj < nbj; jj++) {
+      int jx, jy, jz;
+      morton_decode(jkey[jj], lev, jx, jy, jz);
+      if (jx < ix - 1 || ix + 1 < jx || jy < iy - 1 || iy + 1 < jy || jz < iz - 1 || iz + 1 < jz) row[c++] = jj;
+    }
+  } else {
+    int ixp = (ix + 2) / 2, iyp = (iy + 2) / 2, izp = (iz + 2) / 2;
+    for (int jxp = ixp - 1; jxp <= ixp + 1; jxp++)
+      for (int jyp = iyp - 1; jyp <= iyp + 1; jyp++)
+        for (int jzp = izp - 1; jzp <= izp + 1; jzp++)
+          for (int jx = max(2 * jxp - 2, 0); jx <= min(2 * jxp - 1, g - 1); jx++)
+            for (int jy = max(2 * jyp - 2, 0); jy <= min(2 * jyp - 1, g - 1); jy++)
+              for (int jz = max(2 * jzp - 2, 0); jz <= min(2 * jzp - 1, g - 1); jz++)
+                if (jx < ix - 1 || ix + 1 < jx || jy < iy - 1 || iy + 1 < jy || jz < iz - 1 || iz + 1 < jz) {
+                  int jj = dense_j[morton_encode(jx, jy, jz, lev)];
+                  if (jj >= 0) row[c++] = jj;
+                }
+  }
+  nij[b] = c;
+}
+
+// parity class (x&1, y&1, z&1) of each target box: M2L work is grouped by
+// class so that every target of a tile shares the same 189 offsets
+__global__ void k_class(int nb, const uint32_t* __restrict__ bkey, int lev, uint32_t* __restrict__ ck,
+                        uint32_t* __restrict__ ord, int* __restrict__ cnt) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int x, y, z;
+  morton_decode(bkey[b], lev, x, y, z);
+  uint32_t c = (uint32_t)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
+  ck[b] = c;
+  ord[b] = (uint32_t)b;
+  atomicAdd(&cnt[c], 1);
+}
+
+}  // namespace
+
+int Engine::build_side(Side& s, int n, const float* x, const float* y, const float* z, const float* nx,
+                       const float* ny, const float* nz, bool dense) {
+  s.n = n;
+  s.has_normals = nx && ny && nz;
+  size_t nn = (size_t)(n > 0 ? n : 1);
+  int rc;
+  if ((rc = s.keys_in.ensure(nn * 4)) || (rc = s.idx_in.ensure(nn * 4)) || (rc = s.keys.ensure(nn * 4)) ||
+      (rc = s.perm.ensure(nn * 4)) || (rc = s.x.ensure(nn * 4)) || (rc = s.y.ensure(nn * 4)) ||
+      (rc = s.z.ensure(nn * 4)) || (rc = s.cnt.ensure((MAXL + 1) * 4)) || (rc = s.blast.ensure(nn * 4)))
+    return rc;
+  if (s.has_normals &&
+      ((rc = s.nx.ensure(nn * 4)) || (rc = s.ny.ensure(nn * 4)) || (rc = s.nz.ensure(nn * 4))))
+    return rc;
+  s.nb.fill(0);
+  if (n == 0) return FMMX_OK;
+  double rbinv = (double)(1 << lmax) / (double)rd;
+  k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rbinv, lmax, s.keys_in.as<uint32_t>(),
+                                      s.idx_in.as<uint32_t>());
+  size_t tb = 0;
+  FMMX_HIP(rocprim::radix_sort_pairs(nullptr, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
+                                     s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
+                                     stream));
+  size_t sb = 0;
+  FMMX_HIP(rocprim::exclusive_scan(nullptr, sb, (int*)nullptr, (int*)nullptr, 0, (size_t)n, rocprim::plus<int>(),
+                                   stream));
+  if ((rc = scratch.ensure(std::max(tb, sb) + 256))) return rc;
+  tb = scratch.bytes;
+  FMMX_HIP(rocprim::radix_sort_pairs(scratch.p, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
+                                     s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
+                                     stream));
+  const uint32_t* perm = s.perm.as<uint32_t>();
+  k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, x, s.x.as<float>());
+  k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, y, s.y.as<float>());
+  k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, z, s.z.as<float>());
+  if (s.has_normals) {
+    k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nx, s.nx.as<float>());
+    k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, ny, s.ny.as<float>());
+    k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nz, s.nz.as<float>());
+  }
+  int* cnt = s.cnt.as<int>();
+  for (int lev = lmax; lev >= 2; lev--) {
+    if ((rc = s.flag[lev].ensure(nn * 4)) || (rc = s.pos[lev].ensure(nn * 4)) ||
+        (rc = s.bkey[lev].ensure(nn * 4)) || (rc = s.bfirst[lev].ensure(nn * 4)) ||
+        (rc = s.parent[lev].ensure(nn * 4)) || (rc = s.cfirst[lev].ensure((nn + 1) * 4)))
+      return rc;
+    int shift = 3 * (lmax - lev);
+    k_level_flags<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), shift, s.flag[lev].as<int>());
+    size_t b2 = scratch.bytes;
+    FMMX_HIP(rocprim::exclusive_scan(scratch.p, b2, s.flag[lev].as<int>(), s.pos[lev].as<int>(), 0, (size_t)n,
+                                     rocprim::plus<int>(), stream));
+    k_level_compact<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), shift, s.flag[lev].as<int>(),
+                                                 s.pos[lev].as<int>(), s.bkey[lev].as<uint32_t>(),
+                                                 s.bfirst[lev].as<int>(), cnt + lev);
+  }
+  for (int lev = lmax; lev >= 3; lev--) {
+    k_parent<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, s.bfirst[lev].as<int>(), s.pos[lev - 1].as<int>(),
+                                          s.flag[lev - 1].as<int>(), s.parent[lev].as<int>());
+    k_cfirst<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, cnt + lev - 1, s.parent[lev].as<int>(),
+                                          s.cfirst[lev - 1].as<int>());
+  }
+  k_blast<<<nblk(n), TPB, 0, stream>>>(n, cnt + lmax, s.bfirst[lmax].as<int>(), n, s.blast.as<int>());
+  if (dense) {
+    for (int lev = 2; lev <= lmax; lev++) {
+      long nd = 1L << (3 * lev);
+      if ((rc = s.dense[lev].ensure(nd * 4))) return rc;
+      k_fill_int<<<nblk(nd), TPB, 0, stream>>>(nd, -1, s.dense[lev].as<int>());
+      k_dense<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, s.bkey[lev].as<uint32_t>(), s.dense[lev].as<int>());
+    }
+  }
+  return FMMX_OK;
+}
+
+int Engine::build_lists() {
+  int rc;
+  int nbi = si.nb[lmax];
+  size_t nb1 = (size_t)(nbi > 0 ? nbi : 1);
+  if ((rc = nij.ensure(nb1 * 4)) || (rc = neij.ensure(nb1 * NNEAR * 4))) return rc;
+  DevBuf& ctr = pair_ctr;
+  if ((rc = ctr.ensure(64))) return rc;
+  k_zero_u64<<<1, 32, 0, stream>>>(ctr.as<unsigned long long>(), 8);
+  if (nbi > 0 && sj.nb[lmax] > 0)
+    k_near_list<<<nblk(nbi), TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
+                                               si.bfirst[lmax].as<int>(), si.blast.as<int>(),
+                                               sj.bfirst[lmax].as<int>(), sj.blast.as<int>(), nij.as<int>(),
+                                               neij.as<int>(), ctr.as<unsigned long long>());
+  else if (nbi > 0)
+    k_fill_int<<<nblk(nbi), TPB, 0, stream>>>(nbi, 0, nij.as<int>());
+  return FMMX_OK;
+}
+
+int Engine::build_far_lists() {
+  int rc;
+  for (int lev = 2; lev <= lmax; lev++) {
+    int nbi = si.nb[lev];
+    size_t nb1 = (size_t)(nbi > 0 ? nbi : 1);
+    if ((rc = fnij[lev].ensure(nb1 * 4)) || (rc = fneij[lev].ensure(nb1 * NEBM * 4))) return rc;
+    if (nbi == 0) continue;
+    if (sj.nb[lev] == 0) {
+      k_fill_int<<<nblk(nbi), TPB, 0, stream>>>(nbi, 0, fnij[lev].as<int>());
+      continue;
+    }
+    k_far_list<<<nblk(nbi), TPB, 0, stream>>>(nbi, si.bkey[lev].as<uint32_t>(), sj.nb[lev],
+                                              sj.bkey[lev].as<uint32_t>(), lev, sj.dense[lev].as<int>(),
+                                              fnij[lev].as<int>(), fneij[lev].as<int>());
+  }
+  return FMMX_OK;
+}
+
+int Engine::build_m2l_order() {
+  int rc;
+  for (int lev = 2; lev <= lmax; lev++) {
+    int nb = si.nb[lev];
+    size_t nb1 = (size_t)(nb > 0 ? nb : 1);
+    if ((rc = cls_key[lev].ensure(nb1 * 4)) || (rc = cls_key2[lev].ensure(nb1 * 4)) ||
+        (rc = cls_ord[lev].ensure(nb1 * 4)) || (rc = cls_ord2[lev].ensure(nb1 * 4)) ||
+        (rc = cls_cnt[lev].ensure(8 * 4)))
+      return rc;
+    k_fill_int<<<1, 32, 0, stream>>>(8, 0, cls_cnt[lev].as<int>());
+    if (nb == 0) continue;
+    k_class<<<nblk(nb), TPB, 0, stream>>>(nb, si.bkey[lev].as<uint32_t>(), lev, cls_key2[lev].as<uint32_t>(),
+                                          cls_ord2[lev].as<uint32_t>(), cls_cnt[lev].as<int>());
+    size_t tb = 0;
+    FMMX_HIP(rocprim::radix_sort_pairs(nullptr, tb, cls_key2[lev].as<uint32_t>(), cls_key[lev].as<uint32_t>(),
+                                       cls_ord2[lev].as<uint32_t>(), cls_ord[lev].as<uint32_t>(), (size_t)nb, 0, 3,
+                                       stream));
+    if (tb > scratch.bytes && (rc = scratch.ensure(tb + 256))) return rc;
+    tb = scratch.bytes;
+    FMMX_HIP(rocprim::radix_sort_pairs(scratch.p, tb, cls_key2[lev].as<uint32_t>(), cls_key[lev].as<uint32_t>(),
+                                       cls_ord2[lev].as<uint32_t>(), cls_ord[lev].as<uint32_t>(), (size_t)nb, 0, 3,
+                                       stream));
+  }
+  return FMMX_OK;
+}
+
+}  // namespace fmmx

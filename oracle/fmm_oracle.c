@@ -488,7 +488,9 @@ static void l2l_one(const tables* T, int je, double rh, const cplx* in, cplx* ou
 
 /* kernel/G_l2p.cxx:220-269 for one box: returns unscaled (g, gx, gy, gz) sums
  * through the callback arrays with the 1/(4pi) factor and float rounding. */
-static inline double oround(int conv, double v) { return conv == ORC_CONV_REF ? (double)(float)v : v; }
+/* outputs are float arrays in the reference (include/arrays.h:6): every += rounds */
+static int g_fp64_out = 0;
+static inline double oround(int conv, double v) { return (conv == ORC_CONV_REF && !g_fp64_out) ? (double)(float)v : v; }
 
 static void l2p_G(const tables* T, int i0, int i1, const float* x, const float* y, const float* z,
                   double cx, double cy, double cz, const cplx* aax, int want_grad,
@@ -840,6 +842,7 @@ int orc_run(orc_fmm* f, const float* qj_in, double* vo, double* gxo, double* gyo
   int lmax = o->lmax, mp = o->mp, nmp = mp * (mp + 1) / 2, neq = o->neq, conv = o->conv;
   int ni = f->ni, nj = f->nj;
   float* q = gather_f(nj, qj_in, f->nbj);
+  g_fp64_out = o->fp64_out;
   double* v = (double*)calloc((size_t)ni + 1, sizeof(double));
   double* gx = (double*)calloc((size_t)ni + 1, sizeof(double));
   double* gy = (double*)calloc((size_t)ni + 1, sizeof(double));
@@ -987,6 +990,7 @@ int orc_run(orc_fmm* f, const float* qj_in, double* vo, double* gxo, double* gyo
       l2p_G(T, f->ndi0[ii], f->ndi1[ii], f->xi, f->yi, f->zi, cx, cy, cz, a, neq == ORC_NEQ_G, v, gx, gy, gz);
   }
   f->t[5] = now_s() - t0;
+  g_fp64_out = 0;
 
   /* sort/sorti.cxx:54-66: back to caller order */
   for (int i = 0; i < ni; i++) {

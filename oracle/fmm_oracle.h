@@ -36,6 +36,7 @@ typedef struct orc_opts {
   int conv;      /* ORC_CONV_REF or ORC_CONV_EXACT */
   int nthreads;  /* OpenMP threads (<=0: library default) */
   float xmin, ymin, zmin, rd; /* root cube, float as in include/constants.h:56-59 */
+  int fp64_out;  /* 1: accumulate outputs in double even in ORC_CONV_REF */
 } orc_opts;
 
 typedef struct orc_fmm orc_fmm;

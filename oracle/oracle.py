@@ -28,7 +28,7 @@ _LIB = None
 class _Opts(C.Structure):
     _fields_ = [("mp", C.c_int), ("lmax", C.c_int), ("neq", C.c_int), ("conv", C.c_int),
                 ("nthreads", C.c_int), ("xmin", C.c_float), ("ymin", C.c_float),
-                ("zmin", C.c_float), ("rd", C.c_float)]
+                ("zmin", C.c_float), ("rd", C.c_float), ("fp64_out", C.c_int)]
 
 
 def build() -> str:
@@ -93,13 +93,13 @@ class OracleFMM:
     """Tree + lists built once (as the reference's setup chain), then ``run``."""
 
     def __init__(self, xi, xj, *, mp=10, lmax=3, neq=NEQ_G, conv=REF, box=None, ni_normals=None,
-                 nj_normals=None, nthreads=0):
+                 nj_normals=None, nthreads=0, fp64_out=False):
         xi = np.ascontiguousarray(xi, dtype=np.float32)
         xj = np.ascontiguousarray(xj, dtype=np.float32)
         if box is None:
             box = bounding_cube(np.concatenate([xi, xj]))
         self.box = tuple(np.float32(b) for b in box)
-        o = _Opts(mp, lmax, neq, conv, nthreads, *self.box)
+        o = _Opts(mp, lmax, neq, conv, nthreads, *self.box, int(fp64_out))
         self.mp, self.lmax, self.neq, self.conv = mp, lmax, neq, conv
         self.ni, self.nj = len(xi), len(xj)
         self._keep = [_f(xi[:, k]) for k in range(3)] + [_f(xj[:, k]) for k in range(3)]

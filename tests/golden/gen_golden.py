@@ -117,6 +117,8 @@ def main():
         dst = os.path.join(HERE, f"ref_{name}.npz")
         np.savez_compressed(dst, **keep)
         print(f"{name}: {len(keep)} arrays -> {os.path.relpath(dst, ROOT)} ({os.path.getsize(dst)//1024} KiB)")
+        os.remove(fin)
+        os.remove(fout)
 
 
 if __name__ == "__main__":

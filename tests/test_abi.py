@@ -1,0 +1,152 @@
+"""CPU-side checks of the product: the C-ABI library loads and exports every
+symbol include/*.h declares; the host precompute (dense translation
+operators) reproduces the reference's per-pair translations (via the
+oracle); error paths return status codes instead of aborting."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+import old_fmm_bem_amd as F
+from oracle import oracle as O
+
+
+def _declared(header):
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fmmx\w*)\s*\(", txt)))
+
+
+def test_library_exports_declared_symbols():
+    lib = C.CDLL(F.library_path())
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        names = _declared(h)
+        if os.path.basename(h) == "fmmx.h":
+            assert names
+            for n in names:
+                assert hasattr(lib, n), n
+
+
+def test_refshim_exports_reference_symbols():
+    path = os.path.join(ROOT, "old_fmm_bem_amd", "libfmmx_ref.so")
+    if not os.path.exists(path):
+        pytest.skip("libfmmx_ref.so not built")
+    out = os.popen(f"nm -D --defined-only {path}").read()
+    # C++ symbols of the reference's driver and setup chain (fmm/fmm.cxx:29,
+    # fmm/nlevel.cxx:5, fmm/boxpart.cxx:11, sort/sorti.cxx:12, ...)
+    for mangled in ["_Z3fmmiii", "_Z6nleveli", "_Z7boxpartii", "_Z5sortiRi", "_Z5sortjRi", "_Z11boxallocateiiii",
+                    "_Z5alloci", "_Z7deallocI", "_Z7unsortiRi", "_Z7unsortjRi", "_Z5G_diriiii", "_Z7Gni_diriiii",
+                    "_Z7Gnj_diriiii"]:
+        if mangled == "_Z7deallocI":
+            mangled = "_Z7dealloci"
+        assert mangled in out, mangled
+
+
+def test_abi_version():
+    assert F.lib().fmmx_abi_version() == 1
+
+
+def _c2r(c, p):
+    nr = p * (p + 1) // 2
+    r = np.zeros(p * p)
+    for n in range(p):
+        for m in range(n + 1):
+            s = n * (n + 1) // 2 + m
+            r[s] = c[s].real
+            if m:
+                r[nr + n * (n - 1) // 2 + m - 1] = c[s].imag
+    return r
+
+
+def _r2c(r, p):
+    nr = p * (p + 1) // 2
+    c = np.zeros(nr, complex)
+    for n in range(p):
+        for m in range(n + 1):
+            s = n * (n + 1) // 2 + m
+            c[s] = r[s] + (1j * r[nr + n * (n - 1) // 2 + m - 1] if m else 0)
+    return c
+
+
+def _deg(p):
+    return np.array([n for n in range(p) for m in range(n + 1)])
+
+
+@pytest.mark.parametrize("conv", [O.REF, O.EXACT])
+@pytest.mark.parametrize("p", [9, 10])
+def test_m2l_operator_matches_reference_translation(conv, p):
+    """operator @ M equals the reference's rotate/translate/rotate M2L
+    (kernel/m2l.cxx:43-81, restated in the oracle) for offsets that include
+    the axis-aligned special cases (+z, -z) and the level-dependent eps."""
+    rng = np.random.default_rng(1)
+    pts = ((rng.random((25, 3)) - 0.5) * 0.8).astype(np.float32)
+    q = rng.random(25).astype(np.float32)
+    M = O.p2m_box(p, conv, pts, q, (0, 0, 0))
+    n = _deg(p)
+    for d in [(2, 0, 0), (0, 0, -2), (0, 0, 3), (0, -3, 0), (-3, 2, 1), (2, -2, -2), (3, 3, 3)]:
+        for rb in (1.0, 1.0 / 32):
+            Lref = O.m2l_pair(p, conv, d, rb, M * rb ** n)
+            A = F.operator(p, conv, 2, *d, rb=rb)
+            L = _r2c(A @ _c2r(M, p), p) / rb ** (n + 1)
+            assert np.abs(L - Lref).max() <= 1e-13 * np.abs(Lref).max(), (d, rb)
+
+
+def test_m2m_operator_is_exact_shift():
+    """P2M about the parent centre == M2M(P2M about the child centre): the
+    truncated multipole shift is exact (EXACT convention)."""
+    p = 10
+    rng = np.random.default_rng(2)
+    for oct_ in range(8):
+        cx, cy, cz = (oct_ >> 1) & 1, oct_ & 1, (oct_ >> 2) & 1
+        child = np.array([cx - 0.5, cy - 0.5, cz - 0.5]) * 0.5  # child centre, parent box size 1
+        pts = (child + (rng.random((30, 3)) - 0.5) * 0.5).astype(np.float32)
+        q = rng.random(30).astype(np.float32)
+        Mc = O.p2m_box(p, O.EXACT, pts, q, child)
+        Mp = O.p2m_box(p, O.EXACT, pts, q, (0, 0, 0))
+        A = F.operator(p, O.EXACT, 0, oct_)
+        n = _deg(p)
+        got = _r2c(A @ _c2r(Mc / 0.5 ** n, p), p)  # scaled child (size 1/2) -> parent (size 1)
+        assert np.abs(got - Mp).max() < 1e-13 * np.abs(Mp).max(), oct_
+
+
+def test_l2l_operator_is_exact_shift():
+    """Evaluating a local expansion at points through L2L to the child equals
+    evaluating the parent expansion directly (EXACT convention)."""
+    p = 10
+    rng = np.random.default_rng(3)
+    L = rng.standard_normal(p * (p + 1) // 2) + 1j * rng.standard_normal(p * (p + 1) // 2)
+    for n in range(p):
+        L[n * (n + 1) // 2] = L[n * (n + 1) // 2].real
+    n = _deg(p)
+    for oct_ in range(8):
+        cx, cy, cz = (oct_ >> 1) & 1, oct_ & 1, (oct_ >> 2) & 1
+        child = np.array([cx - 0.5, cy - 0.5, cz - 0.5]) * 0.5
+        pts = (child + (rng.random((20, 3)) - 0.5) * 0.5).astype(np.float32)
+        A = F.operator(p, O.EXACT, 1, oct_)
+        Lc = _r2c(A @ _c2r(L, p), p) / 0.5 ** (n + 1)  # scaled parent (size 1) -> child (size 1/2)
+        v0, g0 = O.l2p_box(p, O.EXACT, pts, (0, 0, 0), L)
+        v1, g1 = O.l2p_box(p, O.EXACT, pts, child, Lc)
+        assert np.abs(v1 - v0).max() < 1e-12 * np.abs(v0).max()
+        assert np.abs(g1 - g0).max() < 1e-11 * np.abs(g0).max()
+
+
+def test_operator_rejects_bad_arguments():
+    A = np.zeros((100, 100))
+    assert F.lib().fmmx_operator(10, 1, 2, 1, 0, 0, 1.0, A.ctypes.data) == 1  # adjacent offset
+    assert F.lib().fmmx_operator(10, 1, 2, 4, 0, 0, 1.0, A.ctypes.data) == 1  # beyond [-3,3]
+    assert F.lib().fmmx_operator(99, 1, 0, 0, 0, 0, 1.0, A.ctypes.data) == 1  # p out of range
+    assert F.lib().fmmx_operator(10, 1, 7, 0, 0, 0, 1.0, A.ctypes.data) == 1  # bad kind
+    assert "kind" in F.lib().fmmx_last_error().decode()
+
+
+def test_create_rejects_bad_config():
+    for kw in [dict(p=1), dict(p=13), dict(lmax=9), dict(precision=7), dict(convention=5)]:
+        cfg = F._Config(kw.get("p", 10), kw.get("lmax", 3), kw.get("precision", 0), kw.get("convention", 0), 0, 0, 0,
+                        0)
+        h = C.c_void_p()
+        rc = F.lib().fmmx_create(C.byref(cfg), 0, None, C.byref(h))
+        assert rc == 1 and not h.value, kw

@@ -1,0 +1,193 @@
+"""The oracle (CPU FP64 restatement) pinned against the reference itself
+(golden vectors written by the compiled reference, tests/golden/) and against
+closed forms, invariants and brute force for what the reference leaves open
+(EXACT convention, p = 10 and 12)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, golden, rel_l2, uniform_cube
+from oracle import oracle as O
+
+
+def _oracle_for(d, **kw):
+    ni, nj, lmax, neq, same, asis, seed = (int(v) for v in d["in_cfg"])
+    return O.OracleFMM(d["in_xi"], d["in_xj"], mp=int(d["meta"][0]), lmax=lmax, neq=neq, conv=O.REF,
+                       box=tuple(d["in_box"]), ni_normals=d["in_ni"], nj_normals=d["in_nj"], nthreads=4, **kw)
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_golden_tree_and_lists(case):
+    d = golden(case)
+    f = _oracle_for(d)
+    lmax = int(d["in_cfg"][2])
+    nbi, nbj = f.perm()
+    assert np.array_equal(nbi, d["nbi"]) and np.array_equal(nbj, d["nbj"])
+    k, a, b = f.boxes("i", lmax)
+    assert np.array_equal(k, d["leaf_nfi"]) and np.array_equal(a, d["leaf_ndi0"]) and np.array_equal(b, d["leaf_ndi1"])
+    k, a, b = f.boxes("j", lmax)
+    assert np.array_equal(k, d["leaf_nfj"]) and np.array_equal(a, d["leaf_ndj0"]) and np.array_equal(b, d["leaf_ndj1"])
+    # all levels, stacked leaf first as nfi/nfj
+    ki = np.concatenate([f.boxes("i", l)[0] for l in range(lmax, 1, -1)])
+    kj = np.concatenate([f.boxes("j", l)[0] for l in range(lmax, 1, -1)])
+    assert np.array_equal(ki, d["all_nfi"]) and np.array_equal(kj, d["all_nfj"])
+    nij, neij = f.interaction_list(lmax, 0)
+    assert np.array_equal(nij, d[f"near_nij_{lmax}"]) and np.array_equal(neij, d[f"near_neij_{lmax}"])
+    for lev in range(2, lmax + 1):
+        nij, neij = f.interaction_list(lev, 1)
+        assert np.array_equal(nij, d[f"far_nij_{lev}"]), lev
+        assert np.array_equal(neij, d[f"far_neij_{lev}"]), lev  # values and order
+    f.close()
+
+
+def test_golden_tables():
+    d = golden("g3")
+    t = O.precalc(9, O.REF)
+    assert np.array_equal(t["fac"], d["fac"].astype(np.float64))
+    assert np.array_equal(t["anm"], d["anm"].astype(np.float64))
+    assert np.array_equal(t["sr"], d["sr"].astype(np.float64))
+    assert np.array_equal(t["ynm"].view(np.float64), d["ynm"])
+    ds = d["dnm_sample"].view(np.complex128).reshape(8, 2, 9, 81)
+    for s, i in enumerate(d["dnm_sample_idx"]):
+        assert np.array_equal(t["dnm"][i], ds[s, 0]) and np.array_equal(t["dnm"][i + 512], ds[s, 1])
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_golden_expansions_and_outputs(case):
+    d = golden(case)
+    f = _oracle_for(d)
+    lmax, neq = int(d["in_cfg"][2]), int(d["in_cfg"][3])
+    v, g = f.run(d["in_q"])
+    assert np.array_equal(v.astype(np.float32), d["fmm_vi"])  # float outputs, bit for bit
+    if neq == 0:
+        for c, k in enumerate(["gxi", "gyi", "gzi"]):
+            assert np.array_equal(g[:, c].astype(np.float32), d["fmm_" + k])
+    if "up_bx" in d:
+        assert np.array_equal(f.bx().ravel().view(np.float64), d["up_bx"])
+        for lev in range(2, lmax + 1):
+            assert np.array_equal(f.ax_level(lev).ravel().view(np.float64), d[f"ax_{lev}"]), lev
+    dv, dg = O.direct(d["in_xi"], d["in_xj"], d["in_q"], neq=neq, conv=O.REF, ni_normals=d["in_ni"],
+                      nj_normals=d["in_nj"])
+    assert np.array_equal(dv.astype(np.float32), d["dir_vi"])
+    # the reference FMM at mp=9 agrees with its direct sum on these cubes
+    assert rel_l2(d["fmm_vi"], d["dir_vi"]) < 1e-4
+    f.close()
+
+
+def test_tables_closed_forms():
+    p = 10
+    t = O.precalc(p, O.EXACT)
+    for n in range(2 * p):
+        for m in range(-n, n + 1):
+            nm = n * n + n + m
+            assert t["fac"][nm] == pytest.approx(math.sqrt(math.factorial(n - abs(m)) / math.factorial(n + abs(m))),
+                                                 rel=1e-14)
+            assert t["anm"][nm] == pytest.approx((-1) ** n / math.sqrt(math.factorial(n - m) * math.factorial(n + m)),
+                                                 rel=1e-14)
+            # Y_n^m at theta = 0: P_n^m(1) = delta_m0
+            assert abs(t["ynm"][nm] - (1.0 if m == 0 else 0.0)) < 1e-14
+
+
+def _spharot(y, d, p):
+    out = np.zeros_like(y)
+    for n in range(p):
+        for m in range(n + 1):
+            s = 0j
+            for k in range(-n, 0):
+                s += d[m, n * (n + 1) + k] * np.conj(y[n * (n + 1) // 2 - k])
+            for k in range(0, n + 1):
+                s += d[m, n * (n + 1) + k] * y[n * (n + 1) // 2 + k]
+            out[n * (n + 1) // 2 + m] = s
+    return out
+
+
+@pytest.mark.parametrize("conv", [O.REF, O.EXACT])
+def test_rotation_then_inverse_is_identity(conv):
+    """Wigner rotations are unitary: the inverse table undoes the forward one
+    for every lattice direction used by M2M, L2L and M2L."""
+    p = 6
+    t = O.precalc(p, conv)
+    rng = np.random.default_rng(3)
+    nr = p * (p + 1) // 2
+    # expansion of a real field: m = 0 coefficients real
+    y = rng.standard_normal(nr) + 1j * rng.standard_normal(nr)
+    for n in range(p):
+        y[n * (n + 1) // 2] = y[n * (n + 1) // 2].real
+    for i in [0, 7, 73, 100, 219, 292, 300, 511, 27, 457]:
+        z = _spharot(_spharot(y, t["dnm"][i], p), t["dnm"][i + 512], p)
+        assert np.abs(z - y).max() < 1e-12 * np.abs(y).max(), i
+
+
+def test_direct_single_charge_closed_form():
+    x = np.array([[0.1, 0.2, 0.3]], np.float32)
+    y = np.array([[0.7, -0.4, 0.9]], np.float32)
+    q = np.array([0.75], np.float32)
+    v, g = O.direct(x, y, q, conv=O.EXACT)
+    d = x.astype(np.float64)[0] - y.astype(np.float64)[0]
+    r = np.linalg.norm(d)
+    assert v[0] == pytest.approx(0.75 / (4 * math.pi * r), rel=1e-14)
+    assert np.allclose(g[0], -0.75 * d / (4 * math.pi * r ** 3), rtol=1e-13)
+    # the reference's softening: r^2 + 1e-6 (float), self pair included
+    v2, _ = O.direct(x, x, q, conv=O.REF)
+    assert v2[0] == pytest.approx(0.75 / (4 * float(np.float32(math.pi)) * math.sqrt(float(np.float32(1e-6)))),
+                                  rel=1e-6)
+
+
+@pytest.mark.parametrize("p,tol", [(6, 2e-3), (10, 3e-5), (12, 1e-5)])
+def test_exact_fmm_vs_direct(p, tol):
+    x, q = uniform_cube(6000, 1)
+    v, g = O.fmm(x, x, q, mp=p, lmax=3, conv=O.EXACT, box=(0, 0, 0, 1), nthreads=4)
+    dv, dg = O.direct(x, x, q, conv=O.EXACT)
+    assert rel_l2(v, dv) < tol
+    assert rel_l2(g, dg) < 5 * tol
+
+
+def test_error_decreases_with_p():
+    x, q = uniform_cube(4000, 2)
+    dv, _ = O.direct(x, x, q, conv=O.EXACT)
+    errs = [rel_l2(O.fmm(x, x, q, mp=p, lmax=3, conv=O.EXACT, box=(0, 0, 0, 1))[0], dv) for p in (3, 5, 7, 9, 11)]
+    assert all(b < a for a, b in zip(errs, errs[1:])), errs
+
+
+def test_linearity_and_symmetry():
+    x, q1 = uniform_cube(3000, 4)
+    _, q2 = uniform_cube(3000, 5)
+    kw = dict(mp=8, lmax=3, conv=O.EXACT, box=(0, 0, 0, 1))
+    a, _ = O.fmm(x, x, q1, **kw)
+    b, _ = O.fmm(x, x, q2, **kw)
+    c, _ = O.fmm(x, x, (q1 + 2 * q2).astype(np.float32), **kw)
+    assert rel_l2(c, a + 2 * b) < 1e-6  # float rounding of q1 + 2 q2
+    # G symmetric: <q1, G q2> = <q2, G q1>
+    assert abs(q1 @ b - q2 @ a) < 1e-6 * abs(q1 @ b)
+
+
+def test_gni_gnj_adjoint():
+    """<u, Gnj v> = -<v, Gni u> for the double-layer kernel and its adjoint
+    (targets/sources swapped, normals on the same points)."""
+    rng = np.random.default_rng(6)
+    x, u = uniform_cube(2500, 6)
+    nrm = rng.standard_normal((2500, 3)).astype(np.float32)
+    v = rng.random(2500).astype(np.float32)
+    kw = dict(mp=10, lmax=3, conv=O.EXACT, box=(0, 0, 0, 1))
+    gnj, _ = O.fmm(x, x, v, neq=O.NEQ_GNJ, nj_normals=nrm, **kw)
+    gni, _ = O.fmm(x, x, u, neq=O.NEQ_GNI, ni_normals=nrm, **kw)
+    lhs, rhs = float(u @ gnj), -float(v @ gni)
+    assert abs(lhs - rhs) < 1e-4 * abs(lhs)
+    dj, _ = O.direct(x, x, v, neq=O.NEQ_GNJ, nj_normals=nrm, conv=O.EXACT)
+    di, _ = O.direct(x, x, u, neq=O.NEQ_GNI, ni_normals=nrm, conv=O.EXACT)
+    assert abs(float(u @ dj) + float(v @ di)) < 1e-10 * abs(float(u @ dj))
+
+
+def test_multipole_translation_consistency():
+    """P2M about a shifted centre equals M2M of the child expansion (exact for
+    the truncated series), checked through the oracle's own M2L+L2P: two
+    far-apart clusters evaluated by the FMM equal the direct sum closely."""
+    rng = np.random.default_rng(8)
+    a = (np.array([0.1, 0.1, 0.1]) + 0.05 * rng.random((50, 3))).astype(np.float32)
+    b = (np.array([0.8, 0.85, 0.9]) + 0.05 * rng.random((50, 3))).astype(np.float32)
+    x = np.concatenate([a, b])
+    q = rng.random(100).astype(np.float32)
+    v, _ = O.fmm(x, x, q, mp=12, lmax=4, conv=O.EXACT, box=(0, 0, 0, 1))
+    dv, _ = O.direct(x, x, q, conv=O.EXACT)
+    assert rel_l2(v, dv) < 1e-7
