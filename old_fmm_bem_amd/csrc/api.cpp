@@ -77,7 +77,9 @@ int nlevel(int nmax) {
 
 // process-wide cache of host operators keyed by (p, conv, lmax, rd)
 struct OpSet {
-  std::vector<double> m2m, l2l, m2l;  // transposed [op][k][row]
+  std::vector<double> m2m, l2l;  // transposed [op][k][row]
+  std::vector<double> m2l_frag;  // MFMA fragment order (FP32 engine)
+  std::vector<double> m2l_kmaj;  // transposed [op][k][row] (FP64 engine)
   int nlev = 1;
 };
 std::mutex g_cache_mu;
@@ -97,6 +99,18 @@ std::shared_ptr<Tables> get_tables(int p, int conv) {
 }
 
 namespace {
+
+// MFMA 16x16x4 fragment order of a row-major P2 x P2 operator: [ks][mt][lane]
+// holds A[16 mt + lane%16][(lane/16) * KS + ks] (zero outside)
+void fragment_into(const double* A, int P2, double* dst) {
+  const int MT = (P2 + 15) / 16, KS = (P2 + 3) / 4;
+  for (int ks = 0; ks < KS; ks++)
+    for (int mt = 0; mt < MT; mt++)
+      for (int l = 0; l < 64; l++) {
+        const int i = 16 * mt + (l & 15), k = (l >> 4) * KS + ks;
+        dst[((size_t)ks * MT + mt) * 64 + l] = (i < P2 && k < P2) ? A[(size_t)i * P2 + k] : 0.0;
+      }
+}
 
 void transpose_into(const double* A, int P2, double* dst) {
   for (int r = 0; r < P2; r++)
@@ -124,7 +138,9 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
     transpose_into(A.data(), P2, &S->l2l[o * sq]);
   }
   S->nlev = per_level ? lmax - 1 : 1;
-  S->m2l.resize((size_t)S->nlev * NOFF * sq);
+  const size_t fr = (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64;
+  S->m2l_frag.resize((size_t)S->nlev * NOFF * fr);
+  S->m2l_kmaj.resize((size_t)S->nlev * NOFF * sq);
   const int njobs = S->nlev * NOFF;
   // parallel over (level, offset)
   std::vector<std::thread> th;
@@ -137,7 +153,8 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
         offset_vector(oi, d);
         double rbv = per_level ? (double)rd / (double)(1 << (li + 2)) : 1.0;
         op_m2l(T, d[0], d[1], d[2], rbv, B.data());
-        transpose_into(B.data(), P2, &S->m2l[((size_t)li * NOFF + oi) * sq]);
+        fragment_into(B.data(), P2, &S->m2l_frag[((size_t)li * NOFF + oi) * fr]);
+        transpose_into(B.data(), P2, &S->m2l_kmaj[((size_t)li * NOFF + oi) * sq]);
       }
     });
   for (auto& x : th) x.join();
@@ -210,11 +227,11 @@ int Engine::upload_operators() {
   int rc;
   if (fp64) {
     if ((rc = upload<double>(op_m2m, S->m2m, stream)) || (rc = upload<double>(op_l2l, S->l2l, stream)) ||
-        (rc = upload<double>(op_m2l, S->m2l, stream)))
+        (rc = upload<double>(op_m2l, S->m2l_kmaj, stream)))
       return rc;
   } else {
     if ((rc = upload<float>(op_m2m, S->m2m, stream)) || (rc = upload<float>(op_l2l, S->l2l, stream)) ||
-        (rc = upload<float>(op_m2l, S->m2l, stream)))
+        (rc = upload<float>(op_m2l, S->m2l_frag, stream)))
       return rc;
   }
   ops_lmax = lmax;

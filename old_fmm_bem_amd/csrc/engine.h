@@ -77,7 +77,7 @@ struct Engine {
   // M2L target ordering by parity class and class counts per level
   std::array<DevBuf, MAXL + 1> cls_key, cls_key2, cls_ord, cls_ord2, cls_cnt;
   // operators (precision of the evaluation): M2M[8], L2L[8], M2L[levels][NOFF]
-  DevBuf op_m2m, op_l2l, op_m2l;
+  DevBuf op_m2m, op_l2l, op_m2l, m2l_part;  // op_m2l in MFMA fragment order
   bool m2l_per_level = false;
   DevBuf offtab;  // int4 (dx,dy,dz,opidx) per (class, 189)
   DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)

@@ -12,6 +12,8 @@
 //        lattice offsets, so a tile of targets runs as 189 GEMMs
 //        (operator [p^2 x p^2] x gathered source multipoles [p^2 x tile]).
 //   direct sums kernel/G_dir.cxx, Gni_dir.cxx, Gnj_dir.cxx.
+#include <cstdlib>
+
 #include "engine.h"
 #include "tree_util.h"
 
@@ -346,29 +348,166 @@ __global__ __launch_bounds__(128) void k_l2l(int nbc, int P2, const int* __restr
 }
 
 // ---------------------------------------------------------------------
-// M2L: a workgroup owns TN targets of one parity class at a level and loops
-// over the class's 189 offsets; per offset the operator (K-chunked) and the
-// gathered source multipoles go through LDS and every thread accumulates a
-// RPT x 4 register block of (rows x targets).
-constexpr int M2L_TN = 64, M2L_KC = 32, M2L_BST = M2L_TN + 4;
+// M2L on MFMA.  A workgroup (4 waves) owns a tile of 64 targets of one
+// parity class at a level; all of them see the same 189 lattice offsets
+// (fmm/ijbox.cxx:366-401).  Per offset the operator (p^2 x p^2, stored by the
+// host in MFMA fragment order) is staged once in LDS and each wave runs
+// ceil(p^2/16) x ceil(p^2/4) v_mfma 16x16x4 steps for its 16 targets, the
+// source multipoles (gathered through the dense key->box table) feeding the
+// B operand straight from registers.  With nsplit > 1 the offsets are split
+// over workgroups and the partial sums reduced by k_m2l_reduce in fixed
+// order (deterministic).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  using v4 = f32x4;
+  static __device__ __forceinline__ v4 fma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<double> {
+  using v4 = f64x4;
+  static __device__ __forceinline__ v4 fma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
 
-template <int P2, typename T>
-__global__ __launch_bounds__(256) void k_m2l(int lev, const int* __restrict__ cls_cnt, const uint32_t* __restrict__ ord,
-                                             const uint32_t* __restrict__ ikey, const int* __restrict__ dense_j,
-                                             const T* __restrict__ M, T* __restrict__ L, const T* __restrict__ opT,
+constexpr int M2L_TN = 64;
+
+template <int P, typename T>
+__global__ __launch_bounds__(256) void k_m2l(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
+                                             const uint32_t* __restrict__ ord, const uint32_t* __restrict__ ikey,
+                                             const int* __restrict__ dense_j, const T* __restrict__ M,
+                                             T* __restrict__ L, T* __restrict__ part, const T* __restrict__ opF,
                                              const int4* __restrict__ offtab, int init,
                                              unsigned long long* __restrict__ npairs) {
-  constexpr int RPT = (P2 + 15) / 16;
-  __shared__ T As[M2L_KC][P2];
-  __shared__ T Bs[M2L_KC][M2L_BST];
+  constexpr int P2 = P * P, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, FR = KS * MT * 64;
+  // k-steps staged per pass: the whole operator unless it exceeds 80 KB
+  constexpr int KC = (KS * MT * 64 * (int)sizeof(T) <= 81920) ? KS : (KS + 1) / 2;
+  using V4 = typename Mfma<T>::v4;
+  __shared__ __align__(16) T As[KC * MT * 64];
+  __shared__ int s_class, s_start, s_n;
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.x / nsplit, split = blockIdx.x % nsplit;
+  if (tid == 0) {
+    int w = tile, start = 0, cls = -1, n = 0;
+    for (int c = 0; c < 8; c++) {
+      const int cnt = cls_cnt[c], nt = (cnt + M2L_TN - 1) / M2L_TN;
+      if (cls < 0 && w < nt) {
+        cls = c;
+        n = min(M2L_TN, cnt - w * M2L_TN);
+        start += w * M2L_TN;
+      } else if (cls < 0) {
+        w -= nt;
+        start += cnt;
+      }
+    }
+    s_class = cls;
+    s_start = start;
+    s_n = n;
+  }
+  __syncthreads();
+  const int cls = s_class, ntile = s_n;
+  if (cls < 0) return;
+  const int w = tid >> 6, l = tid & 63, j = l & 15, kq = l >> 4;
+  const int t = 16 * w + j;  // this lane's target (B column / C column)
+  const bool wave_on = 16 * w < ntile;
+  const int g = 1 << lev;
+  int tord = -1, tx = 0, ty = 0, tz = 0;
+  if (t < ntile) {
+    tord = (int)ord[s_start + t];
+    morton_decode(ikey[tord], lev, tx, ty, tz);
+  }
+  const int per = (NEBM + nsplit - 1) / nsplit, o0 = split * per, o1 = min(NEBM, o0 + per);
+  V4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) acc[mt] = V4{T(0), T(0), T(0), T(0)};
+  unsigned long long mypairs = 0;
+  for (int o = o0; o < o1; o++) {
+    const int4 off = offtab[cls * NEBM + o];
+    // this lane's B values: multipole slots kq*KS + ks of the source box
+    int s = -1;
+    if (tord >= 0) {
+      const int jx = tx - off.x, jy = ty - off.y, jz = tz - off.z;
+      if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g) s = dense_j[morton_encode(jx, jy, jz, lev)];
+    }
+    T b[KS];
+    const T* ms = M + (size_t)(s >= 0 ? s : 0) * P2 + kq * KS;
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) b[ks] = (s >= 0 && kq * KS + ks < P2) ? ms[ks] : T(0);
+    if (s >= 0 && kq == 0) mypairs++;
+#pragma unroll
+    for (int k0 = 0; k0 < KS; k0 += KC) {
+      constexpr int NVC = KC * MT * 64 * (int)sizeof(T) / 16;
+      const int nv = (min(KC, KS - k0) * MT * 64 * (int)sizeof(T)) / 16;
+      __syncthreads();
+      {
+        // operator fragments -> LDS, 16-byte copies
+        const float4* src = reinterpret_cast<const float4*>(opF + (size_t)off.w * FR + (size_t)k0 * MT * 64);
+        float4* dst = reinterpret_cast<float4*>(As);
+        for (int e = tid; e < nv && e < NVC; e += 256) dst[e] = src[e];
+      }
+      __syncthreads();
+      if (wave_on) {
+#pragma unroll
+        for (int kk = 0; kk < KC; kk++) {
+          const int ks = k0 + kk;
+          if (ks < KS) {
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) acc[mt] = Mfma<T>::fma(As[(kk * MT + mt) * 64 + l], b[ks], acc[mt]);
+          }
+        }
+      }
+    }
+  }
+  if (tord >= 0) {
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * mt + 4 * kq + r;
+        if (i < P2) {
+          const T v = acc[mt][r];
+          if (nsplit == 1) {
+            T* dst = L + (size_t)tord * P2 + i;
+            *dst = init ? v : *dst + v;
+          } else {
+            part[((size_t)split * nb + tord) * P2 + i] = v;
+          }
+        }
+      }
+  }
+  if (mypairs) atomicAdd(npairs, mypairs);
+}
+
+// FP64 M2L on the vector FMA units (MI355X: FP64 vector peak = FP64 matrix
+// peak).  Same tiles and offset split as k_m2l; per offset the operator
+// (k-major) and the gathered source multipoles go through LDS in K-chunks
+// and each thread accumulates a (rows x 4 targets) register block.
+constexpr int M2LV_KC = 32, M2LV_BST = M2L_TN + 4;
+
+template <int P, typename T>
+__global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
+                                                 const uint32_t* __restrict__ ord, const uint32_t* __restrict__ ikey,
+                                                 const int* __restrict__ dense_j, const T* __restrict__ M,
+                                                 T* __restrict__ L, T* __restrict__ part, const T* __restrict__ opT,
+                                                 const int4* __restrict__ offtab, int init,
+                                                 unsigned long long* __restrict__ npairs) {
+  constexpr int P2 = P * P, RPT = (P2 + 15) / 16;
+  __shared__ T As[M2LV_KC][P2];
+  __shared__ T Bs[M2LV_KC][M2LV_BST];
   __shared__ int tpos[M2L_TN][3];
   __shared__ int tord[M2L_TN];
   __shared__ int src[M2L_TN];
   __shared__ int s_class, s_start, s_n;
   const int tid = threadIdx.x;
+  const int tile = blockIdx.x / nsplit, split = blockIdx.x % nsplit;
   if (tid == 0) {
-    // find this workgroup's (class, tile)
-    int w = blockIdx.x, start = 0, cls = -1, n = 0;
+    int w = tile, start = 0, cls = -1, n = 0;
     for (int c = 0; c < 8; c++) {
       const int cnt = cls_cnt[c], nt = (cnt + M2L_TN - 1) / M2L_TN;
       if (cls < 0 && w < nt) {
@@ -406,7 +545,8 @@ __global__ __launch_bounds__(256) void k_m2l(int lev, const int* __restrict__ cl
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[r][c] = T(0);
   unsigned long long mypairs = 0;
-  for (int oi = 0; oi < NEBM; oi++) {
+  const int per = (NEBM + nsplit - 1) / nsplit, o0 = split * per, o1 = min(NEBM, o0 + per);
+  for (int oi = o0; oi < o1; oi++) {
     const int4 off = offtab[cls * NEBM + oi];
     __syncthreads();
     if (tid < M2L_TN) {
@@ -419,8 +559,8 @@ __global__ __launch_bounds__(256) void k_m2l(int lev, const int* __restrict__ cl
       if (s >= 0) mypairs++;
     }
     const T* A = opT + (size_t)off.w * P2 * P2;
-    for (int k0 = 0; k0 < P2; k0 += M2L_KC) {
-      const int kc = min(M2L_KC, P2 - k0);
+    for (int k0 = 0; k0 < P2; k0 += M2LV_KC) {
+      const int kc = min(M2LV_KC, P2 - k0);
       __syncthreads();
       for (int e = tid; e < kc * P2; e += 256) As[e / P2][e % P2] = A[(size_t)k0 * P2 + e];
       for (int e = tid; e < kc * M2L_TN; e += 256) {
@@ -446,15 +586,34 @@ __global__ __launch_bounds__(256) void k_m2l(int lev, const int* __restrict__ cl
   for (int c = 0; c < 4; c++) {
     const int t = tx * 4 + c;
     if (t >= ntile) continue;
-    T* dst = L + (size_t)tord[t] * P2;
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       const int row = ty + 16 * r;
-      if (row < P2) dst[row] = init ? acc[r][c] : dst[row] + acc[r][c];
+      if (row >= P2) continue;
+      if (nsplit == 1) {
+        T* dst = L + (size_t)tord[t] * P2 + row;
+        *dst = init ? acc[r][c] : *dst + acc[r][c];
+      } else {
+        part[((size_t)split * nb + tord[t]) * P2 + row] = acc[r][c];
+      }
     }
   }
   if (tid < M2L_TN && mypairs) atomicAdd(npairs, mypairs);
 }
+
+template <typename T>
+__global__ void k_m2l_reduce(long n, int nsplit, const T* __restrict__ part, T* __restrict__ L, int init) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  T s = init ? T(0) : L[e];
+  for (int k = 0; k < nsplit; k++) s += part[(size_t)k * n + e];
+  L[e] = s;
+}
+
+// ---------------------------------------------------------------------
+// reciprocal square root: v_rsq_f32 (1 ulp) in FP32, full precision in FP64
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
 
 // ---------------------------------------------------------------------
 // P2P: one workgroup per target leaf box (chunks of BS targets), sources of
@@ -475,9 +634,10 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
   T r2 = dx * dx + dy * dy + dz * dz;
   T inv;
   if (CONV == FMMX_CONV_REF) {
-    inv = rsqrt(r2 + T(EPSF));
+    inv = rsq(r2 + T(EPSF));
   } else {
-    inv = r2 > T(0) ? rsqrt(r2) : T(0);
+    inv = rsq(r2);
+    inv = r2 > T(0) ? inv : T(0);  // self pair excluded (select, no branch)
   }
   const T qi = T(s.w) * inv;
   if (NEQ == FMMX_G) {
@@ -765,31 +925,46 @@ int Engine::run_l2l(int lev) {
 int Engine::run_m2l(int lev, bool init) {
   const int nb = si.nb[lev];
   if (nb == 0) return FMMX_OK;
-  const size_t opstride = (size_t)NOFF * P2 * P2;
-  const size_t opoff = m2l_per_level ? (size_t)(lev - 2) * opstride : 0;
-  const int grid = (nb + M2L_TN - 1) / M2L_TN + 8;
+  const long nl = (long)nb * P2;
   if (sj.nb[lev] == 0) {
     if (init) {
-      long n = (long)nb * P2;
-      if (fp64) k_zero_t<double><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, lexp[lev].as<double>());
-      else k_zero_t<float><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, lexp[lev].as<float>());
+      if (fp64) k_zero_t<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, lexp[lev].as<double>());
+      else k_zero_t<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, lexp[lev].as<float>());
     }
     return FMMX_OK;
   }
+  const int MT = (P2 + 15) / 16, KS = (P2 + 3) / 4;
+  // FP32: MFMA fragment order; FP64: k-major (transposed) operators
+  const size_t FR = fp64 ? (size_t)P2 * P2 : (size_t)KS * MT * 64;
+  const size_t opoff = m2l_per_level ? (size_t)(lev - 2) * NOFF * FR : 0;
+  const int tiles = (nb + M2L_TN - 1) / M2L_TN + 8;
+  // enough workgroups to cover the chip: split the 189 offsets when tiles are few
+  int nsplit = std::max(1, std::min(16, 1024 / tiles));
+  if (const char* e = getenv("FMMX_M2L_SPLIT")) nsplit = std::max(1, std::min(16, atoi(e)));
+  const size_t es = fp64 ? 8 : 4;
+  if (nsplit > 1) {
+    int rc = m2l_part.ensure((size_t)nsplit * nl * es);
+    if (rc) return rc;
+  }
   return dispatch_p(p, [&](auto pc) -> int {
-    constexpr int P = decltype(pc)::value, Q = P * P;
-    if (fp64)
-      k_m2l<Q, double><<<grid, 256, 0, stream>>>(lev, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(),
-                                                 si.bkey[lev].as<uint32_t>(), sj.dense[lev].as<int>(),
-                                                 mexp[lev].as<double>(), lexp[lev].as<double>(),
-                                                 op_m2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0,
-                                                 pair_ctr.as<unsigned long long>() + 1);
-    else
-      k_m2l<Q, float><<<grid, 256, 0, stream>>>(lev, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(),
-                                                si.bkey[lev].as<uint32_t>(), sj.dense[lev].as<int>(),
-                                                mexp[lev].as<float>(), lexp[lev].as<float>(),
-                                                op_m2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0,
-                                                pair_ctr.as<unsigned long long>() + 1);
+    constexpr int P = decltype(pc)::value;
+    if (fp64) {
+      k_m2l_vec<P, double><<<tiles * nsplit, 256, 0, stream>>>(
+          lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
+          sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), m2l_part.as<double>(),
+          op_m2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 1);
+      if (nsplit > 1)
+        k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<double>(),
+                                                                          lexp[lev].as<double>(), init ? 1 : 0);
+    } else {
+      k_m2l<P, float><<<tiles * nsplit, 256, 0, stream>>>(
+          lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
+          sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), m2l_part.as<float>(),
+          op_m2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 1);
+      if (nsplit > 1)
+        k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<float>(),
+                                                                         lexp[lev].as<float>(), init ? 1 : 0);
+    }
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
