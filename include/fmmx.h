@@ -112,6 +112,20 @@ int fmmx_direct(fmmx_handle* h, int neq, int ni, const float* xi, const float* y
                 const float* nzj, const float* qj, double* vi, double* gxi, double* gyi,
                 double* gzi, int flags);
 
+/* Replace the root cube and leaf level fixed at fmmx_create (rd <= 0: from
+ * the points at each set_points; lmax 0: fmm/nlevel.cxx thresholds).  The
+ * reference reads them from globals at every call (include/constants.h:56-59,
+ * fmm/nlevel.cxx:11); takes effect at the next set_points. */
+int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin, float rd);
+
+/* Morton keys of n points at the leaf level and their stable sort
+ * (fmm/boxn.cxx:5-31 then sort/csort.cxx:5-17, as sort/sorti.cxx:12-26
+ * runs them): perm[k] = caller index of the k-th point in key order (the
+ * reference's nbi/nbj), keys[k] = its key (may be NULL).  Needs an explicit
+ * root cube and lmax (fmmx_set_domain or the config).  Synchronous. */
+int fmmx_sort(fmmx_handle* h, int n, const float* x, const float* y, const float* z, int* keys, int* perm,
+              int flags);
+
 /* Wait for the handle's stream. */
 int fmmx_synchronize(fmmx_handle* h);
 

@@ -1,0 +1,78 @@
+/*
+ * fmmx_refshim.h -- the reference's own C++ entry points, served by the
+ * MI355X engine (libfmmx_ref.so, linked against libfmmx.so).
+ *
+ * A caller written against the reference (test/test.cxx, bio/main.cxx)
+ * links against libfmmx_ref.so instead of the reference's fmm/, sort/ and
+ * kernel/ objects and runs unchanged: the symbols below have the
+ * reference's names, C++ signatures and side effects on the caller's global
+ * arrays.  As in the reference, the globals themselves are DEFINED by the
+ * caller (include/arrays.h and include/constants.h compiled with MAIN);
+ * the shim only references the ones listed at the end of this header.
+ *
+ * Differences from the reference, by design:
+ *  - the FMM runs on the GPU through one process-wide fmmx handle (device 0
+ *    unless configured), FP64 arithmetic and the reference's conventions
+ *    (softened kernel, self pair included) by default; expansion order p is
+ *    9 as in include/constants.h:12 unless configured;
+ *  - boxpart() fills nek/irank (used only by the MPI partitioner) but the
+ *    tree lists, expansions and tables of alloc(1) stay inside the engine;
+ *  - fmm(mi, mj, 2) leaves gxi, gyi, gzi untouched (the reference adds far
+ *    field gradients into them, fmm/fmm.cxx:145, on top of the caller's
+ *    values);
+ *  - Yukawa (neq 10) is not provided;
+ *  - errors print a message and exit(1), as the reference's CUDA error path
+ *    does (cutil-mp.h:776-787).
+ */
+#ifndef FMMX_REFSHIM_H
+#define FMMX_REFSHIM_H
+
+#include "fmmx.h"
+
+/* Configuration (C linkage, no counterpart in the reference).  Must be
+ * called before the first shim call to take effect; the environment
+ * variables FMMX_REF_P, FMMX_REF_PRECISION (fp64|mixed),
+ * FMMX_REF_CONVENTION (ref|exact) and FMMX_REF_DEVICE set the same values
+ * for unchanged binaries.  Returns FMMX_OK or FMMX_EINVAL. */
+extern "C" int fmmx_ref_configure(int p, int precision, int convention, int device);
+/* The shim's handle (created on first use), e.g. for fmmx_get_stats. */
+extern "C" fmmx_handle* fmmx_ref_handle(void);
+
+/* fmm/nlevel.cxx:5 -- lmax from the source count (prints it as the
+ * reference does) */
+void nlevel(int nmax);
+/* fmm/boxpart.cxx:11 -- nbmax, nek (box key -> ordinal), irank (all 0) */
+void boxpart(int mi, int mj);
+/* sort/sorti.cxx:12 and sort/sortj.cxx:12 -- Morton-sort the i arrays
+ * (xi, yi, zi, gxi, gyi, gzi) / j arrays (xj, yj, zj, gxj, gyj, gzj, vj) in
+ * place; nbi / nbj receive the permutation */
+void sorti(int& mi);
+void sortj(int& mj);
+/* sort/sorti.cxx:38 and sort/sortj.cxx:38 -- undo it (the i side also
+ * restores vi) */
+void unsorti(int& mi);
+void unsortj(int& mj);
+/* fmm/boxallocate.cxx:10 -- nbmax, nbne, nbnet */
+void boxallocate(int n0, int n1, int n2, int n3);
+/* fmm/alloc.cxx:32, 124 */
+void alloc(int ialloc);
+void dealloc(int ialloc);
+/* fmm/fmm.cxx:29 -- one FMM matvec on the sorted arrays: vi (and gxi..gzi
+ * for neq 0) from vj at xj (normals gxi.. for neq 1, gxj.. for neq 2) */
+void fmm(int mi, int mj, int neq);
+/* kernel/G_dir.cxx:6, Gni_dir.cxx:6, Gnj_dir.cxx:6 -- direct sums over
+ * targets n0..n1 and sources n2..n3 (inclusive), overwriting the outputs */
+void G_dir(int n0, int n1, int n2, int n3);
+void Gni_dir(int n0, int n1, int n2, int n3);
+void Gnj_dir(int n0, int n1, int n2, int n3);
+
+/* The caller's globals the shim reads or writes (include/arrays.h:5-8, 30,
+ * 31, 60, 64; include/constants.h:39-43, 57-60). */
+extern float *xi, *yi, *zi, *gxi, *gyi, *gzi, *vi;
+extern float *xj, *yj, *zj, *gxj, *gyj, *gzj, *vj;
+extern int *nbi, *nbj, *nek, *irank;
+extern double* tfmm;
+extern int nbmax, nbne, nbnet, lmax;
+extern float xmin, ymin, zmin, rd;
+
+#endif

@@ -128,12 +128,17 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
   }
   auto S = std::make_shared<OpSet>();
   const size_t sq = (size_t)P2 * P2;
-  S->m2m.resize(8 * sq);
   S->l2l.resize(8 * sq);
   std::vector<double> A(sq);
+  const int KS = (P2 + 3) / 4, KSP = (KS + 3) / 4 * 4, VS = 4 * KSP;
+  const size_t vq = (size_t)VS * VS;
+  S->m2m.assign(8 * vq, 0.0);
   for (int o = 0; o < 8; o++) {
     op_m2m(T, o, A.data());
-    transpose_into(A.data(), P2, &S->m2m[o * sq]);
+    // padded multipole layout, transposed [k][row]
+    for (int r = 0; r < P2; r++)
+      for (int k = 0; k < P2; k++)
+        S->m2m[o * vq + (size_t)((k / KS) * KSP + k % KS) * VS + (r / KS) * KSP + r % KS] = A[(size_t)r * P2 + k];
     op_l2l(T, o, A.data());
     transpose_into(A.data(), P2, &S->l2l[o * sq]);
   }
@@ -194,6 +199,9 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   stream = (hipStream_t)strm;
   p = c->p;
   P2 = p * p;
+  KS = (P2 + 3) / 4;
+  KSP = (KS + 3) / 4 * 4;
+  VS = 4 * KSP;
   fp64 = c->precision == FMMX_FP64;
   user_box = c->rd > 0;
   for (auto& e : ev) FMMX_HIP(hipEventCreate(&e));
@@ -217,6 +225,21 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   }
   if ((rc = offtab.ensure(ot.size() * 4))) return rc;
   FMMX_HIP(hipMemcpy(offtab.p, ot.data(), ot.size() * 4, hipMemcpyHostToDevice));
+  return FMMX_OK;
+}
+
+int Engine::set_domain(int lm, float x0, float y0, float z0, float r) {
+  if (lm != 0 && (lm < 2 || lm > MAXL)) return fail(FMMX_EINVAL, "lmax out of range [2, 8]");
+  if (r > 0 && !(std::isfinite(x0) && std::isfinite(y0) && std::isfinite(z0) && std::isfinite(r)))
+    return fail(FMMX_EINVAL, "root cube not finite");
+  if (cfg.lmax == lm && cfg.xmin == x0 && cfg.ymin == y0 && cfg.zmin == z0 && cfg.rd == r) return FMMX_OK;
+  cfg.lmax = lm;
+  cfg.xmin = x0;
+  cfg.ymin = y0;
+  cfg.zmin = z0;
+  cfg.rd = r;
+  user_box = r > 0;
+  have_points = false;  // the tree of the previous domain no longer applies
   return FMMX_OK;
 }
 
@@ -314,6 +337,10 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   FMMX_HIP(hipEventRecord(ev[1], stream));
   unsigned long long pc = 0;
   if (si.nb[lmax] > 0) FMMX_HIP(hipMemcpyAsync(&pc, pair_ctr.p, 8, hipMemcpyDeviceToHost, stream));
+  for (int l = 2; l <= lmax; l++) {
+    std::fill(cls_cnt_h[l], cls_cnt_h[l] + 8, 0);
+    if (si.nb[l] > 0) FMMX_HIP(hipMemcpyAsync(cls_cnt_h[l], cls_cnt[l].p, 32, hipMemcpyDeviceToHost, stream));
+  }
   FMMX_HIP(hipStreamSynchronize(stream));
   p2p_pairs = (int64_t)pc;
   float ms = 0;
@@ -340,7 +367,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
       (rc = gys.ensure(ni1 * 8)) || (rc = gzs.ensure(ni1 * 8)))
     return rc;
   for (int l = 2; l <= lmax; l++) {
-    if ((rc = mexp[l].ensure((size_t)std::max(1, sj.nb[l]) * P2 * es)) ||
+    if ((rc = mexp[l].ensure((size_t)std::max(1, sj.nb[l]) * VS * es)) ||
         (rc = lexp[l].ensure((size_t)std::max(1, si.nb[l]) * P2 * es)))
       return rc;
   }
@@ -528,6 +555,21 @@ int fmmx_direct(fmmx_handle* h, int neq, int ni, const float* xi, const float* y
   })
 }
 
+int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin, float rd) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.set_domain(lmax, xmin, ymin, zmin, rd);
+  })
+}
+
+int fmmx_sort(fmmx_handle* h, int n, const float* x, const float* y, const float* z, int* keys, int* perm,
+              int flags) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.sort_points(n, x, y, z, keys, perm, flags);
+  })
+}
+
 int fmmx_synchronize(fmmx_handle* h) {
   if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
   if (hipStreamSynchronize(h->e.stream) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "stream synchronize failed");
@@ -628,7 +670,9 @@ int fmmx_get_list(fmmx_handle* h, int lev, int kind, int* nij_out, int* neij_out
 }
 
 static int copy_expansion(Engine& e, fmmx::DevBuf& b, int nb, int lev, bool local, double* out) {
-  const int p = e.p, P2 = e.P2, nr = fmmx::nre(p);
+  const int p = e.p, nr = fmmx::nre(p);
+  const int P2 = local ? e.P2 : e.VS;  // stored vector stride
+  auto at = [&](int c) { return local ? c : e.mslot(c); };
   std::vector<double> v((size_t)nb * P2);
   if (nb == 0) return FMMX_OK;
   if (e.fp64) {
@@ -645,8 +689,8 @@ static int copy_expansion(Engine& e, fmmx::DevBuf& b, int nb, int lev, bool loca
     for (int n = 0; n < p; n++) {
       const double s = local ? 1.0 / std::pow(rb, n + 1) : std::pow(rb, n);
       for (int m = 0; m <= n; m++) {
-        const double re = v[(size_t)i * P2 + fmmx::slot_re(n, m)];
-        const double im = m ? v[(size_t)i * P2 + fmmx::slot_im(p, n, m)] : 0.0;
+        const double re = v[(size_t)i * P2 + at(fmmx::slot_re(n, m))];
+        const double im = m ? v[(size_t)i * P2 + at(fmmx::slot_im(p, n, m))] : 0.0;
         out[2 * ((size_t)i * nr + fmmx::slot_re(n, m))] = s * re;
         out[2 * ((size_t)i * nr + fmmx::slot_re(n, m)) + 1] = s * im;
       }

@@ -63,6 +63,10 @@ struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
   int p = 0, P2 = 0, lmax = 0;
+  // multipoles are stored padded for the M2L B operand: real slot c lives at
+  // (c / KS) * KSP + c % KS, KS = ceil(P2/4), KSP = KS rounded up to 4; VS = 4*KSP
+  int KS = 0, KSP = 0, VS = 0;
+  int mslot(int c) const { return (c / KS) * KSP + c % KS; }
   bool fp64 = false;
   float xmin = 0, ymin = 0, zmin = 0, rd = 0;
   bool have_points = false, keep_lists = false, user_box = false;
@@ -76,8 +80,9 @@ struct Engine {
   std::array<DevBuf, MAXL + 1> fnij, fneij;
   // M2L target ordering by parity class and class counts per level
   std::array<DevBuf, MAXL + 1> cls_key, cls_key2, cls_ord, cls_ord2, cls_cnt;
+  int cls_cnt_h[MAXL + 1][8] = {};  // host copy of the class counts
   // operators (precision of the evaluation): M2M[8], L2L[8], M2L[levels][NOFF]
-  DevBuf op_m2m, op_l2l, op_m2l, m2l_part;  // op_m2l in MFMA fragment order
+  DevBuf op_m2m, op_l2l, op_m2l, m2l_part, m2m_part;  // op_m2l in MFMA fragment order
   bool m2l_per_level = false;
   DevBuf offtab;  // int4 (dx,dy,dz,opidx) per (class, 189)
   DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)
@@ -88,6 +93,8 @@ struct Engine {
   // evaluation buffers
   DevBuf qs, vs, gxs, gys, gzs;
   std::array<DevBuf, MAXL + 1> mexp, lexp;
+  // stand-alone Morton sort (fmmx_sort): staging, keys, permutation
+  DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm;
   int last_neq = -1;
   // timing
   bool per_phase = false;
@@ -106,7 +113,10 @@ struct Engine {
              const float* nxj, const float* nyj, const float* nzj, const float* qj, double* vi, double* gxi,
              double* gyi, double* gzi, int flags);
 
+  int set_domain(int lmax, float xmin, float ymin, float zmin, float rd);
+
   // tree.hip
+  int sort_points(int n, const float* x, const float* y, const float* z, int* keys, int* perm, int flags);
   int build_side(Side& s, int n, const float* x, const float* y, const float* z, const float* nx,
                  const float* ny, const float* nz, bool dense);
   int build_lists();

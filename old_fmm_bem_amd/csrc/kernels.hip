@@ -61,6 +61,14 @@ __constant__ HarmCoef c_hc = make_coef();
 
 __host__ __device__ constexpr int sl(int n, int m) { return n * (n + 1) / 2 + m; }
 
+// storage of a multipole slot c in the padded layout read by the M2L B
+// operand: four segments of KSP = roundup4(ceil(P2/4)) floats
+template <int P>
+struct MLayout {
+  static constexpr int P2 = P * P, KS = (P2 + 3) / 4, KSP = (KS + 3) / 4 * 4, VS = 4 * KSP;
+  static __host__ __device__ constexpr int at(int c) { return (c / KS) * KSP + c % KS; }
+};
+
 template <int P, typename T>
 __device__ __forceinline__ void solid_harmonics(T x, T y, T z, T* Sr, T* Si) {
   const T r2 = x * x + y * y + z * z;
@@ -225,11 +233,15 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
     }
     __syncthreads();
   }
+  using ML = MLayout<P>;
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const int col = lane + 64 * c;
-    if (col < P2) M[(size_t)b * P2 + col] = acc[c];
+    if (col < P2) M[(size_t)b * ML::VS + ML::at(col)] = acc[c];
   }
+  // zero the padding slots (M2M applies its operator to whole stored vectors)
+  for (int c = lane; c < ML::VS; c += 64)
+    if (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2) M[(size_t)b * ML::VS + c] = T(0);
 }
 
 // ---------------------------------------------------------------------
@@ -301,49 +313,120 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
 }
 
 // ---------------------------------------------------------------------
-// M2M: one workgroup per parent; operators stored transposed [oct][k][row]
-template <typename T>
-__global__ __launch_bounds__(128) void k_m2m(int nbp, int P2, const int* __restrict__ cfirst,
-                                             const uint32_t* __restrict__ ckey, const T* __restrict__ Mc,
-                                             T* __restrict__ Mp, const T* __restrict__ opT) {
-  extern __shared__ unsigned char smem_raw[];
-  T* v = reinterpret_cast<T*>(smem_raw);
-  const int b = blockIdx.x;
-  if (b >= nbp) return;
-  T acc0 = T(0), acc1 = T(0);
-  const int r0 = threadIdx.x, r1 = threadIdx.x + 128;
-  for (int c = cfirst[b]; c < cfirst[b + 1]; c++) {
-    for (int k = threadIdx.x; k < P2; k += blockDim.x) v[k] = Mc[(size_t)c * P2 + k];
+// M2M and L2L as small batched GEMMs: a workgroup (128 threads) owns 16
+// output boxes; per child octant the operator (VS x VS, k-major) is staged
+// in LDS in K-chunks and applied to the 16 gathered input vectors, each
+// thread holding a (rows x 4 boxes) register block.
+//   M2M: output parent b, input = its child with octant o, summed over the
+//        octants present (kernel/m2m.cxx:115-155).
+//   L2L: the 16 children of a tile share one parity class, hence one octant
+//        and one operator; output child = operator x its parent
+//        (kernel/l2l.cxx:203-237).
+constexpr int TR_TB = 16, TR_KC = 32, TR_BST = TR_TB + 4, TR_NT = 128;
+
+template <int VS, typename T, int L2L>
+__global__ __launch_bounds__(TR_NT) void k_translate(int nout, const int* __restrict__ cfirst,
+                                                     const uint32_t* __restrict__ key, const int* __restrict__ parent,
+                                                     const int* __restrict__ cls_cnt, const uint32_t* __restrict__ ord,
+                                                     const T* __restrict__ in, T* __restrict__ out,
+                                                     const T* __restrict__ opT) {
+  // M2M: blockIdx.y = child octant; this block writes that octant's partial
+  // product into out + octant * nout * VS (summed afterwards in fixed order)
+  constexpr int RPT = (VS + 31) / 32;
+  __shared__ T As[TR_KC][VS];
+  __shared__ T Bs[TR_KC][TR_BST];
+  __shared__ int src[TR_TB], dst[TR_TB];
+  __shared__ int s_oct;
+  const int tid = threadIdx.x;
+  const int tx = tid & 3, ty = tid >> 2;  // 4 boxes x 32 row groups
+  if (tid == 0) {
+    int oct = -1;
+    if (L2L) {
+      // tile -> (class, index) over the class-sorted child list
+      int t = blockIdx.x, start = 0;
+      for (int c = 0; c < 8; c++) {
+        const int cnt = cls_cnt[c], nt = (cnt + TR_TB - 1) / TR_TB;
+        if (oct < 0 && t < nt) {
+          // class bits (x,y,z) -> octant bits (y | x<<1 | z<<2)
+          oct = ((c >> 1) & 1) | ((c & 1) << 1) | (c & 4);
+          start += t * TR_TB;
+          s_oct = oct;
+          for (int k = 0; k < TR_TB; k++) {
+            const int i = t * TR_TB + k;
+            dst[k] = i < cnt ? (int)ord[start - t * TR_TB + i] : -1;
+          }
+        } else if (oct < 0) {
+          t -= nt;
+          start += cnt;
+        }
+      }
+      if (oct < 0) s_oct = -1;
+    } else {
+      s_oct = 0;
+      for (int k = 0; k < TR_TB; k++) {
+        const int b = blockIdx.x * TR_TB + k;
+        dst[k] = b < nout ? b : -1;
+      }
+    }
+  }
+  __syncthreads();
+  if (s_oct < 0) return;
+  T acc[RPT][4];
+#pragma unroll
+  for (int r = 0; r < RPT; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[r][c] = T(0);
+  const int o_begin = L2L ? s_oct : (int)blockIdx.y, o_end = o_begin + 1;
+  if (!L2L) out += (size_t)blockIdx.y * nout * VS;
+  for (int o = o_begin; o < o_end; o++) {
     __syncthreads();
-    const T* A = opT + (size_t)(ckey[c] & 7) * P2 * P2;
-    for (int k = 0; k < P2; k++) {
-      const T vk = v[k];
-      if (r0 < P2) acc0 += A[(size_t)k * P2 + r0] * vk;
-      if (r1 < P2) acc1 += A[(size_t)k * P2 + r1] * vk;
+    if (tid < TR_TB) {
+      const int b = dst[tid];
+      int s = -1;
+      if (b >= 0) {
+        if (L2L) {
+          s = parent[b];
+        } else {
+          for (int c = cfirst[b]; c < cfirst[b + 1]; c++)
+            if ((int)(key[c] & 7) == o) s = c;
+        }
+      }
+      src[tid] = s;
     }
     __syncthreads();
+    const T* A = opT + (size_t)o * VS * VS;
+    for (int k0 = 0; k0 < VS; k0 += TR_KC) {
+      const int kc = min(TR_KC, VS - k0);
+      __syncthreads();
+      for (int e = tid; e < kc * VS; e += TR_NT) As[e / VS][e % VS] = A[(size_t)k0 * VS + e];
+      for (int e = tid; e < kc * TR_TB; e += TR_NT) {
+        const int t = e / kc, k = e % kc, s = src[t];
+        Bs[k][t] = s >= 0 ? in[(size_t)s * VS + k0 + k] : T(0);
+      }
+      __syncthreads();
+      for (int k = 0; k < kc; k++) {
+        T bv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) bv[c] = Bs[k][tx * 4 + c];
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+          const int row = ty + 32 * r;
+          const T av = row < VS ? As[k][row] : T(0);
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[r][c] += av * bv[c];
+        }
+      }
+    }
   }
-  if (r0 < P2) Mp[(size_t)b * P2 + r0] = acc0;
-  if (r1 < P2) Mp[(size_t)b * P2 + r1] = acc1;
-}
-
-// L2L: one workgroup per child box; writes (M2L then accumulates)
-template <typename T>
-__global__ __launch_bounds__(128) void k_l2l(int nbc, int P2, const int* __restrict__ parent,
-                                             const uint32_t* __restrict__ ckey, const T* __restrict__ Lp,
-                                             T* __restrict__ Lc, const T* __restrict__ opT) {
-  extern __shared__ unsigned char smem_raw[];
-  T* v = reinterpret_cast<T*>(smem_raw);
-  const int b = blockIdx.x;
-  if (b >= nbc) return;
-  const int pb = parent[b];
-  for (int k = threadIdx.x; k < P2; k += blockDim.x) v[k] = Lp[(size_t)pb * P2 + k];
-  __syncthreads();
-  const T* A = opT + (size_t)(ckey[b] & 7) * P2 * P2;
-  for (int r = threadIdx.x; r < P2; r += blockDim.x) {
-    T acc = T(0);
-    for (int k = 0; k < P2; k++) acc += A[(size_t)k * P2 + r] * v[k];
-    Lc[(size_t)b * P2 + r] = acc;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int b = dst[tx * 4 + c];
+    if (b < 0) continue;
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int row = ty + 32 * r;
+      if (row < VS) out[(size_t)b * VS + row] = acc[r][c];
+    }
   }
 }
 
@@ -376,107 +459,156 @@ struct Mfma<double> {
   }
 };
 
-constexpr int M2L_TN = 64;
+constexpr int M2L_TN = 64;  // targets per workgroup of the FP64 vector kernel
 
-template <int P, typename T>
-__global__ __launch_bounds__(256) void k_m2l(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
+// FP32 MFMA kernel: NT N-tiles (16 targets each) per wave, 4 waves, so a
+// workgroup owns 64*NT targets.  With PA the next offset's operator
+// fragments are prefetched into registers while the current offset runs on
+// the matrix cores; the next offset's source multipoles always are
+// (software pipelining).  The operator goes through one LDS buffer shared by
+// the 4 waves.
+template <int P, int NT, int PA, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
                                              const uint32_t* __restrict__ ord, const uint32_t* __restrict__ ikey,
-                                             const int* __restrict__ dense_j, const T* __restrict__ M,
-                                             T* __restrict__ L, T* __restrict__ part, const T* __restrict__ opF,
-                                             const int4* __restrict__ offtab, int init,
-                                             unsigned long long* __restrict__ npairs) {
+                                             const int* __restrict__ dense_j, const float* __restrict__ M,
+                                             float* __restrict__ L, float* __restrict__ part,
+                                             const float* __restrict__ opF, const int4* __restrict__ offtab,
+                                             int init, unsigned long long* __restrict__ npairs) {
+  using T = float;
+  using ML = MLayout<P>;
   constexpr int P2 = P * P, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, FR = KS * MT * 64;
-  // k-steps staged per pass: the whole operator unless it exceeds 80 KB
-  constexpr int KC = (KS * MT * 64 * (int)sizeof(T) <= 81920) ? KS : (KS + 1) / 2;
-  using V4 = typename Mfma<T>::v4;
-  __shared__ __align__(16) T As[KC * MT * 64];
+  constexpr int NTH = 64 * WPG;          // threads per workgroup
+  constexpr int TN = 16 * NT * WPG;      // targets per workgroup
+  constexpr int NV = FR / 4;             // float4 per operator
+  constexpr int NVT = (NV + NTH - 1) / NTH;  // float4 per thread
+  using V4 = f32x4;
+  __shared__ __align__(16) T As[FR];
   __shared__ int s_class, s_start, s_n;
   const int tid = threadIdx.x;
-  const int tile = blockIdx.x / nsplit, split = blockIdx.x % nsplit;
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+  // so workgroup b runs on XCD b%8; class b%8 keeps each XCD's L2 on one
+  // parity class's 189 operators.  Work item k = b/8 of that class.
+  const int cls0 = blockIdx.x & 7, item = blockIdx.x >> 3;
+  const int tile = item / nsplit, split = item % nsplit;
   if (tid == 0) {
-    int w = tile, start = 0, cls = -1, n = 0;
-    for (int c = 0; c < 8; c++) {
-      const int cnt = cls_cnt[c], nt = (cnt + M2L_TN - 1) / M2L_TN;
-      if (cls < 0 && w < nt) {
-        cls = c;
-        n = min(M2L_TN, cnt - w * M2L_TN);
-        start += w * M2L_TN;
-      } else if (cls < 0) {
-        w -= nt;
-        start += cnt;
-      }
-    }
-    s_class = cls;
-    s_start = start;
-    s_n = n;
+    int start = 0;
+    for (int c = 0; c < cls0; c++) start += cls_cnt[c];
+    const int cnt = cls_cnt[cls0], nt = (cnt + TN - 1) / TN;
+    s_class = tile < nt ? cls0 : -1;
+    s_start = start + tile * TN;
+    s_n = tile < nt ? min(TN, cnt - tile * TN) : 0;
   }
   __syncthreads();
   const int cls = s_class, ntile = s_n;
   if (cls < 0) return;
   const int w = tid >> 6, l = tid & 63, j = l & 15, kq = l >> 4;
-  const int t = 16 * w + j;  // this lane's target (B column / C column)
-  const bool wave_on = 16 * w < ntile;
   const int g = 1 << lev;
-  int tord = -1, tx = 0, ty = 0, tz = 0;
-  if (t < ntile) {
-    tord = (int)ord[s_start + t];
-    morton_decode(ikey[tord], lev, tx, ty, tz);
+  int tord[NT], tx[NT], ty[NT], tz[NT];
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    const int t = (w * NT + n) * 16 + j;
+    tord[n] = -1;
+    tx[n] = ty[n] = tz[n] = 0;
+    if (t < ntile) {
+      tord[n] = (int)ord[s_start + t];
+      morton_decode(ikey[tord[n]], lev, tx[n], ty[n], tz[n]);
+    }
   }
+  const bool wave_on = w * NT * 16 < ntile;
   const int per = (NEBM + nsplit - 1) / nsplit, o0 = split * per, o1 = min(NEBM, o0 + per);
-  V4 acc[MT];
+  V4 acc[NT][MT];
 #pragma unroll
-  for (int mt = 0; mt < MT; mt++) acc[mt] = V4{T(0), T(0), T(0), T(0)};
+  for (int n = 0; n < NT; n++)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) acc[n][mt] = V4{0.f, 0.f, 0.f, 0.f};
   unsigned long long mypairs = 0;
+  float4 areg[PA ? NVT : 1];
+  T bcur[NT][KS];
+#define FMMX_LOAD_B(o_, dst_)                                                                 \
+  {                                                                                           \
+    const int4 off_ = offtab[cls * NEBM + (o_)];                                              \
+    _Pragma("unroll") for (int n = 0; n < NT; n++) {                                          \
+      int s_ = -1;                                                                            \
+      if (tord[n] >= 0) {                                                                     \
+        const int jx = tx[n] - off_.x, jy = ty[n] - off_.y, jz = tz[n] - off_.z;              \
+        if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g)                     \
+          s_ = dense_j[morton_encode(jx, jy, jz, lev)];                                       \
+      }                                                                                       \
+      if (s_ >= 0 && kq == 0) mypairs++;                                                      \
+      const float4* ms_ = reinterpret_cast<const float4*>(M + (size_t)(s_ >= 0 ? s_ : 0) * ML::VS + kq * ML::KSP); \
+      _Pragma("unroll") for (int v4 = 0; v4 < ML::KSP / 4; v4++) {                            \
+        const float4 m4 = s_ >= 0 ? ms_[v4] : make_float4(0.f, 0.f, 0.f, 0.f);                \
+        if (4 * v4 + 0 < KS) dst_[n][4 * v4 + 0] = (kq * KS + 4 * v4 + 0 < P2) ? m4.x : 0.f;  \
+        if (4 * v4 + 1 < KS) dst_[n][4 * v4 + 1] = (kq * KS + 4 * v4 + 1 < P2) ? m4.y : 0.f;  \
+        if (4 * v4 + 2 < KS) dst_[n][4 * v4 + 2] = (kq * KS + 4 * v4 + 2 < P2) ? m4.z : 0.f;  \
+        if (4 * v4 + 3 < KS) dst_[n][4 * v4 + 3] = (kq * KS + 4 * v4 + 3 < P2) ? m4.w : 0.f;  \
+      }                                                                                       \
+    }                                                                                         \
+  }
+#define FMMX_LOAD_A(o_)                                                                       \
+  {                                                                                           \
+    const float4* src_ = reinterpret_cast<const float4*>(opF + (size_t)offtab[cls * NEBM + (o_)].w * FR); \
+    _Pragma("unroll") for (int v = 0; v < NVT; v++) {                                         \
+      const int e = tid + NTH * v;                                                            \
+      if (e < NV) areg[v] = src_[e];                                                          \
+    }                                                                                         \
+  }
+  if (o0 < o1) {
+    if (PA) FMMX_LOAD_A(o0);
+    FMMX_LOAD_B(o0, bcur);
+  }
   for (int o = o0; o < o1; o++) {
-    const int4 off = offtab[cls * NEBM + o];
-    // this lane's B values: multipole slots kq*KS + ks of the source box
-    int s = -1;
-    if (tord >= 0) {
-      const int jx = tx - off.x, jy = ty - off.y, jz = tz - off.z;
-      if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g) s = dense_j[morton_encode(jx, jy, jz, lev)];
+    __syncthreads();  // every wave is done with the previous operator
+    if (PA) {
+#pragma unroll
+      for (int v = 0; v < NVT; v++) {
+        const int e = tid + NTH * v;
+        if (e < NV) reinterpret_cast<float4*>(As)[e] = areg[v];
+      }
+    } else {
+      const float4* src = reinterpret_cast<const float4*>(opF + (size_t)offtab[cls * NEBM + o].w * FR);
+      for (int e = tid; e < NV; e += NTH) reinterpret_cast<float4*>(As)[e] = src[e];
     }
-    T b[KS];
-    const T* ms = M + (size_t)(s >= 0 ? s : 0) * P2 + kq * KS;
+    __syncthreads();
+    T bnext[NT][KS];
+    if (o + 1 < o1) {
+      if (PA) FMMX_LOAD_A(o + 1);
+      FMMX_LOAD_B(o + 1, bnext);
+    }
+    if (wave_on) {
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) b[ks] = (s >= 0 && kq * KS + ks < P2) ? ms[ks] : T(0);
-    if (s >= 0 && kq == 0) mypairs++;
+      for (int ks = 0; ks < KS; ks++)
 #pragma unroll
-    for (int k0 = 0; k0 < KS; k0 += KC) {
-      constexpr int NVC = KC * MT * 64 * (int)sizeof(T) / 16;
-      const int nv = (min(KC, KS - k0) * MT * 64 * (int)sizeof(T)) / 16;
-      __syncthreads();
-      {
-        // operator fragments -> LDS, 16-byte copies
-        const float4* src = reinterpret_cast<const float4*>(opF + (size_t)off.w * FR + (size_t)k0 * MT * 64);
-        float4* dst = reinterpret_cast<float4*>(As);
-        for (int e = tid; e < nv && e < NVC; e += 256) dst[e] = src[e];
-      }
-      __syncthreads();
-      if (wave_on) {
+        for (int mt = 0; mt < MT; mt++) {
+          const T a = As[(ks * MT + mt) * 64 + l];
 #pragma unroll
-        for (int kk = 0; kk < KC; kk++) {
-          const int ks = k0 + kk;
-          if (ks < KS) {
-#pragma unroll
-            for (int mt = 0; mt < MT; mt++) acc[mt] = Mfma<T>::fma(As[(kk * MT + mt) * 64 + l], b[ks], acc[mt]);
-          }
+          for (int n = 0; n < NT; n++) acc[n][mt] = Mfma<T>::fma(a, bcur[n][ks], acc[n][mt]);
         }
-      }
+    }
+    if (o + 1 < o1) {
+#pragma unroll
+      for (int n = 0; n < NT; n++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) bcur[n][ks] = bnext[n][ks];
     }
   }
-  if (tord >= 0) {
+#undef FMMX_LOAD_A
+#undef FMMX_LOAD_B
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    if (tord[n] < 0) continue;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = 16 * mt + 4 * kq + r;
         if (i < P2) {
-          const T v = acc[mt][r];
+          const T v = acc[n][mt][r];
           if (nsplit == 1) {
-            T* dst = L + (size_t)tord * P2 + i;
+            T* dst = L + (size_t)tord[n] * P2 + i;
             *dst = init ? v : *dst + v;
           } else {
-            part[((size_t)split * nb + tord) * P2 + i] = v;
+            part[((size_t)split * nb + tord[n]) * P2 + i] = v;
           }
         }
       }
@@ -565,7 +697,7 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
       for (int e = tid; e < kc * P2; e += 256) As[e / P2][e % P2] = A[(size_t)k0 * P2 + e];
       for (int e = tid; e < kc * M2L_TN; e += 256) {
         const int t = e / kc, k = e % kc, s = src[t];
-        Bs[k][t] = s >= 0 ? M[(size_t)s * P2 + k0 + k] : T(0);
+        Bs[k][t] = s >= 0 ? M[(size_t)s * MLayout<P>::VS + MLayout<P>::at(k0 + k)] : T(0);
       }
       __syncthreads();
       for (int k = 0; k < kc; k++) {
@@ -618,7 +750,7 @@ __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
 // ---------------------------------------------------------------------
 // P2P: one workgroup per target leaf box (chunks of BS targets), sources of
 // each near box staged through LDS in tiles of BS.
-template <typename T, int NEQ, int CONV>
+template <typename T, int NEQ, int CONV, bool SELF = true>
 __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, float nyi, float nzi, float4 s,
                                      float4 sn, T& p, T& fx, T& fy, T& fz) {
   T dx, dy, dz;
@@ -637,7 +769,7 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
     inv = rsq(r2 + T(EPSF));
   } else {
     inv = rsq(r2);
-    inv = r2 > T(0) ? inv : T(0);  // self pair excluded (select, no branch)
+    if (SELF) inv = r2 > T(0) ? inv : T(0);  // coincident pair excluded (select, no branch)
   }
   const T qi = T(s.w) * inv;
   if (NEQ == FMMX_G) {
@@ -654,7 +786,8 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
 }
 
 template <typename T, int NEQ, int CONV, int BS>
-__global__ __launch_bounds__(BS) void k_p2p(int nbi, const int* __restrict__ ifirst, const int* __restrict__ ilast,
+__global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ jkey,
+                                            const int* __restrict__ ifirst, const int* __restrict__ ilast,
                                             const int* __restrict__ nij, const int* __restrict__ neij,
                                             const int* __restrict__ jfirst, const int* __restrict__ jlast,
                                             const float* __restrict__ xi, const float* __restrict__ yi,
@@ -700,13 +833,25 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const int* __restrict__ ifi
         const int cnt = min(BS, j1 - jb + 1);
         T p = T(0), fx = T(0), fy = T(0), fz = T(0);
         int s = 0;
-        for (; s + 4 <= cnt; s += 4) {
+        // coincident points can only share a box key; other box pairs skip the test
+        if (CONV == FMMX_CONV_EXACT && !(jkey[jj] == ikey[b])) {
+          for (; s + 4 <= cnt; s += 4) {
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx, fy, fz);
+            for (int u = 0; u < 4; u++)
+              pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx,
+                                        fy, fz);
+          }
+          for (; s < cnt; s++)
+            pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz);
+        } else {
+          for (; s + 4 <= cnt; s += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx, fy, fz);
+          }
+          for (; s < cnt; s++)
+            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz);
         }
-        for (; s < cnt; s++)
-          pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz);
         ap += (double)p;
         if (NEQ == FMMX_G) {
           ax += (double)fx;
@@ -819,7 +964,7 @@ int Engine::run_p2p(int neq) {
     using T = typename decltype(tag_t)::type;
     constexpr int NEQ = decltype(neq_c)::value, CONV = decltype(conv_c)::value, BS = decltype(bs_c)::value;
     k_p2p<T, NEQ, CONV, BS><<<nbi, BS, 0, stream>>>(
-        nbi, si.bfirst[lmax].as<int>(), si.blast.as<int>(), nij.as<int>(), neij.as<int>(), sj.bfirst[lmax].as<int>(),
+        nbi, si.bkey[lmax].as<uint32_t>(), sj.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(), si.blast.as<int>(), nij.as<int>(), neij.as<int>(), sj.bfirst[lmax].as<int>(),
         sj.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(),
         si.nz.as<float>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), qs.as<float>(), sj.nx.as<float>(),
         sj.ny.as<float>(), sj.nz.as<float>(), cc, vs.as<double>(), gxs.as<double>(), gys.as<double>(),
@@ -897,29 +1042,48 @@ int Engine::run_l2p(int neq) {
 int Engine::run_m2m(int lev) {
   const int nbp = sj.nb[lev];
   if (nbp == 0) return FMMX_OK;
-  const size_t sh = (size_t)P2 * (fp64 ? 8 : 4);
-  if (fp64)
-    k_m2m<double><<<nbp, 128, sh, stream>>>(nbp, P2, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(),
-                                            mexp[lev + 1].as<double>(), mexp[lev].as<double>(), op_m2m.as<double>());
-  else
-    k_m2m<float><<<nbp, 128, sh, stream>>>(nbp, P2, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(),
-                                           mexp[lev + 1].as<float>(), mexp[lev].as<float>(), op_m2m.as<float>());
-  FMMX_HIP(hipGetLastError());
-  return FMMX_OK;
+  const dim3 grid((nbp + TR_TB - 1) / TR_TB, 8);
+  const size_t es = fp64 ? 8 : 4;
+  const long n = (long)nbp * VS;
+  int rc = m2m_part.ensure((size_t)8 * n * es);
+  if (rc) return rc;
+  return dispatch_p(p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value, V = MLayout<P>::VS;
+    if (fp64) {
+      k_translate<V, double, 0><<<grid, TR_NT, 0, stream>>>(
+          nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
+          mexp[lev + 1].as<double>(), m2m_part.as<double>(), op_m2m.as<double>());
+      k_m2l_reduce<double><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, 8, m2m_part.as<double>(),
+                                                                       mexp[lev].as<double>(), 1);
+    } else {
+      k_translate<V, float, 0><<<grid, TR_NT, 0, stream>>>(
+          nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
+          mexp[lev + 1].as<float>(), m2m_part.as<float>(), op_m2m.as<float>());
+      k_m2l_reduce<float><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, 8, m2m_part.as<float>(),
+                                                                      mexp[lev].as<float>(), 1);
+    }
+    FMMX_HIP(hipGetLastError());
+    return FMMX_OK;
+  });
 }
 
 int Engine::run_l2l(int lev) {
   const int nbc = si.nb[lev];
   if (nbc == 0) return FMMX_OK;
-  const size_t sh = (size_t)P2 * (fp64 ? 8 : 4);
-  if (fp64)
-    k_l2l<double><<<nbc, 128, sh, stream>>>(nbc, P2, si.parent[lev].as<int>(), si.bkey[lev].as<uint32_t>(),
-                                            lexp[lev - 1].as<double>(), lexp[lev].as<double>(), op_l2l.as<double>());
-  else
-    k_l2l<float><<<nbc, 128, sh, stream>>>(nbc, P2, si.parent[lev].as<int>(), si.bkey[lev].as<uint32_t>(),
-                                           lexp[lev - 1].as<float>(), lexp[lev].as<float>(), op_l2l.as<float>());
-  FMMX_HIP(hipGetLastError());
-  return FMMX_OK;
+  const int grid = (nbc + TR_TB - 1) / TR_TB + 8;
+  return dispatch_p(p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value, Q = P * P;
+    if (fp64)
+      k_translate<Q, double, 1><<<grid, TR_NT, 0, stream>>>(
+          nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
+          cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<double>(), lexp[lev].as<double>(), op_l2l.as<double>());
+    else
+      k_translate<Q, float, 1><<<grid, TR_NT, 0, stream>>>(
+          nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
+          cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<float>(), lexp[lev].as<float>(), op_l2l.as<float>());
+    FMMX_HIP(hipGetLastError());
+    return FMMX_OK;
+  });
 }
 
 int Engine::run_m2l(int lev, bool init) {
@@ -939,13 +1103,15 @@ int Engine::run_m2l(int lev, bool init) {
   const size_t opoff = m2l_per_level ? (size_t)(lev - 2) * NOFF * FR : 0;
   const int tiles = (nb + M2L_TN - 1) / M2L_TN + 8;
   // enough workgroups to cover the chip: split the 189 offsets when tiles are few
-  int nsplit = std::max(1, std::min(16, 1024 / tiles));
-  if (const char* e = getenv("FMMX_M2L_SPLIT")) nsplit = std::max(1, std::min(16, atoi(e)));
+  int split_env = 0;
+  if (const char* e = getenv("FMMX_M2L_SPLIT")) split_env = atoi(e);
+  int nsplit = std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / tiles));
   const size_t es = fp64 ? 8 : 4;
-  if (nsplit > 1) {
+  if (fp64 && nsplit > 1) {
     int rc = m2l_part.ensure((size_t)nsplit * nl * es);
     if (rc) return rc;
   }
+  int rc2 = FMMX_OK;
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if (fp64) {
@@ -957,10 +1123,33 @@ int Engine::run_m2l(int lev, bool init) {
         k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<double>(),
                                                                           lexp[lev].as<double>(), init ? 1 : 0);
     } else {
-      k_m2l<P, float><<<tiles * nsplit, 256, 0, stream>>>(
-          lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
-          sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), m2l_part.as<float>(),
-          op_m2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 1);
+      // variant: N-tiles per wave (1|2) and operator prefetch (0|1)
+      int nt = 1, pa = 0, wpg = 8;
+      if (const char* e = getenv("FMMX_M2L_NT")) nt = atoi(e) == 2 ? 2 : 1;
+      if (const char* e = getenv("FMMX_M2L_PA")) pa = atoi(e) ? 1 : 0;
+      if (const char* e = getenv("FMMX_M2L_WPG")) wpg = atoi(e) == 4 ? 4 : 8;
+      // per class at most ceil(max class count / TN) tiles; grid = 8 classes x tiles x splits
+      int cmax = 0;
+      for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
+      const int tpc = std::max(1, (cmax + 16 * wpg * nt - 1) / (16 * wpg * nt));
+      const int ns2 = std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / (8 * tpc)));
+      if (ns2 > 1 && (rc2 = m2l_part.ensure((size_t)ns2 * nl * es))) return rc2;
+#define FMMX_M2L_LAUNCH(NT_, PA_, W_)                                                                    \
+  k_m2l<P, NT_, PA_, W_><<<8 * tpc * ns2, 64 * W_, 0, stream>>>(                                        \
+      lev, nb, ns2, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),   \
+      sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), m2l_part.as<float>(),     \
+      op_m2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 1)
+      if (wpg == 8) {
+        if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 8);
+        else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 8);
+        else FMMX_M2L_LAUNCH(2, 0, 8);
+      } else {
+        if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 4);
+        else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 4);
+        else FMMX_M2L_LAUNCH(2, 0, 4);
+      }
+#undef FMMX_M2L_LAUNCH
+      nsplit = ns2;
       if (nsplit > 1)
         k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<float>(),
                                                                          lexp[lev].as<float>(), init ? 1 : 0);

@@ -16,7 +16,8 @@ namespace {
 constexpr int TPB = 256;
 inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
 
-// fmm/boxn.cxx:12-20: (x - xmin) in float, times kl/rd in double, truncate.
+// fmm/boxn.cxx:12-20: (x - xmin) in float, times the scale kl/rd (a float
+// quotient widened to double, fmm/boxn.cxx:9), truncate.
 // The reference clamps only the upper face; coordinates outside the cube are
 // clamped to the nearest box here instead of producing invalid keys.
 __global__ void k_keys(int n, const float* __restrict__ x, const float* __restrict__ y,
@@ -178,6 +179,50 @@ __global__ void k_class(int nb, const uint32_t* __restrict__ bkey, int lev, uint
 
 }  // namespace
 
+// Keys of one point set at the leaf level and their stable sort, as
+// sort/sorti.cxx:12-26 (boxn, then the counting sort of sort/csort.cxx) leaves
+// them in nfn/na (sorted keys) and nbi (sorted position -> caller index).
+int Engine::sort_points(int n, const float* x, const float* y, const float* z, int* keys_out, int* perm_out,
+                        int flags) {
+  if (n < 0) return fail(FMMX_EINVAL, "negative particle count");
+  if (n > 0 && (!x || !y || !z || !perm_out)) return fail(FMMX_EINVAL, "NULL pointer");
+  if (!user_box || cfg.lmax <= 0) return fail(FMMX_ESTATE, "fmmx_sort needs an explicit root cube and lmax");
+  if (n == 0) return FMMX_OK;
+  FMMX_HIP(hipSetDevice(device));
+  const bool dev = flags & FMMX_DEVICE_PTRS;
+  const size_t nb = (size_t)n * 4;
+  int rc;
+  if ((rc = srt_keys_in.ensure(nb)) || (rc = srt_idx_in.ensure(nb)) || (rc = srt_keys.ensure(nb)) ||
+      (rc = srt_perm.ensure(nb)))
+    return rc;
+  const float* d[3] = {x, y, z};
+  if (!dev) {
+    for (int k = 0; k < 3; k++) {
+      if ((rc = srt_in[k].ensure(nb))) return rc;
+      FMMX_HIP(hipMemcpyAsync(srt_in[k].p, d[k], nb, hipMemcpyHostToDevice, stream));
+      d[k] = srt_in[k].as<float>();
+    }
+  }
+  const int lm = cfg.lmax;
+  double rbinv = (double)((float)(1 << lm) / cfg.rd);
+  k_keys<<<nblk(n), TPB, 0, stream>>>(n, d[0], d[1], d[2], cfg.xmin, cfg.ymin, cfg.zmin, rbinv, lm,
+                                      srt_keys_in.as<uint32_t>(), srt_idx_in.as<uint32_t>());
+  size_t tb = 0;
+  FMMX_HIP(rocprim::radix_sort_pairs(nullptr, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
+                                     srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
+                                     stream));
+  if ((rc = scratch.ensure(tb + 256))) return rc;
+  tb = scratch.bytes;
+  FMMX_HIP(rocprim::radix_sort_pairs(scratch.p, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
+                                     srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
+                                     stream));
+  const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  FMMX_HIP(hipMemcpyAsync(perm_out, srt_perm.p, nb, kind, stream));
+  if (keys_out) FMMX_HIP(hipMemcpyAsync(keys_out, srt_keys.p, nb, kind, stream));
+  FMMX_HIP(hipStreamSynchronize(stream));
+  return FMMX_OK;
+}
+
 int Engine::build_side(Side& s, int n, const float* x, const float* y, const float* z, const float* nx,
                        const float* ny, const float* nz, bool dense) {
   s.n = n;
@@ -193,7 +238,7 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
     return rc;
   s.nb.fill(0);
   if (n == 0) return FMMX_OK;
-  double rbinv = (double)(1 << lmax) / (double)rd;
+  double rbinv = (double)((float)(1 << lmax) / rd);
   k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rbinv, lmax, s.keys_in.as<uint32_t>(),
                                       s.idx_in.as<uint32_t>());
   size_t tb = 0;

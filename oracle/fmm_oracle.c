@@ -82,11 +82,13 @@ static void boxc(int nf, int* nc) {
   nc[2] = t[0];
 }
 
-/* fmm/boxn.cxx:5-31: key per particle; (x-xmin) in float, times double */
+/* fmm/boxn.cxx:5-31: key per particle; (x-xmin) in float, times the scale
+ * kl/rd, which fmm/boxn.cxx:9 computes as int/float (a float quotient)
+ * before widening it to double */
 static void boxn(int n, const float* x, const float* y, const float* z, int lmax,
                  float xmin, float ymin, float zmin, float rd, int* key) {
   int kl = 1 << lmax;
-  double rb = kl / (double)rd;
+  double rb = (double)((float)kl / rd);
   for (int j = 0; j < n; j++) {
     int f[3];
     f[0] = (int)((double)(x[j] - xmin) * rb);

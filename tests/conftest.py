@@ -18,7 +18,7 @@ def golden(name):
     return dict(np.load(os.path.join(GOLDEN, f"ref_{name}.npz")))
 
 
-GOLDEN_CASES = ["g3", "gni3", "gnj3", "g2", "g4same"]
+GOLDEN_CASES = ["g3", "gni3", "gnj3", "g2", "g4same", "gedge4"]
 
 
 def uniform_cube(n, seed, lo=0.0, hi=1.0):

@@ -37,6 +37,7 @@ CASES = {
     "gnj3": (1000, 900, 3, 2, False, False, 13),
     "g2": (400, 450, 2, 0, False, False, 14),
     "g4same": (2500, 2500, 4, 0, True, False, 15),
+    "gedge4": (1500, 1400, 4, 0, False, False, 16),
 }
 # arrays kept in every case; expansion dumps only where listed (size)
 FULL = {"g3", "gnj3", "g2"}
@@ -55,6 +56,26 @@ def make_inputs(ni, nj, same, seed):
     nj_n = rng.random((nj, 3), dtype=np.float32)
     q = rng.random(nj, dtype=np.float32)
     return xi, ni_n, xj, nj_n, q
+
+
+def face_nudged(n, lmax, rng):
+    """Points whose coordinates sit within 8 ulp of the box faces of level
+    lmax: fmm/boxn.cxx:9 scales by kl/rd computed in float, and the keys of
+    such points depend on that rounding."""
+    kl = 1 << lmax
+    rd = np.float32(2) * PI_F
+    faces = np.array([np.float32(-PI_F + np.float32(c) * (rd / np.float32(kl))) for c in range(1, kl)], np.float32)
+    x = (rng.random((n, 3), dtype=np.float32) * np.float32(2) * PI_F - PI_F).astype(np.float32)
+    for k in range(3):
+        sel = rng.random(n) < 0.7
+        base = faces[rng.integers(0, kl - 1, sel.sum())]
+        ulp = rng.integers(-8, 9, sel.sum())
+        v = base.copy()
+        for i in range(len(v)):
+            for _ in range(abs(int(ulp[i]))):
+                v[i] = np.nextafter(v[i], np.float32(np.inf) if ulp[i] > 0 else np.float32(-np.inf))
+        x[sel, k] = v
+    return x
 
 
 def write_input(path, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q):
@@ -102,6 +123,10 @@ def main():
         if a.only and name != a.only:
             continue
         xi, ni_n, xj, nj_n, q = make_inputs(ni, nj, same, seed)
+        if name.startswith("gedge"):
+            rng = np.random.default_rng(seed + 1000)
+            xi = face_nudged(ni, lmax, rng)
+            xj = face_nudged(nj, lmax, rng)
         fin = os.path.join(tmp, f"{name}.in")
         fout = os.path.join(tmp, f"{name}.out")
         box = write_input(fin, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q)

@@ -23,12 +23,19 @@ def _declared(header):
 
 def test_library_exports_declared_symbols():
     lib = C.CDLL(F.library_path())
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
-        names = _declared(h)
-        if os.path.basename(h) == "fmmx.h":
-            assert names
-            for n in names:
-                assert hasattr(lib, n), n
+    headers = {os.path.basename(h): h for h in glob.glob(os.path.join(ROOT, "include", "*.h"))}
+    assert set(headers) == {"fmmx.h", "fmmx_refshim.h"}
+    names = _declared(headers["fmmx.h"])
+    assert "fmmx_sort" in names and "fmmx_set_domain" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    # the shim's C-linkage entry points (the library itself needs the
+    # caller's globals to load, so its dynamic symbol table is read instead)
+    path = os.path.join(ROOT, "old_fmm_bem_amd", "libfmmx_ref.so")
+    if os.path.exists(path):
+        out = os.popen(f"nm -D --defined-only {path}").read()
+        for n in _declared(headers["fmmx_refshim.h"]):
+            assert re.search(rf"\b[TW] {n}\b", out), n
 
 
 def test_refshim_exports_reference_symbols():
@@ -39,11 +46,13 @@ def test_refshim_exports_reference_symbols():
     # C++ symbols of the reference's driver and setup chain (fmm/fmm.cxx:29,
     # fmm/nlevel.cxx:5, fmm/boxpart.cxx:11, sort/sorti.cxx:12, ...)
     for mangled in ["_Z3fmmiii", "_Z6nleveli", "_Z7boxpartii", "_Z5sortiRi", "_Z5sortjRi", "_Z11boxallocateiiii",
-                    "_Z5alloci", "_Z7deallocI", "_Z7unsortiRi", "_Z7unsortjRi", "_Z5G_diriiii", "_Z7Gni_diriiii",
+                    "_Z5alloci", "_Z7dealloci", "_Z7unsortiRi", "_Z7unsortjRi", "_Z5G_diriiii", "_Z7Gni_diriiii",
                     "_Z7Gnj_diriiii"]:
-        if mangled == "_Z7deallocI":
-            mangled = "_Z7dealloci"
-        assert mangled in out, mangled
+        assert re.search(rf"\bT {mangled}\b", out), mangled
+    # the globals stay the caller's (include/arrays.h with MAIN defines them)
+    und = os.popen(f"nm -D --undefined-only {path}").read()
+    for g in ["xi", "vj", "nbi", "lmax", "rd", "tfmm"]:
+        assert re.search(rf"\bU {g}\b", und), g
 
 
 def test_abi_version():

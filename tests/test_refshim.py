@@ -1,0 +1,94 @@
+"""libfmmx_ref.so: the reference's own C++ entry points over the caller's
+globals.  tests/refshim/ref_caller is a caller written against the reference
+interface (the test/test.cxx:98-146 sequence) and linked only against the
+shim; its outputs are checked against what the compiled reference wrote for
+the same inputs (tests/golden/)."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, ROOT, golden, rel_l2
+
+CALLER = os.path.join(ROOT, "tests", "refshim", "ref_caller")
+
+
+def _write_input(path, d):
+    ni, nj, lmax, neq = (int(v) for v in d["in_cfg"][:4])
+    with open(path, "wb") as f:
+        f.write(struct.pack("<5i", ni, nj, lmax, neq, 0))
+        f.write(np.asarray(d["in_box"], np.float32).tobytes())
+        for a in (d["in_xi"], d["in_ni"], d["in_xj"], d["in_nj"]):
+            for k in range(3):
+                f.write(np.ascontiguousarray(a[:, k], dtype=np.float32).tobytes())
+        f.write(np.asarray(d["in_q"], np.float32).tobytes())
+
+
+def _read_output(path):
+    out, data, p = {}, open(path, "rb").read(), 0
+    while p < len(data):
+        name = data[p:p + 32].split(b"\0", 1)[0].decode()
+        typ, n = struct.unpack_from("<2i", data, p + 32)
+        p += 40
+        out[name] = np.frombuffer(data[p:p + 4 * n], dtype=np.int32 if typ == 0 else np.float32).copy()
+        p += 4 * n
+    return out
+
+
+def _run(d, tmp_path, env=None):
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, d)
+    r = subprocess.run([CALLER, fin, fout], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, **(env or {})))
+    return r, (_read_output(fout) if r.returncode == 0 else None)
+
+
+def _need_caller():
+    if not os.path.exists(CALLER):
+        pytest.skip("tests/refshim/ref_caller not built (build() builds it)")
+
+
+def test_caller_without_device_reports_and_exits(tmp_path):
+    """Errors follow the reference's CUDA error path: a message and exit(1),
+    never an abort.  Runs where no HIP device is visible."""
+    _need_caller()
+    r, _ = _run(golden("g2"), tmp_path, env={"HIP_VISIBLE_DEVICES": "-1", "ROCR_VISIBLE_DEVICES": "-1"})
+    assert r.returncode == 1, (r.returncode, r.stderr)
+    assert "fmmx_ref: " in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_reference_caller_matches_reference(case, tmp_path):
+    _need_caller()
+    d = golden(case)
+    ni, nj, lmax, neq = (int(v) for v in d["in_cfg"][:4])
+    print(f"case={case} neq={neq} lmax={lmax}", flush=True)
+    r, o = _run(d, tmp_path)
+    assert r.returncode == 0, r.stderr
+    # sort permutations and box counts: bit for bit (sort/sorti.cxx:21-23,
+    # fmm/boxallocate.cxx:88-89)
+    assert np.array_equal(o["nbi"], d["nbi"]) and np.array_equal(o["nbj"], d["nbj"])
+    nbmax, nbne, nbnet, lm = o["counts"]
+    assert (nbmax, lm) == (1 << 3 * lmax, lmax)
+    assert nbne == max(len(d["leaf_nfi"]), len(d["leaf_nfj"]))
+    assert nbnet == max(len(d["all_nfi"]), len(d["all_nfj"]))
+    assert o["nek_boxes"][0] == len(np.union1d(d["leaf_nfi"], d["leaf_nfj"]))
+    # matvec outputs, sorted and caller order, as the reference wrote them
+    assert rel_l2(o["fmm_vi_sorted"], d["fmm_vi_sorted"]) < 2e-6
+    assert rel_l2(o["fmm_vi"], d["fmm_vi"]) < 2e-6
+    if neq == 0:
+        g = np.stack([o["fmm_gxi"], o["fmm_gyi"], o["fmm_gzi"]], 1)
+        assert rel_l2(g, np.stack([d["fmm_gxi"], d["fmm_gyi"], d["fmm_gzi"]], 1)) < 2e-6
+    # second matvec on the cached tree, strengths doubled
+    assert rel_l2(o["fmm2_vi_sorted"], 2 * o["fmm_vi_sorted"]) < 1e-6
+    # unsortj restored the caller's arrays exactly
+    assert np.array_equal(o["xj_restored"], d["in_xj"][:, 0]) and np.array_equal(o["vj_restored"], d["in_q"])
+    assert o["tfmm_total"][0] > 0
+    # direct sums (kernel/G_dir.cxx, Gni_dir.cxx, Gnj_dir.cxx)
+    assert rel_l2(o["dir_vi"], d["dir_vi"]) < 1e-7
+    if neq == 0:
+        g = np.stack([o["dir_gxi"], o["dir_gyi"], o["dir_gzi"]], 1)
+        assert rel_l2(g, np.stack([d["dir_gxi"], d["dir_gyi"], d["dir_gzi"]], 1)) < 1e-7
