@@ -38,8 +38,10 @@ enum {
   FMMX_EUNSUPPORTED = 5
 };
 
-/* kernels: the reference's neq values (fmm/fmm.cxx:47-65) */
-enum { FMMX_G = 0, FMMX_GNI = 1, FMMX_GNJ = 2 };
+/* kernels: the reference's neq values (fmm/fmm.cxx:47-65); FMMX_YUKAWA is
+ * its "Helmholtz" (neq 10): the screened kernel (pi/2) exp(-lambda r) /
+ * (lambda r) (kernel/Helmholtz_G_p2p.cxx:22), lambda from fmmx_set_yukawa */
+enum { FMMX_G = 0, FMMX_GNI = 1, FMMX_GNJ = 2, FMMX_YUKAWA = 10 };
 
 /* arithmetic of the evaluation */
 enum {
@@ -89,7 +91,8 @@ int fmmx_destroy(fmmx_handle* h);
  * (test/test.cxx:107-112) plus fmm.cxx steps boxdatai/boxdataj and the
  * interaction lists.  nx*: target normals (needed for FMMX_GNI) and source
  * normals (FMMX_GNJ); NULL when unused.  Reused by every evaluate until the
- * next set_points. */
+ * next set_points.  Passing the same x, y, z pointers for both sides (ni ==
+ * nj) builds the tree once and shares it. */
 int fmmx_set_points(fmmx_handle* h, int ni, const float* xi, const float* yi, const float* zi,
                     const float* nxi, const float* nyi, const float* nzi, int nj, const float* xj,
                     const float* yj, const float* zj, const float* nxj, const float* nyj,
@@ -99,7 +102,9 @@ int fmmx_set_points(fmmx_handle* h, int ni, const float* xi, const float* yi, co
  * order).  Outputs in caller order: vi (potential, or the Gni/Gnj layer
  * potential) and, for FMMX_G only, the gradient gxi,gyi,gzi (may be NULL).
  * Unlike the reference, outputs are overwritten (not accumulated) and are
- * double.  Factor 1/(4 pi) included as in kernel/G_p2p.cxx:25. */
+ * double.  Factor 1/(4 pi) included as in kernel/G_p2p.cxx:25.
+ * FMMX_YUKAWA also returns the gradient (the reference computes the
+ * potential only, kernel/Helmholtz_G_p2p.cxx). */
 int fmmx_evaluate(fmmx_handle* h, int neq, const float* qj, double* vi, double* gxi, double* gyi,
                   double* gzi, int flags);
 
@@ -117,6 +122,11 @@ int fmmx_direct(fmmx_handle* h, int neq, int ni, const float* xi, const float* y
  * reference reads them from globals at every call (include/constants.h:56-59,
  * fmm/nlevel.cxx:11); takes effect at the next set_points. */
 int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin, float rd);
+
+/* Screening constant lambda > 0 of FMMX_YUKAWA (the reference's global
+ * lambda, include/constants.h:61, test/testh.cxx:42).  Its translation
+ * operators are built at the first Yukawa evaluate after a change. */
+int fmmx_set_yukawa(fmmx_handle* h, double lambda);
 
 /* Morton keys of n points at the leaf level and their stable sort
  * (fmm/boxn.cxx:5-31 then sort/csort.cxx:5-17, as sort/sorti.cxx:12-26
@@ -137,6 +147,9 @@ typedef struct fmmx_stats {
   int64_t p2p_pairs;     /* sum over near box pairs of n_i * n_j */
   int64_t m2l_pairs;     /* number of (target box, source box) M2L pairs */
   int64_t m2l_tiles;     /* M2L work tiles launched */
+  int64_t m2l_pairs_leaf; /* M2L pairs at the leaf level */
+  double ms_m2l_leaf;     /* duration of the leaf-level M2L kernel alone
+                             (per-phase timing only) */
 } fmmx_stats;
 int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st);
 /* Enable/disable per-phase event timing (default off: one event pair). */
@@ -169,7 +182,13 @@ int fmmx_get_locals(fmmx_handle* h, int lev, double* ax_reim);
  * with real slots n(n+1)/2+m (Re) and p(p+1)/2 + n(n-1)/2 + m-1 (Im, m>0).
  * kind 0: M2M from child octant a0 (kernel/m2m.cxx:126-154); 1: L2L to
  * child octant a0 (kernel/l2l.cxx:203-237); 2: M2L for target-minus-source
- * offset (a0,a1,a2) at box size rb (kernel/m2l.cxx:43-81). */
+ * offset (a0,a1,a2) at box size rb (kernel/m2l.cxx:43-81).
+ * Yukawa (neq 10, kernel/Helmholtz_m2m.cxx, Helmholtz_m2l.cxx), where rb is
+ * lambda times the box size (of the child for 3 and 4) and the bases are
+ * M~_n = M_n / ((2n+1) i_n(lambda rb)), L~_n = L_n i_n(lambda rb) with
+ * multipole basis k_n(lambda r) Y_n^m and local basis i_n(lambda r) Y_n^m:
+ * kind 3: M2M from child octant a0; 4: L2L to child octant a0; 5: M2L for
+ * offset (a0,a1,a2).  The convention does not enter kinds 3-5. */
 int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, double rb, double* A);
 
 /* Last error message of the calling thread. */

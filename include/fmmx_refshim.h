@@ -20,7 +20,12 @@
  *  - fmm(mi, mj, 2) leaves gxi, gyi, gzi untouched (the reference adds far
  *    field gradients into them, fmm/fmm.cxx:145, on top of the caller's
  *    values);
- *  - Yukawa (neq 10) is not provided;
+ *  - fmm(mi, mj, 10) (Yukawa, screening from the global lambda) writes the
+ *    field gradient into gxi, gyi, gzi (the reference zeroes them and adds a
+ *    far-field term that is not the Yukawa gradient,
+ *    kernel/Helmholtz_G_l2p.cxx:60-70), and translates exactly instead of
+ *    with plane waves, at any depth (the reference's path runs only at
+ *    lmax 2, kernel/Helmholtz_m2l.cxx:111-122);
  *  - errors print a message and exit(1), as the reference's CUDA error path
  *    does (cutil-mp.h:776-787).
  */
@@ -65,14 +70,17 @@ void fmm(int mi, int mj, int neq);
 void G_dir(int n0, int n1, int n2, int n3);
 void Gni_dir(int n0, int n1, int n2, int n3);
 void Gnj_dir(int n0, int n1, int n2, int n3);
+/* kernel/Helmholtz_G_dir.cxx:6 -- Yukawa direct sum, potential only */
+void Helmholtz_G_dir(int n0, int n1, int n2, int n3);
 
 /* The caller's globals the shim reads or writes (include/arrays.h:5-8, 30,
- * 31, 60, 64; include/constants.h:39-43, 57-60). */
+ * 31, 60, 64; include/constants.h:39-43, 57-61). */
 extern float *xi, *yi, *zi, *gxi, *gyi, *gzi, *vi;
 extern float *xj, *yj, *zj, *gxj, *gyj, *gzj, *vj;
 extern int *nbi, *nbj, *nek, *irank;
 extern double* tfmm;
 extern int nbmax, nbne, nbnet, lmax;
 extern float xmin, ymin, zmin, rd;
+extern double lambda;
 
 #endif

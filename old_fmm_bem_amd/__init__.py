@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = ["FMM", "FMMError", "lib", "operator", "G", "GNI", "GNJ", "MIXED", "FP64", "REF", "EXACT"]
 
-G, GNI, GNJ = 0, 1, 2
+G, GNI, GNJ, YUKAWA = 0, 1, 2, 10
 MIXED, FP64 = 0, 1
 REF, EXACT = 0, 1
 NEBM = 189
@@ -46,7 +46,8 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("ms_tree", "ms_lists", "ms_p2p", "ms_p2m", "ms_m2m", "ms_m2l",
                                           "ms_l2l", "ms_l2p", "ms_total")] + \
                [(n, C.c_int) for n in ("lmax", "nbi_leaf", "nbj_leaf", "nbi_total", "nbj_total")] + \
-               [(n, C.c_int64) for n in ("p2p_pairs", "m2l_pairs", "m2l_tiles")]
+               [(n, C.c_int64) for n in ("p2p_pairs", "m2l_pairs", "m2l_tiles", "m2l_pairs_leaf")] + \
+               [("ms_m2l_leaf", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -59,6 +60,10 @@ _SYMBOLS = {
     "fmmx_evaluate": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
     "fmmx_direct": (C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_int] + [C.c_void_p] * 11
                     + [C.c_int]),
+    "fmmx_set_yukawa": (C.c_int, [C.c_void_p, C.c_double]),
+    "fmmx_set_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "fmmx_sort": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                            C.c_int]),
     "fmmx_synchronize": (C.c_int, [C.c_void_p]),
     "fmmx_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "fmmx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -130,7 +135,8 @@ def operator(p, convention, kind, a0, a1=0, a2=0, rb=1.0):
 
 
 class FMM:
-    def __init__(self, p=10, lmax=0, precision="mixed", convention="exact", box=None, device=0, stream=None):
+    def __init__(self, p=10, lmax=0, precision="mixed", convention="exact", box=None, device=0, stream=None,
+                 lam=None):
         cfg = _Config(p, lmax, _PREC[precision], _CONV[convention], 0.0, 0.0, 0.0, 0.0)
         if box is not None:
             cfg.xmin, cfg.ymin, cfg.zmin, cfg.rd = [float(np.float32(b)) for b in box]
@@ -141,6 +147,13 @@ class FMM:
         self.h = h
         self.ni = self.nj = 0
         self._device = False
+        if lam is not None:
+            self.set_yukawa(lam)
+
+    def set_yukawa(self, lam):
+        """Screening constant of the YUKAWA kernel (pi/2) exp(-lam r)/(lam r)."""
+        _check(lib().fmmx_set_yukawa(self.h, float(lam)))
+        return self
 
     def close(self):
         if getattr(self, "h", None):
@@ -154,15 +167,37 @@ class FMM:
             pass
 
     # -- set-up ---------------------------------------------------------
+    def set_domain(self, lmax=0, box=None):
+        """Replace the leaf level and root cube (xmin, ymin, zmin, rd) fixed at
+        construction; box None: from the points at each set_points."""
+        b = [0.0, 0.0, 0.0, 0.0] if box is None else [float(np.float32(v)) for v in box]
+        _check(lib().fmmx_set_domain(self.h, int(lmax), *b))
+        self.lmax_cfg = lmax
+        return self
+
+    def sort(self, x):
+        """Morton keys at the leaf level and their stable sort, as
+        sort/sorti.cxx:12-26: returns (keys in sorted order, perm) with
+        perm[k] = index of the k-th point in key order."""
+        n = int(x.shape[0])
+        cols = [_Arr(np.asarray(x)[:, k], False) for k in range(3)]
+        keys = np.zeros(max(n, 1), np.int32)
+        perm = np.zeros(max(n, 1), np.int32)
+        _check(lib().fmmx_sort(self.h, n, *[c.ptr for c in cols], keys.ctypes.data, perm.ctypes.data, 0))
+        return keys[:n], perm[:n]
+
     def set_points(self, xi, xj=None, ni_normals=None, nj_normals=None, keep_lists=False):
-        """Targets xi (n,3), sources xj (m,3) (default: the same points)."""
+        """Targets xi (n,3), sources xj (m,3) (default: the same points, which
+        then share one tree)."""
+        same = xj is None or xj is xi
         if xj is None:
             xj = xi
         dev = _is_torch(xi) and xi.is_cuda
         self._device = dev
         cols = [_Arr(xi[:, k], dev) for k in range(3)]
         cols += [_Arr(None if ni_normals is None else ni_normals[:, k], dev) for k in range(3)]
-        cols += [_Arr(xj[:, k], dev) for k in range(3)]
+        # the same column buffers for both sides: the engine then builds one tree
+        cols += cols[:3] if same else [_Arr(xj[:, k], dev) for k in range(3)]
         cols += [_Arr(None if nj_normals is None else nj_normals[:, k], dev) for k in range(3)]
         self._cols = cols
         self.ni, self.nj = int(xi.shape[0]), int(xj.shape[0])
@@ -173,10 +208,10 @@ class FMM:
 
     # -- matvec ---------------------------------------------------------
     def evaluate(self, q, neq=G, gradient=True):
-        """Potential v (ni,) and, for neq=G, gradient g (ni,3), caller order."""
+        """Potential v (ni,) and, for neq=G or YUKAWA, gradient g (ni,3), caller order."""
         dev = self._device
         qa = _Arr(q, dev)
-        want_g = gradient and neq == G
+        want_g = gradient and neq in (G, YUKAWA)
         if dev:
             import torch
             v = torch.empty(self.ni, dtype=torch.float64, device=q.device)
@@ -212,7 +247,7 @@ class FMM:
             ptr = lambda t: t.ctypes.data  # noqa: E731
         _check(lib().fmmx_direct(self.h, neq, n, *[c.ptr for c in a], int(xj.shape[0]), *[c.ptr for c in b], qa.ptr,
                                  ptr(v), ptr(g[0]), ptr(g[1]), ptr(g[2]), 1 if dev else 0))
-        return v, (g.T if neq == G else None)
+        return v, (g.T if neq in (G, YUKAWA) else None)
 
     def synchronize(self):
         _check(lib().fmmx_synchronize(self.h))

@@ -9,6 +9,7 @@
 #include <tuple>
 
 #include "engine.h"
+#include "yukawa.h"
 #include "tree_util.h"
 
 namespace fmmx {
@@ -168,6 +169,62 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
   return S;
 }
 
+// Yukawa operator sets keyed by (p, lambda, rd, lmax): M2M per parent level
+// 2..lmax-1 and L2L per child level 3..lmax (8 octants each, layouts as the
+// Laplace sets), M2L per level 2..lmax (316 offsets)
+std::map<std::tuple<int, double, float, int>, std::shared_ptr<OpSet>> g_yuk_cache;
+
+std::shared_ptr<OpSet> get_yuk_ops(int p, double lam, float rd, int lmax) {
+  const int P2 = p * p;
+  auto key = std::make_tuple(p, lam, rd, lmax);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_yuk_cache.find(key);
+    if (it != g_yuk_cache.end()) return it->second;
+  }
+  auto S = std::make_shared<OpSet>();
+  const size_t sq = (size_t)P2 * P2;
+  const int KS = (P2 + 3) / 4, KSP = (KS + 3) / 4 * 4, VS = 4 * KSP;
+  const size_t vq = (size_t)VS * VS;
+  const int nt_lev = std::max(1, lmax - 2);  // translation levels
+  S->m2m.assign((size_t)nt_lev * 8 * vq, 0.0);
+  S->l2l.assign((size_t)nt_lev * 8 * sq, 0.0);
+  S->nlev = lmax - 1;
+  const size_t fr = (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64;
+  S->m2l_frag.resize((size_t)S->nlev * NOFF * fr);
+  S->m2l_kmaj.resize((size_t)S->nlev * NOFF * sq);
+  const int ntr = lmax > 2 ? (lmax - 2) * 8 : 0, njobs = ntr + S->nlev * NOFF;
+  std::vector<std::thread> th;
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      std::vector<double> A(sq), B(sq);
+      for (int j = (int)t; j < njobs; j += (int)nt) {
+        if (j < ntr) {
+          // child level lc = 3 + j/8 (parent lc-1), octant j%8
+          const int li = j / 8, o = j % 8, lc = 3 + li;
+          yuk_op_m2m_l2l(p, lam * (double)rd / (double)(1 << lc), o, A.data(), B.data());
+          double* M = &S->m2m[((size_t)li * 8 + o) * vq];
+          for (int r = 0; r < P2; r++)
+            for (int k = 0; k < P2; k++)
+              M[(size_t)((k / KS) * KSP + k % KS) * VS + (r / KS) * KSP + r % KS] = A[(size_t)r * P2 + k];
+          transpose_into(B.data(), P2, &S->l2l[((size_t)li * 8 + o) * sq]);
+        } else {
+          const int jj = j - ntr, li = jj / NOFF, oi = jj % NOFF;
+          int d[3];
+          offset_vector(oi, d);
+          yuk_op_m2l(p, lam * (double)rd / (double)(1 << (li + 2)), d[0], d[1], d[2], A.data());
+          fragment_into(A.data(), P2, &S->m2l_frag[((size_t)li * NOFF + oi) * fr]);
+          transpose_into(A.data(), P2, &S->m2l_kmaj[((size_t)li * NOFF + oi) * sq]);
+        }
+      }
+    });
+  for (auto& x : th) x.join();
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_yuk_cache[key] = S;
+  return S;
+}
+
 template <typename T>
 int upload(DevBuf& dst, const std::vector<double>& src, hipStream_t s) {
   int rc = dst.ensure(src.size() * sizeof(T));
@@ -243,6 +300,31 @@ int Engine::set_domain(int lm, float x0, float y0, float z0, float r) {
   return FMMX_OK;
 }
 
+int Engine::set_yukawa(double lambda) {
+  if (!(lambda > 0) || !std::isfinite(lambda)) return fail(FMMX_EINVAL, "lambda must be positive and finite");
+  lam = lambda;
+  return FMMX_OK;
+}
+
+int Engine::upload_yukawa_operators() {
+  if (yops_lmax == lmax && yops_rd == rd && yops_lam == lam) return FMMX_OK;
+  auto S = get_yuk_ops(p, lam, rd, lmax);
+  int rc;
+  if (fp64) {
+    if ((rc = upload<double>(yop_m2m, S->m2m, stream)) || (rc = upload<double>(yop_l2l, S->l2l, stream)) ||
+        (rc = upload<double>(yop_m2l, S->m2l_kmaj, stream)))
+      return rc;
+  } else {
+    if ((rc = upload<float>(yop_m2m, S->m2m, stream)) || (rc = upload<float>(yop_l2l, S->l2l, stream)) ||
+        (rc = upload<float>(yop_m2l, S->m2l_frag, stream)))
+      return rc;
+  }
+  yops_lmax = lmax;
+  yops_rd = rd;
+  yops_lam = lam;
+  return FMMX_OK;
+}
+
 int Engine::upload_operators() {
   if (ops_lmax == lmax && ops_rd == rd) return FMMX_OK;
   auto S = get_ops(tab, lmax, rd);
@@ -285,7 +367,13 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   int rc = FMMX_OK;
   const float* a[14] = {xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj, nullptr, nullptr};
   const float* d[14];
+  // one tree for both sides when the caller passes the same coordinate arrays
+  const bool same = ni == nj && ni > 0 && xi == xj && yi == yj && zi == zj;
   for (int k = 0; k < 12; k++) {
+    if (same && k >= 6 && k < 9) {
+      d[k] = d[k - 6];
+      continue;
+    }
     d[k] = stage(in[k], a[k], k < 6 ? ni : nj, dev, stream, rc);
     if (rc) return rc;
   }
@@ -322,7 +410,7 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   if ((rc = upload_operators())) return rc;
   // tree of both sides (keys, sort, gathers, per-level boxes)
   if ((rc = build_side(si, ni, d[0], d[1], d[2], d[3], d[4], d[5], false))) return rc;
-  if ((rc = build_side(sj, nj, d[6], d[7], d[8], d[9], d[10], d[11], true))) return rc;
+  if ((rc = build_side(sj, nj, d[6], d[7], d[8], d[9], d[10], d[11], true, same ? &si : nullptr))) return rc;
   int hc[2][MAXL + 1] = {};
   if (ni > 0) FMMX_HIP(hipMemcpyAsync(hc[0], si.cnt.p, (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
   if (nj > 0) FMMX_HIP(hipMemcpyAsync(hc[1], sj.cnt.p, (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
@@ -353,7 +441,9 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
 
 int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags) {
   if (!have_points) return fail(FMMX_ESTATE, "evaluate before set_points");
-  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ) return fail(FMMX_EINVAL, "neq must be 0, 1 or 2");
+  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ && neq != FMMX_YUKAWA)
+    return fail(FMMX_EINVAL, "neq must be 0, 1, 2 or 10");
+  if (neq == FMMX_YUKAWA && !(lam > 0)) return fail(FMMX_ESTATE, "Yukawa needs lambda > 0 (fmmx_set_yukawa)");
   if (neq == FMMX_GNI && si.n > 0 && !si.has_normals) return fail(FMMX_EINVAL, "Gni needs target normals");
   if (neq == FMMX_GNJ && sj.n > 0 && !sj.has_normals) return fail(FMMX_EINVAL, "Gnj needs source normals");
   if ((sj.n > 0 && !qj) || (si.n > 0 && !vi)) return fail(FMMX_EINVAL, "q or v pointer is NULL");
@@ -373,6 +463,10 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   }
   const float* q = stage(qin, qj, nj, dev, stream, rc);
   if (rc) return rc;
+  yk = neq == FMMX_YUKAWA;
+  if (yk && (rc = upload_yukawa_operators())) return rc;
+  // Yukawa: no far field where lambda * (leaf box) makes it vanish in FP64
+  const bool far = !yk || lam * rb(lmax) <= YUK_LRB_MAX;
   int e = 2;
   auto mark = [&]() -> int {
     if (per_phase) FMMX_HIP(hipEventRecord(ev[e++], stream));
@@ -380,27 +474,28 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   };
   FMMX_HIP(hipEventRecord(ev[2], stream));
   e = 3;
-  k_zero_f64<<<1, 64, 0, stream>>>(1, pair_ctr.as<double>() + 1);  // M2L pair counter
+  k_zero_f64<<<1, 64, 0, stream>>>(14, pair_ctr.as<double>() + 2);  // M2L pair counters per level
   if (nj > 0) k_gather_q<<<nblk(nj), TPB, 0, stream>>>(nj, sj.perm.as<uint32_t>(), q, qs.as<float>());
   if (ni > 0) {
     if ((rc = run_p2p(neq))) return rc;
     if ((rc = mark())) return rc;  // ev3: after P2P
-    if ((rc = run_p2m(neq))) return rc;
+    if (far && (rc = run_p2m(neq))) return rc;
     if ((rc = mark())) return rc;  // ev4: after P2M
-    for (int l = lmax - 1; l >= 2; l--)
+    for (int l = lmax - 1; far && l >= 2; l--)
       if ((rc = run_m2m(l))) return rc;
     if ((rc = mark())) return rc;  // ev5: after M2M
-    if ((rc = run_m2l(2, true))) return rc;
-    for (int l = 3; l <= lmax; l++) {
+    if (far && (rc = run_m2l(2, true))) return rc;
+    for (int l = 3; far && l <= lmax; l++) {
       if ((rc = run_l2l(l))) return rc;
       if ((rc = run_m2l(l, false))) return rc;
     }
     if ((rc = mark())) return rc;  // ev6: after M2L+L2L
-    if ((rc = run_l2p(neq))) return rc;
+    if (far && (rc = run_l2p(neq))) return rc;
     if ((rc = mark())) return rc;  // ev7: after L2P
   }
   // outputs in caller order
-  double* outs[4] = {vi, neq == FMMX_G ? gxi : nullptr, neq == FMMX_G ? gyi : nullptr, neq == FMMX_G ? gzi : nullptr};
+  const bool g = neq == FMMX_G || neq == FMMX_YUKAWA;
+  double* outs[4] = {vi, g ? gxi : nullptr, g ? gyi : nullptr, g ? gzi : nullptr};
   double* srcs[4] = {vs.as<double>(), gxs.as<double>(), gys.as<double>(), gzs.as<double>()};
   for (int k = 0; k < 4; k++) {
     if (!outs[k] || ni == 0) continue;
@@ -423,7 +518,9 @@ int Engine::direct(int neq, int ni, const float* xi, const float* yi, const floa
                    const float* nxj, const float* nyj, const float* nzj, const float* qj, double* vi, double* gxi,
                    double* gyi, double* gzi, int flags) {
   if (ni < 0 || nj < 0) return fail(FMMX_EINVAL, "negative particle count");
-  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ) return fail(FMMX_EINVAL, "neq must be 0, 1 or 2");
+  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ && neq != FMMX_YUKAWA)
+    return fail(FMMX_EINVAL, "neq must be 0, 1, 2 or 10");
+  if (neq == FMMX_YUKAWA && !(lam > 0)) return fail(FMMX_ESTATE, "Yukawa needs lambda > 0 (fmmx_set_yukawa)");
   if (neq == FMMX_GNI && ni > 0 && (!nxi || !nyi || !nzi)) return fail(FMMX_EINVAL, "Gni needs target normals");
   if (neq == FMMX_GNJ && nj > 0 && (!nxj || !nyj || !nzj)) return fail(FMMX_EINVAL, "Gnj needs source normals");
   if (ni > 0 && (!xi || !yi || !zi || !vi)) return fail(FMMX_EINVAL, "NULL target pointer");
@@ -440,9 +537,11 @@ int Engine::direct(int neq, int ni, const float* xi, const float* yi, const floa
   }
   double* o[4] = {vi, gxi, gyi, gzi};
   double* od[4];
+  const bool ygrad = neq == FMMX_YUKAWA && gxi && gyi && gzi;  // Yukawa gradients are optional
+  if (neq == FMMX_YUKAWA && !ygrad) o[1] = o[2] = o[3] = nullptr;
   for (int k = 0; k < 4; k++) {
     od[k] = o[k];
-    if (!dev && o[k]) {
+    if ((!dev && o[k]) || (k > 0 && neq == FMMX_YUKAWA && !ygrad)) {
       if ((rc = out[k].ensure((size_t)std::max(1, ni) * 8))) return rc;
       od[k] = out[k].as<double>();
     }
@@ -452,7 +551,7 @@ int Engine::direct(int neq, int ni, const float* xi, const float* yi, const floa
     return rc;
   if (!dev) {
     for (int k = 0; k < 4; k++)
-      if (o[k] && ni > 0 && (k == 0 || neq == FMMX_G))
+      if (o[k] && ni > 0 && (k == 0 || neq == FMMX_G || neq == FMMX_YUKAWA))
         FMMX_HIP(hipMemcpyAsync(o[k], od[k], (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
     FMMX_HIP(hipStreamSynchronize(stream));
   }
@@ -494,6 +593,18 @@ int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, doubl
     if (kind == 2) {
       if (fmmx::offset_index(a0, a1, a2) < 0 || !(rb > 0)) return fmmx::fail(FMMX_EINVAL, "bad M2L offset or rb");
       fmmx::op_m2l(*T, a0, a1, a2, rb, A);
+      return FMMX_OK;
+    }
+    if (kind == 3 || kind == 4) {
+      if (a0 < 0 || a0 > 7 || !(rb > 0)) return fmmx::fail(FMMX_EINVAL, "octant out of range or lambda*rb <= 0");
+      std::vector<double> other((size_t)p * p * p * p);
+      if (kind == 3) fmmx::yuk_op_m2m_l2l(p, rb, a0, A, other.data());
+      else fmmx::yuk_op_m2m_l2l(p, rb, a0, other.data(), A);
+      return FMMX_OK;
+    }
+    if (kind == 5) {
+      if (fmmx::offset_index(a0, a1, a2) < 0 || !(rb > 0)) return fmmx::fail(FMMX_EINVAL, "bad M2L offset or lambda*rb");
+      fmmx::yuk_op_m2l(p, rb, a0, a1, a2, A);
       return FMMX_OK;
     }
     return fmmx::fail(FMMX_EINVAL, "bad kind");
@@ -562,6 +673,13 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
   })
 }
 
+int fmmx_set_yukawa(fmmx_handle* h, double lambda) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.set_yukawa(lambda);
+  })
+}
+
 int fmmx_sort(fmmx_handle* h, int n, const float* x, const float* y, const float* z, int* keys, int* perm,
               int flags) {
   FMMX_GUARD({
@@ -604,9 +722,16 @@ int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st) {
     s.nbj_total += e.sj.nb[l];
   }
   s.p2p_pairs = e.p2p_pairs;
-  unsigned long long mp = 0;
-  if (e.pair_ctr.p) (void)hipMemcpy(&mp, e.pair_ctr.as<unsigned long long>() + 1, 8, hipMemcpyDeviceToHost);
-  s.m2l_pairs = (int64_t)mp;
+  unsigned long long mp[16] = {};
+  if (e.pair_ctr.p && hipMemcpy(mp, e.pair_ctr.p, sizeof(mp), hipMemcpyDeviceToHost) != hipSuccess)
+    return fmmx::fail(FMMX_EDEVICE, "hipMemcpy D2H failed");
+  s.m2l_pairs = 0;
+  for (int l = 2; l <= e.lmax; l++) s.m2l_pairs += (int64_t)mp[2 + l];
+  s.m2l_pairs_leaf = (int64_t)mp[2 + e.lmax];
+  s.ms_m2l_leaf = 0;
+  if (e.per_phase && e.last_neq >= 0 && e.si.nb[e.lmax] > 0 && e.sj.nb[e.lmax] > 0 &&
+      hipEventElapsedTime(&ms, e.ev[8], e.ev[9]) == hipSuccess)
+    s.ms_m2l_leaf = ms;
   *st = s;
   return FMMX_OK;
 }

@@ -18,17 +18,33 @@ constexpr int MAXL = 8;  // deepest supported leaf level (keys of 3*8 bits)
       return fail(FMMX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
-// Grow-only device allocation owned by a handle.
+// Grow-only device allocation owned by a handle, or a non-owning view of
+// another DevBuf (alias); ensure() on a view drops the view and allocates.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool view = false;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !view) (void)hipFree(p);
+  }
+  void alias(const DevBuf& o) {
+    if (p && !view) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
+    p = o.p;
+    bytes = o.bytes;
+    view = true;
   }
   int ensure(size_t need) {
+    if (view) {
+      p = nullptr;
+      bytes = 0;
+      view = false;
+    }
     if (need <= bytes && p) return FMMX_OK;
     if (p) {
       // growing: earlier asynchronous work may still read the old block
@@ -93,6 +109,14 @@ struct Engine {
   // evaluation buffers
   DevBuf qs, vs, gxs, gys, gzs;
   std::array<DevBuf, MAXL + 1> mexp, lexp;
+  // Yukawa (neq 10): screening lambda and the operator sets per level,
+  // M2M by parent level 2..lmax-1, L2L by child level 3..lmax, M2L by level
+  double lam = 0;
+  bool yk = false;  // the evaluation in progress is Yukawa
+  DevBuf yop_m2m, yop_l2l, yop_m2l;
+  int yops_lmax = -1;
+  float yops_rd = -1;
+  double yops_lam = -1;
   // stand-alone Morton sort (fmmx_sort): staging, keys, permutation
   DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm;
   int last_neq = -1;
@@ -114,11 +138,13 @@ struct Engine {
              double* gyi, double* gzi, int flags);
 
   int set_domain(int lmax, float xmin, float ymin, float zmin, float rd);
+  int set_yukawa(double lambda);
+  int upload_yukawa_operators();
 
   // tree.hip
   int sort_points(int n, const float* x, const float* y, const float* z, int* keys, int* perm, int flags);
   int build_side(Side& s, int n, const float* x, const float* y, const float* z, const float* nx,
-                 const float* ny, const float* nz, bool dense);
+                 const float* ny, const float* nz, bool dense, const Side* same = nullptr);
   int build_lists();
   int build_far_lists();
   int build_m2l_order();
@@ -135,6 +161,17 @@ struct Engine {
                  double* vi, double* gxi, double* gyi, double* gzi);
   int upload_operators();
   double c4() const { return cfg.convention == FMMX_CONV_REF ? 0.25 / (double)PIF : 0.25 / M_PI; }
+  // (pi/2)/lambda of kernel/Helmholtz_G_p2p.cxx:22 (float pi in REF)
+  double c_yukawa() const { return (cfg.convention == FMMX_CONV_REF ? 0.5 * (double)PIF : 0.5 * M_PI) / lam; }
+  // translation operators of the evaluation in progress
+  template <class T>
+  const T* m2m_ops(int parent_lev) const {
+    return yk ? yop_m2m.as<T>() + (size_t)(parent_lev - 2) * 8 * VS * VS : op_m2m.as<T>();
+  }
+  template <class T>
+  const T* l2l_ops(int child_lev) const {
+    return yk ? yop_l2l.as<T>() + (size_t)(child_lev - 3) * 8 * P2 * P2 : op_l2l.as<T>();
+  }
   double rb(int lev) const { return (double)rd / (double)(1 << lev); }
 };
 

@@ -151,6 +151,37 @@ __device__ __forceinline__ void eff_point(int conv, double dx, double dy, double
   uz = rh * ca * rbinv;
 }
 
+// i~_n(x) = i_n(x) / x^n for n = 0..N (modified spherical Bessel function
+// of the first kind over x^n): power series for N+1 and N, then the stable
+// backward recurrence i~_{n-1} = (2n+1) i~_n + x^2 i~_{n+1}
+template <int N>
+__device__ inline void ibar_dev(double x, double* out) {
+  const double h = 0.5 * x * x;
+  double top[2];
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int n = N + 1 - u;
+    double df = 1;
+    for (int k = 3; k <= 2 * n + 1; k += 2) df *= k;
+    double t = 1.0 / df, sum = t;
+    for (int k = 1; k < 400; k++) {
+      t *= h / (k * (2.0 * n + 2.0 * k + 1.0));
+      sum += t;
+      if (t < 1e-18 * sum) break;
+    }
+    top[u] = sum;
+  }
+  double a = top[0], b = top[1];
+  out[N] = b;
+#pragma unroll
+  for (int n = N; n >= 1; n--) {
+    const double c = (2 * n + 1) * b + x * x * a;
+    a = b;
+    b = c;
+    out[n - 1] = c;
+  }
+}
+
 __device__ __forceinline__ void box_center(uint32_t key, int lev, double xmin, double ymin, double zmin, double rb,
                                            double& cx, double& cy, double& cz) {
   int ix, iy, iz;
@@ -162,14 +193,17 @@ __device__ __forceinline__ void box_center(uint32_t key, int lev, double xmin, d
 
 // ---------------------------------------------------------------------
 // P2M: one wave per source leaf box
-template <int P, typename T>
+//   YK: Yukawa multipoles, M~_n^m = sum q i~_n(lambda rho)/i~_n(lambda rb)
+//       (rho/rb)^n conj(Y_n^m): the Laplace harmonics times a radial weight
+//       per degree (lrb = lambda rb)
+template <int P, typename T, int YK = 0>
 __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
                                             const float* __restrict__ y, const float* __restrict__ z,
                                             const float* __restrict__ nx, const float* __restrict__ ny,
                                             const float* __restrict__ nz, const float* __restrict__ q, int gn,
                                             int conv, int lev, double xmin, double ymin, double zmin, double rb,
-                                            const double* __restrict__ corr, T* __restrict__ M) {
+                                            const double* __restrict__ corr, T* __restrict__ M, double lrb) {
   constexpr int NR = P * (P + 1) / 2, P2 = P * P, NC = (P2 + 63) / 64;
   __shared__ T red[64][P2 + 1];
   const int b = blockIdx.x, lane = threadIdx.x;
@@ -189,7 +223,21 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
       eff_point(conv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
       T Sr[NR], Si[NR];
       solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
-      if (!gn) {
+      if (YK) {
+        double ib0[P], ibx[P];
+        ibar_dev<P - 1>(lrb, ib0);
+        ibar_dev<P - 1>(lrb * sqrt(ux * ux + uy * uy + uz * uz), ibx);
+        const double qd = (double)q[i];
+#pragma unroll
+        for (int n = 0; n < P; n++) {
+          const T w = T(qd * ibx[n] / ib0[n]);
+#pragma unroll
+          for (int m = 0; m <= n; m++) {
+            row[sl(n, m)] = w * Sr[sl(n, m)];
+            if (m > 0) row[NR + n * (n - 1) / 2 + m - 1] = -w * Si[sl(n, m)];
+          }
+        }
+      } else if (!gn) {
         const T qi = T(q[i]);
 #pragma unroll
         for (int n = 0; n < P; n++)
@@ -246,7 +294,10 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
 
 // ---------------------------------------------------------------------
 // L2P: one wave per target leaf box; adds to the P2P results (sorted order)
-template <int P, typename T>
+//   YK: Yukawa locals, phi = sum_n h_n(r) sum_m L~ S_n^m(u) with
+//       h_n = i~_n(lambda r)/i~_n(lambda rb); grad adds
+//       lrb^2 i~_{n+1}(lambda r)/i~_n(lambda rb) (sum_m L~ S_n^m) u
+template <int P, typename T, int YK = 0>
 __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
                                             const float* __restrict__ y, const float* __restrict__ z,
@@ -255,7 +306,7 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
                                             int conv, int lev, double xmin, double ymin, double zmin, double rb,
                                             const double* __restrict__ corr, double c4, double* __restrict__ vs,
                                             double* __restrict__ gxs, double* __restrict__ gys,
-                                            double* __restrict__ gzs) {
+                                            double* __restrict__ gzs, double lrb) {
   constexpr int NR = P * (P + 1) / 2, P2 = P * P;
   __shared__ T Lr[NR], Li[NR];
   const int b = blockIdx.x, lane = threadIdx.x;
@@ -265,7 +316,7 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
     int n = 0;
     while (sl(n + 1, 0) <= s) n++;
     const int m = s - sl(n, 0);
-    const double w = (m == 0 ? 1.0 : 2.0) * corr[s];
+    const double w = (m == 0 ? 1.0 : 2.0) * (YK ? 1.0 : corr[s]);
     Lr[s] = T(w * (double)L[(size_t)b * P2 + s]);
     Li[s] = m == 0 ? T(0) : T(w * (double)L[(size_t)b * P2 + NR + n * (n - 1) / 2 + m - 1]);
   }
@@ -280,24 +331,43 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
     T Sr[NR], Si[NR];
     solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
     T phi = T(0), gx = T(0), gy = T(0), gz = T(0);
+    double hn[P + 1], h0[P];
+    if (YK) {
+      ibar_dev<P - 1>(lrb, h0);
+      ibar_dev<P>(lrb * sqrt(ux * ux + uy * uy + uz * uz), hn);
+    }
 #pragma unroll
-    for (int n = 0; n < P; n++)
+    for (int n = 0; n < P; n++) {
+      T pn = T(0), gxn = T(0), gyn = T(0), gzn = T(0);
 #pragma unroll
       for (int m = 0; m <= n; m++) {
         const int s = sl(n, m);
         const T lr = Lr[s], li = Li[s];
-        phi += Sr[s] * lr - Si[s] * li;
+        pn += Sr[s] * lr - Si[s] * li;
         if (n >= 1) {
           T pr, pi, mr, mi, zr, zi;
           harm_grad<P, T>(Sr, Si, n, m, pr, pi, mr, mi, zr, zi);
           // Re(Ap L), Im(Ap L), Re(Am L), Im(Am L)
           const T apr = pr * lr - pi * li, api = pr * li + pi * lr;
           const T amr = mr * lr - mi * li, ami = mr * li + mi * lr;
-          gx += T(0.5) * (apr + amr);
-          gy += T(0.5) * (api - ami);
-          gz += zr * lr - zi * li;
+          gxn += T(0.5) * (apr + amr);
+          gyn += T(0.5) * (api - ami);
+          gzn += zr * lr - zi * li;
         }
       }
+      if (YK) {
+        const T h = T(hn[n] / h0[n]), e = T(lrb * lrb * hn[n + 1] / h0[n]) * pn;
+        phi += h * pn;
+        gx += h * gxn + e * T(ux);
+        gy += h * gyn + e * T(uy);
+        gz += h * gzn + e * T(uz);
+      } else {
+        phi += pn;
+        gx += gxn;
+        gy += gyn;
+        gz += gzn;
+      }
+    }
     const double s1 = c4 * rbinv, s2 = c4 * rbinv * rbinv;
     if (gn) {
       vs[i] -= s2 * ((double)nx[i] * (double)gx + (double)ny[i] * (double)gy + (double)nz[i] * (double)gz);
@@ -487,9 +557,11 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
   const int tid = threadIdx.x;
   // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
   // so workgroup b runs on XCD b%8; class b%8 keeps each XCD's L2 on one
-  // parity class's 189 operators.  Work item k = b/8 of that class.
-  const int cls0 = blockIdx.x & 7, item = blockIdx.x >> 3;
-  const int tile = item / nsplit, split = item % nsplit;
+  // parity class's 189 operators.  Work item k = b/8 of that class, split
+  // major: the workgroups resident on an XCD at a time share one split's
+  // 189/nsplit operators, which then stay in its L2 across tiles.
+  const int cls0 = blockIdx.x & 7, item = blockIdx.x >> 3, tpc = (int)(gridDim.x >> 3) / nsplit;
+  const int split = item / tpc, tile = item % tpc;
   if (tid == 0) {
     int start = 0;
     for (int c = 0; c < cls0; c++) start += cls_cnt[c];
@@ -746,13 +818,15 @@ __global__ void k_m2l_reduce(long n, int nsplit, const T* __restrict__ part, T* 
 // reciprocal square root: v_rsq_f32 (1 ulp) in FP32, full precision in FP64
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
+__device__ __forceinline__ float yexp(float x) { return __expf(x); }
+__device__ __forceinline__ double yexp(double x) { return exp(x); }
 
 // ---------------------------------------------------------------------
 // P2P: one workgroup per target leaf box (chunks of BS targets), sources of
 // each near box staged through LDS in tiles of BS.
 template <typename T, int NEQ, int CONV, bool SELF = true>
 __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, float nyi, float nzi, float4 s,
-                                     float4 sn, T& p, T& fx, T& fy, T& fz) {
+                                     float4 sn, T& p, T& fx, T& fy, T& fz, T lam = T(0)) {
   T dx, dy, dz;
   if (CONV == FMMX_CONV_REF || sizeof(T) == 4) {
     dx = T(xi - s.x);  // float subtraction as kernel/G_p2p.cxx:21-23
@@ -766,13 +840,24 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
   T r2 = dx * dx + dy * dy + dz * dz;
   T inv;
   if (CONV == FMMX_CONV_REF) {
-    inv = rsq(r2 + T(EPSF));
+    r2 += T(EPSF);
+    inv = rsq(r2);
   } else {
     inv = rsq(r2);
     if (SELF) inv = r2 > T(0) ? inv : T(0);  // coincident pair excluded (select, no branch)
   }
   const T qi = T(s.w) * inv;
-  if (NEQ == FMMX_G) {
+  if (NEQ == FMMX_YUKAWA) {
+    // exp(-lambda r)/r (kernel/Helmholtz_G_p2p.cxx:22 without its constant
+    // (pi/2)/lambda); gradient -(1 + lambda r) e d / r^3
+    const T r = r2 * inv;
+    const T e = qi * yexp(-lam * r);
+    p += e;
+    const T rs = e * (T(1) + lam * r) * inv * inv;
+    fx -= dx * rs;
+    fy -= dy * rs;
+    fz -= dz * rs;
+  } else if (NEQ == FMMX_G) {
     p += qi;
     const T rs = qi * inv * inv;
     fx -= dx * rs;
@@ -798,7 +883,9 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
                                             const float* __restrict__ nxj, const float* __restrict__ nyj,
                                             const float* __restrict__ nzj, double c4, double* __restrict__ vs,
                                             double* __restrict__ gxs, double* __restrict__ gys,
-                                            double* __restrict__ gzs) {
+                                            double* __restrict__ gzs, double lamd) {
+  constexpr bool GRAD = NEQ == FMMX_G || NEQ == FMMX_YUKAWA;
+  const T lam = T(lamd);
   __shared__ float4 sp[BS];
   __shared__ float4 sn[NEQ == FMMX_GNJ ? BS : 1];
   const int b = blockIdx.x, tid = threadIdx.x;
@@ -839,21 +926,23 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
 #pragma unroll
             for (int u = 0; u < 4; u++)
               pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx,
-                                        fy, fz);
+                                        fy, fz, lam);
           }
           for (; s < cnt; s++)
-            pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz);
+            pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz,
+                                      lam);
         } else {
           for (; s + 4 <= cnt; s += 4) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
-              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx, fy, fz);
+              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx, fy, fz,
+                                 lam);
           }
           for (; s < cnt; s++)
-            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz);
+            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz, lam);
         }
         ap += (double)p;
-        if (NEQ == FMMX_G) {
+        if (GRAD) {
           ax += (double)fx;
           ay += (double)fy;
           az += (double)fz;
@@ -862,7 +951,7 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
     }
     if (act) {
       vs[i] = c4 * ap;
-      if (NEQ == FMMX_G) {
+      if (GRAD) {
         gxs[i] = c4 * ax;
         gys[i] = c4 * ay;
         gzs[i] = c4 * az;
@@ -882,7 +971,8 @@ __global__ __launch_bounds__(64) void k_direct(int ni, const float* __restrict__
                                                const float* __restrict__ nxj, const float* __restrict__ nyj,
                                                const float* __restrict__ nzj, int slice, double c4,
                                                double* __restrict__ v, double* __restrict__ gx,
-                                               double* __restrict__ gy, double* __restrict__ gz) {
+                                               double* __restrict__ gy, double* __restrict__ gz, double lam) {
+  constexpr bool GRAD = NEQ == FMMX_G || NEQ == FMMX_YUKAWA;
   __shared__ float4 sp[64];
   __shared__ float4 sn[64];
   const int i = blockIdx.x * 64 + threadIdx.x;
@@ -909,11 +999,12 @@ __global__ __launch_bounds__(64) void k_direct(int ni, const float* __restrict__
     }
     __syncthreads();
     const int cnt = min(64, j1 - jb);
-    for (int s = 0; s < cnt; s++) pair<double, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], sn[s], p, fx, fy, fz);
+    for (int s = 0; s < cnt; s++)
+      pair<double, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], sn[s], p, fx, fy, fz, lam);
   }
   if (act) {
     atomicAdd(&v[i], c4 * p);
-    if (NEQ == FMMX_G) {
+    if (GRAD) {
       atomicAdd(&gx[i], c4 * fx);
       atomicAdd(&gy[i], c4 * fy);
       atomicAdd(&gz[i], c4 * fz);
@@ -957,7 +1048,7 @@ int dispatch_p(int p, F&& f) {
 int Engine::run_p2p(int neq) {
   const int nbi = si.nb[lmax];
   if (nbi == 0) return FMMX_OK;
-  const double cc = c4();
+  const double cc = neq == FMMX_YUKAWA ? c_yukawa() : c4();
   const bool ref = cfg.convention == FMMX_CONV_REF;
   const double mean = (double)si.n / nbi;
   auto go = [&](auto tag_t, auto neq_c, auto conv_c, auto bs_c) -> int {
@@ -968,7 +1059,7 @@ int Engine::run_p2p(int neq) {
         sj.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(),
         si.nz.as<float>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), qs.as<float>(), sj.nx.as<float>(),
         sj.ny.as<float>(), sj.nz.as<float>(), cc, vs.as<double>(), gxs.as<double>(), gys.as<double>(),
-        gzs.as<double>());
+        gzs.as<double>(), lam);
     return FMMX_OK;
   };
   auto by_bs = [&](auto t, auto nq, auto cv) -> int {
@@ -981,6 +1072,7 @@ int Engine::run_p2p(int neq) {
                : by_bs(t, nq, std::integral_constant<int, FMMX_CONV_EXACT>{});
   };
   auto by_neq = [&](auto t) -> int {
+    if (neq == FMMX_YUKAWA) return by_conv(t, std::integral_constant<int, FMMX_YUKAWA>{});
     if (neq == FMMX_G) return by_conv(t, std::integral_constant<int, FMMX_G>{});
     if (neq == FMMX_GNI) return by_conv(t, std::integral_constant<int, FMMX_GNI>{});
     return by_conv(t, std::integral_constant<int, FMMX_GNJ>{});
@@ -997,18 +1089,21 @@ int Engine::run_p2m(int neq) {
   const double* corr = corr_d.as<double>();
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    if (fp64)
-      k_p2m<P, double><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),
-                                              sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(),
-                                              sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), qs.as<float>(),
-                                              neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax),
-                                              corr, mexp[lmax].as<double>());
-    else
-      k_p2m<P, float><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),
-                                             sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(),
-                                             sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), qs.as<float>(),
-                                             neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr,
-                                             mexp[lmax].as<float>());
+    const double lrb = lam * rb(lmax);
+#define FMMX_P2M(T_, YK_)                                                                                          \
+  k_p2m<P, T_, YK_><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),          \
+                                           sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), \
+                                           sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), qs.as<float>(),  \
+                                           neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, \
+                                           mexp[lmax].as<T_>(), lrb)
+    if (fp64) {
+      if (yk) FMMX_P2M(double, 1);
+      else FMMX_P2M(double, 0);
+    } else {
+      if (yk) FMMX_P2M(float, 1);
+      else FMMX_P2M(float, 0);
+    }
+#undef FMMX_P2M
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
@@ -1020,20 +1115,24 @@ int Engine::run_l2p(int neq) {
   const double* corr = corr_d.as<double>();
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    if (fp64)
-      k_l2p<P, double><<<nb, 64, 0, stream>>>(nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(),
-                                              si.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(),
-                                              si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),
-                                              lexp[lmax].as<double>(), neq == FMMX_GNI, neq == FMMX_G, cfg.convention,
-                                              lmax, xmin, ymin, zmin, rb(lmax), corr, c4(), vs.as<double>(),
-                                              gxs.as<double>(), gys.as<double>(), gzs.as<double>());
-    else
-      k_l2p<P, float><<<nb, 64, 0, stream>>>(nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(),
-                                             si.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(),
-                                             si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),
-                                             lexp[lmax].as<float>(), neq == FMMX_GNI, neq == FMMX_G, cfg.convention,
-                                             lmax, xmin, ymin, zmin, rb(lmax), corr, c4(), vs.as<double>(),
-                                             gxs.as<double>(), gys.as<double>(), gzs.as<double>());
+    // Yukawa locals carry the kernel's constant: phi = sum L~ h_n S_n(u)
+    const double lrb = lam * rb(lmax), cl = yk ? rb(lmax) : c4();
+    const int grad = neq == FMMX_G || neq == FMMX_YUKAWA;
+#define FMMX_L2P(T_, YK_)                                                                                          \
+  k_l2p<P, T_, YK_><<<nb, 64, 0, stream>>>(nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(),          \
+                                           si.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), \
+                                           si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),                 \
+                                           lexp[lmax].as<T_>(), neq == FMMX_GNI, grad, cfg.convention, lmax, xmin,  \
+                                           ymin, zmin, rb(lmax), corr, cl, vs.as<double>(), gxs.as<double>(),      \
+                                           gys.as<double>(), gzs.as<double>(), lrb)
+    if (fp64) {
+      if (yk) FMMX_L2P(double, 1);
+      else FMMX_L2P(double, 0);
+    } else {
+      if (yk) FMMX_L2P(float, 1);
+      else FMMX_L2P(float, 0);
+    }
+#undef FMMX_L2P
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
@@ -1052,13 +1151,13 @@ int Engine::run_m2m(int lev) {
     if (fp64) {
       k_translate<V, double, 0><<<grid, TR_NT, 0, stream>>>(
           nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
-          mexp[lev + 1].as<double>(), m2m_part.as<double>(), op_m2m.as<double>());
+          mexp[lev + 1].as<double>(), m2m_part.as<double>(), m2m_ops<double>(lev));
       k_m2l_reduce<double><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, 8, m2m_part.as<double>(),
                                                                        mexp[lev].as<double>(), 1);
     } else {
       k_translate<V, float, 0><<<grid, TR_NT, 0, stream>>>(
           nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
-          mexp[lev + 1].as<float>(), m2m_part.as<float>(), op_m2m.as<float>());
+          mexp[lev + 1].as<float>(), m2m_part.as<float>(), m2m_ops<float>(lev));
       k_m2l_reduce<float><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, 8, m2m_part.as<float>(),
                                                                       mexp[lev].as<float>(), 1);
     }
@@ -1076,11 +1175,11 @@ int Engine::run_l2l(int lev) {
     if (fp64)
       k_translate<Q, double, 1><<<grid, TR_NT, 0, stream>>>(
           nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
-          cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<double>(), lexp[lev].as<double>(), op_l2l.as<double>());
+          cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<double>(), lexp[lev].as<double>(), l2l_ops<double>(lev));
     else
       k_translate<Q, float, 1><<<grid, TR_NT, 0, stream>>>(
           nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
-          cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<float>(), lexp[lev].as<float>(), op_l2l.as<float>());
+          cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<float>(), lexp[lev].as<float>(), l2l_ops<float>(lev));
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
@@ -1100,7 +1199,8 @@ int Engine::run_m2l(int lev, bool init) {
   const int MT = (P2 + 15) / 16, KS = (P2 + 3) / 4;
   // FP32: MFMA fragment order; FP64: k-major (transposed) operators
   const size_t FR = fp64 ? (size_t)P2 * P2 : (size_t)KS * MT * 64;
-  const size_t opoff = m2l_per_level ? (size_t)(lev - 2) * NOFF * FR : 0;
+  const size_t opoff = (yk || m2l_per_level) ? (size_t)(lev - 2) * NOFF * FR : 0;
+  DevBuf& opm2l = yk ? yop_m2l : op_m2l;
   const int tiles = (nb + M2L_TN - 1) / M2L_TN + 8;
   // enough workgroups to cover the chip: split the 189 offsets when tiles are few
   int split_env = 0;
@@ -1115,10 +1215,13 @@ int Engine::run_m2l(int lev, bool init) {
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if (fp64) {
+      const bool tmark = per_phase && lev == lmax;
+      if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
       k_m2l_vec<P, double><<<tiles * nsplit, 256, 0, stream>>>(
           lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
           sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), m2l_part.as<double>(),
-          op_m2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 1);
+          opm2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev);
+      if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
       if (nsplit > 1)
         k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<double>(),
                                                                           lexp[lev].as<double>(), init ? 1 : 0);
@@ -1138,7 +1241,9 @@ int Engine::run_m2l(int lev, bool init) {
   k_m2l<P, NT_, PA_, W_><<<8 * tpc * ns2, 64 * W_, 0, stream>>>(                                        \
       lev, nb, ns2, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),   \
       sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), m2l_part.as<float>(),     \
-      op_m2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 1)
+      opm2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev)
+      const bool tmark = per_phase && lev == lmax;
+      if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
       if (wpg == 8) {
         if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 8);
         else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 8);
@@ -1149,6 +1254,7 @@ int Engine::run_m2l(int lev, bool init) {
         else FMMX_M2L_LAUNCH(2, 0, 4);
       }
 #undef FMMX_M2L_LAUNCH
+      if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
       nsplit = ns2;
       if (nsplit > 1)
         k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<float>(),
@@ -1166,7 +1272,7 @@ int Engine::run_direct(int neq, int ni, const float* xi, const float* yi, const 
   if (ni == 0) return FMMX_OK;
   long nz4 = ni;
   k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, v);
-  if (neq == FMMX_G) {
+  if (neq == FMMX_G || neq == FMMX_YUKAWA) {
     k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gx);
     k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gy);
     k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gz);
@@ -1179,11 +1285,14 @@ int Engine::run_direct(int neq, int ni, const float* xi, const float* yi, const 
   ns = (nj + slice - 1) / slice;
   dim3 grid(tb, ns);
   const bool ref = cfg.convention == FMMX_CONV_REF;
-  const double cc = c4();
+  const double cc = neq == FMMX_YUKAWA ? c_yukawa() : c4();
 #define FMMX_DIR(NEQ, CONV)                                                                                      \
   k_direct<NEQ, CONV><<<grid, 64, 0, stream>>>(ni, xi, yi, zi, nxi, nyi, nzi, nj, xj, yj, zj, qj, nxj, nyj, nzj, \
-                                               slice, cc, v, gx, gy, gz)
-  if (ref) {
+                                               slice, cc, v, gx, gy, gz, lam)
+  if (neq == FMMX_YUKAWA) {
+    if (ref) FMMX_DIR(FMMX_YUKAWA, FMMX_CONV_REF);
+    else FMMX_DIR(FMMX_YUKAWA, FMMX_CONV_EXACT);
+  } else if (ref) {
     if (neq == FMMX_G) FMMX_DIR(FMMX_G, FMMX_CONV_REF);
     else if (neq == FMMX_GNI) FMMX_DIR(FMMX_GNI, FMMX_CONV_REF);
     else FMMX_DIR(FMMX_GNJ, FMMX_CONV_REF);

@@ -1,6 +1,6 @@
 // refshim.cpp -- the reference's C++ symbols (fmm, nlevel, boxpart, sorti,
 // sortj, unsorti, unsortj, boxallocate, alloc, dealloc, G_dir, Gni_dir,
-// Gnj_dir) over the caller's global arrays, served by one process-wide fmmx
+// Gnj_dir, Helmholtz_G_dir) over the caller's global arrays, served by one process-wide fmmx
 // handle.  Built into libfmmx_ref.so; declarations and the list of
 // deliberate differences are in include/fmmx_refshim.h.
 //
@@ -173,6 +173,7 @@ void direct(const char* fn, int neq, int n0, int n1, int n2, int n3) {
     c.convention = s.convention;
     check(fn, fmmx_create(&c, s.device, nullptr, &s.h));
   }
+  if (neq == FMMX_YUKAWA) check(fn, fmmx_set_yukawa(s.h, lambda));
   for (auto& v : s.out) v.resize(ni);
   const bool g = neq == FMMX_G;
   check(fn, fmmx_direct(s.h, neq, ni, xi + n0, yi + n0, zi + n0, neq == FMMX_GNI ? gxi + n0 : nullptr,
@@ -334,16 +335,17 @@ void dealloc(int ialloc) {
 // fmm/fmm.cxx:29-152
 void fmm(int mi, int mj, int neq) {
   const char* fn = "fmm";
-  if (neq < 0 || neq > 2)
-    die(fn, "neq " + std::to_string(neq) + " is not provided (0 G, 1 Gni, 2 Gnj; Yukawa neq 10 is not built)");
+  if ((neq < 0 || neq > 2) && neq != FMMX_YUKAWA)
+    die(fn, "neq " + std::to_string(neq) + " is not provided (0 G, 1 Gni, 2 Gnj, 10 Yukawa)");
   Shim& s = S();
   fmmx_handle* h = handle(fn);
+  if (neq == FMMX_YUKAWA) check(fn, fmmx_set_yukawa(h, lambda));
   const float* a[12] = {xi, yi, zi, nullptr, nullptr, nullptr, xj, yj, zj, nullptr, nullptr, nullptr};
   if (neq == FMMX_GNI) a[3] = gxi, a[4] = gyi, a[5] = gzi;
   if (neq == FMMX_GNJ) a[9] = gxj, a[10] = gyj, a[11] = gzj;
   int n[12];
   for (int k = 0; k < 12; k++) n[k] = k < 6 ? mi : mj;
-  const int nmode = neq;
+  const int nmode = neq == FMMX_YUKAWA ? FMMX_G : neq;  // the tree depends on the normals only
   const bool rebuild = !tree_is_current(s.tree, mi, mj, nmode, a, n);
   if (rebuild) {
     check(fn, fmmx_set_timing(h, 1));
@@ -351,7 +353,7 @@ void fmm(int mi, int mj, int neq) {
                               0));
   }
   for (auto& v : s.out) v.resize(std::max(mi, 1));
-  const bool g = neq == FMMX_G;
+  const bool g = neq == FMMX_G || neq == FMMX_YUKAWA;
   check(fn, fmmx_evaluate(h, neq, vj, s.out[0].data(), g ? s.out[1].data() : nullptr, g ? s.out[2].data() : nullptr,
                           g ? s.out[3].data() : nullptr, 0));
   // fmm/fmm.cxx:46-52 zeroes vi (and gxi.. for neq 0) before accumulating
@@ -379,3 +381,4 @@ void fmm(int mi, int mj, int neq) {
 void G_dir(int n0, int n1, int n2, int n3) { direct("G_dir", FMMX_G, n0, n1, n2, n3); }
 void Gni_dir(int n0, int n1, int n2, int n3) { direct("Gni_dir", FMMX_GNI, n0, n1, n2, n3); }
 void Gnj_dir(int n0, int n1, int n2, int n3) { direct("Gnj_dir", FMMX_GNJ, n0, n1, n2, n3); }
+void Helmholtz_G_dir(int n0, int n1, int n2, int n3) { direct("Helmholtz_G_dir", FMMX_YUKAWA, n0, n1, n2, n3); }

@@ -224,11 +224,48 @@ int Engine::sort_points(int n, const float* x, const float* y, const float* z, i
 }
 
 int Engine::build_side(Side& s, int n, const float* x, const float* y, const float* z, const float* nx,
-                       const float* ny, const float* nz, bool dense) {
+                       const float* ny, const float* nz, bool dense, const Side* same) {
   s.n = n;
   s.has_normals = nx && ny && nz;
   size_t nn = (size_t)(n > 0 ? n : 1);
   int rc;
+  if (same && n > 0) {
+    // the same points as `same` (targets = sources): its keys, order and
+    // boxes are this side's too; only normals and dense tables are built
+    s.keys.alias(same->keys);
+    s.perm.alias(same->perm);
+    s.x.alias(same->x);
+    s.y.alias(same->y);
+    s.z.alias(same->z);
+    s.blast.alias(same->blast);
+    s.cnt.alias(same->cnt);
+    for (int lev = 2; lev <= lmax; lev++) {
+      s.flag[lev].alias(same->flag[lev]);
+      s.pos[lev].alias(same->pos[lev]);
+      s.bkey[lev].alias(same->bkey[lev]);
+      s.bfirst[lev].alias(same->bfirst[lev]);
+      s.parent[lev].alias(same->parent[lev]);
+      s.cfirst[lev].alias(same->cfirst[lev]);
+    }
+    s.nb.fill(0);
+    if (s.has_normals) {
+      if ((rc = s.nx.ensure(nn * 4)) || (rc = s.ny.ensure(nn * 4)) || (rc = s.nz.ensure(nn * 4))) return rc;
+      const uint32_t* perm = s.perm.as<uint32_t>();
+      k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nx, s.nx.as<float>());
+      k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, ny, s.ny.as<float>());
+      k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nz, s.nz.as<float>());
+    }
+    if (dense) {
+      const int* cnt = s.cnt.as<int>();
+      for (int lev = 2; lev <= lmax; lev++) {
+        long nd = 1L << (3 * lev);
+        if ((rc = s.dense[lev].ensure(nd * 4))) return rc;
+        k_fill_int<<<nblk(nd), TPB, 0, stream>>>(nd, -1, s.dense[lev].as<int>());
+        k_dense<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, s.bkey[lev].as<uint32_t>(), s.dense[lev].as<int>());
+      }
+    }
+    return FMMX_OK;
+  }
   if ((rc = s.keys_in.ensure(nn * 4)) || (rc = s.idx_in.ensure(nn * 4)) || (rc = s.keys.ensure(nn * 4)) ||
       (rc = s.perm.ensure(nn * 4)) || (rc = s.x.ensure(nn * 4)) || (rc = s.y.ensure(nn * 4)) ||
       (rc = s.z.ensure(nn * 4)) || (rc = s.cnt.ensure((MAXL + 1) * 4)) || (rc = s.blast.ensure(nn * 4)))
@@ -301,8 +338,9 @@ int Engine::build_lists() {
   size_t nb1 = (size_t)(nbi > 0 ? nbi : 1);
   if ((rc = nij.ensure(nb1 * 4)) || (rc = neij.ensure(nb1 * NNEAR * 4))) return rc;
   DevBuf& ctr = pair_ctr;
-  if ((rc = ctr.ensure(64))) return rc;
-  k_zero_u64<<<1, 32, 0, stream>>>(ctr.as<unsigned long long>(), 8);
+  // [0] near pairs, [2 + lev] M2L pairs at level lev
+  if ((rc = ctr.ensure(16 * 8))) return rc;
+  k_zero_u64<<<1, 32, 0, stream>>>(ctr.as<unsigned long long>(), 16);
   if (nbi > 0 && sj.nb[lmax] > 0)
     k_near_list<<<nblk(nbi), TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
                                                si.bfirst[lmax].as<int>(), si.blast.as<int>(),

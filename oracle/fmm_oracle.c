@@ -1098,6 +1098,47 @@ void orc_direct(int neq, int conv, int nthreads, int ni, const float* xi, const 
   }
 }
 
+void orc_direct_yukawa(int conv, int nthreads, double lambda, int ni, const float* xi, const float* yi,
+                       const float* zi, int nj, const float* xj, const float* yj, const float* zj,
+                       const float* qj, double* vi, double* gxi, double* gyi, double* gzi) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < ni; i++) {
+    double p = 0, fx = 0, fy = 0, fz = 0;
+    for (int j = 0; j < nj; j++) {
+      if (conv == ORC_CONV_REF) {
+        /* kernel/Helmholtz_G_dir.cxx:13-17, float subtraction, the reference's order */
+        double dx = xi[i] - xj[j], dy = yi[i] - yj[j], dz = zi[i] - zj[j];
+        double r = sqrt(dx * dx + dy * dy + dz * dz + EPSF);
+        p += 0.5 * PIF * qj[j] / r / lambda * exp(-lambda * r);
+      } else {
+        double dx = (double)xi[i] - xj[j], dy = (double)yi[i] - yj[j], dz = (double)zi[i] - zj[j];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0) continue;
+        double r = sqrt(r2), e = exp(-lambda * r) / r;
+        p += qj[j] * e;
+        double s = -qj[j] * (1 + lambda * r) * e / r2;
+        fx += s * dx;
+        fy += s * dy;
+        fz += s * dz;
+      }
+    }
+    if (conv == ORC_CONV_REF) {
+      vi[i] = oround(conv, p);
+    } else {
+      double c = 0.5 * M_PI / lambda;
+      vi[i] = c * p;
+      if (gxi) gxi[i] = c * fx;
+      if (gyi) gyi[i] = c * fy;
+      if (gzi) gzi[i] = c * fz;
+    }
+  }
+}
+
 void orc_p2m_box(int mp, int conv, int n, const float* x, const float* y, const float* z, const float* q,
                  double cx, double cy, double cz, double* m) {
   tables T;

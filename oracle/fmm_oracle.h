@@ -94,6 +94,16 @@ void orc_direct(int neq, int conv, int nthreads,
                 const float* nxj, const float* nyj, const float* nzj,
                 const float* qj, double* vi, double* gxi, double* gyi, double* gzi);
 
+/* Yukawa direct sum (kernel/Helmholtz_G_dir.cxx:6-20): vi = sum_j q_j
+ * (pi/2) exp(-lambda r)/(lambda r).  REF: r = sqrt(r^2 + eps), self pair
+ * included, float pi, float outputs, potential only (as the reference).
+ * EXACT: unsoftened, r = 0 pairs excluded, double pi, plus the gradient
+ * grad_i = -sum_j q_j (pi/2)/lambda (1 + lambda r) exp(-lambda r) d/r^3
+ * (d = x_i - x_j; not in the reference; pinned by closed form). */
+void orc_direct_yukawa(int conv, int nthreads, double lambda, int ni, const float* xi, const float* yi,
+                       const float* zi, int nj, const float* xj, const float* yj, const float* zj,
+                       const float* qj, double* vi, double* gxi, double* gyi, double* gzi);
+
 /* ---- single-box expansion helpers for closed-form checks ---- */
 /* Multipole of point charges about centre c (kernel/G_p2m.cxx), and the
  * local-expansion evaluation at points (kernel/G_l2p.cxx) of given

@@ -61,6 +61,8 @@ def lib():
         L.orc_precalc.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp]
         L.orc_direct.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp,
                                  C.c_int, fp, fp, fp, fp, fp, fp, fp, dp, dp, dp, dp]
+        L.orc_direct_yukawa.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, fp, fp, fp, C.c_int, fp, fp, fp, fp,
+                                        dp, dp, dp, dp]
         L.orc_p2m_box.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, C.c_double, C.c_double,
                                   C.c_double, dp]
         L.orc_l2p_box.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, C.c_double, C.c_double,
@@ -207,6 +209,21 @@ def direct(xi, xj, q, *, neq=NEQ_G, conv=REF, ni_normals=None, nj_normals=None, 
     g = np.zeros((3, ni))
     lib().orc_direct(neq, conv, nthreads, ni, *map(_pf, a), *map(_pf, n1), nj, *map(_pf, b), *map(_pf, n2),
                      _pf(q), _pd(v), _pd(g[0]), _pd(g[1]), _pd(g[2]))
+    return v, g.T.copy()
+
+
+def direct_yukawa(xi, xj, q, lam, *, conv=REF, nthreads=0):
+    """kernel/Helmholtz_G_dir.cxx: potential (and, EXACT only, gradient)."""
+    xi = np.ascontiguousarray(xi, dtype=np.float32)
+    xj = np.ascontiguousarray(xj, dtype=np.float32)
+    q = _f(q)
+    ni, nj = len(xi), len(xj)
+    a = [_f(xi[:, k]) for k in range(3)]
+    b = [_f(xj[:, k]) for k in range(3)]
+    v = np.zeros(ni)
+    g = np.zeros((3, ni))
+    lib().orc_direct_yukawa(conv, nthreads, float(lam), ni, *map(_pf, a), nj, *map(_pf, b), _pf(q), _pd(v),
+                            _pd(g[0]), _pd(g[1]), _pd(g[2]))
     return v, g.T.copy()
 
 

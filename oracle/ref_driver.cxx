@@ -12,8 +12,14 @@
 // needs, all non-adjacent boxes, is neib 1.  With "asis" the driver also
 // runs the shipped fmm() itself, for the record.
 //
+// Yukawa (neq 10) runs the shipped fmm() itself, as test/testh.cxx:134-141
+// does, with lambda and scale set as test/testh.cxx:42,119-123: its M2L
+// (kernel/Helmholtz_m2l.cxx) builds its own directional lists and does not
+// read the level-2 list.
+//
 // Input (binary, little endian): int ni,nj,lmax,neq,asis; float xmin,ymin,
-// zmin,rd; float xi,yi,zi,gxi,gyi,gzi [ni]; float xj,yj,zj,gxj,gyj,gzj,vj [nj].
+// zmin,rd; float xi,yi,zi,gxi,gyi,gzi [ni]; float xj,yj,zj,gxj,gyj,gzj,vj [nj];
+// for neq 10 one more float, lambda.
 // Output: records {char name[32]; int type(0 int,1 float,2 double); int n; data}.
 #define MAIN
 #include "constants.h"
@@ -53,6 +59,7 @@ extern void m2l(int, int, int, int, int, int);
 extern void l2l(int, int, int, int);
 extern void G_l2p(int, int, int);
 extern void Gn_l2p(int, int, int);
+extern void Helmholtz_G_dir(int, int, int, int);
 
 static FILE* out;
 
@@ -190,6 +197,12 @@ int main(int argc, char** argv) {
   read_floats(fi, gxi, ni); read_floats(fi, gyi, ni); read_floats(fi, gzi, ni);
   read_floats(fi, xj, nj); read_floats(fi, yj, nj); read_floats(fi, zj, nj);
   read_floats(fi, gxj, nj); read_floats(fi, gyj, nj); read_floats(fi, gzj, nj); read_floats(fi, vj, nj);
+  if (neq == 10) {
+    float lam;
+    read_floats(fi, &lam, 1);
+    lambda = lam;
+    scale = rd * lambda > 1 ? 1 / rd : lambda;  // test/testh.cxx:119-123
+  }
   fclose(fi);
   std::vector<float> x0(xi, xi + ni), y0(yi, yi + ni), z0(zi, zi + ni), nx0(gxi, gxi + ni),
       ny0(gyi, gyi + ni), nz0(gzi, gzi + ni);
@@ -209,7 +222,13 @@ int main(int argc, char** argv) {
   alloc(1);
   reci("nbi", ni, nbi);
   reci("nbj", nj, nbj);
-  fmm_traced(ni, nj, neq);
+  if (neq == 10) {
+    double sc[2] = {lambda, scale};
+    recd("yukawa", 2, sc);
+    fmm(ni, nj, neq);
+  } else {
+    fmm_traced(ni, nj, neq);
+  }
   {
     int nmp = mp * (mp + 1) / 2;
     (void)nmp;
@@ -254,7 +273,8 @@ int main(int argc, char** argv) {
   }
   if (neq == 0) G_dir(0, ni - 1, 0, nj - 1);
   else if (neq == 1) Gni_dir(0, ni - 1, 0, nj - 1);
-  else Gnj_dir(0, ni - 1, 0, nj - 1);
+  else if (neq == 2) Gnj_dir(0, ni - 1, 0, nj - 1);
+  else Helmholtz_G_dir(0, ni - 1, 0, nj - 1);
   recf("dir_vi", ni, vi);
   if (neq == 0) {
     recf("dir_gxi", ni, gxi);

@@ -8,7 +8,8 @@ ref_<name>.npz holding the inputs and every array the driver recorded:
 sort permutations, box keys/ranges, near and far interaction lists, precalc
 tables, multipoles after P2M and after the upward pass, local expansions per
 level, P2P-only and final potentials/gradients (sorted and caller order), the
-direct sum.  (The shipped fmm() itself is not run: with its level-2 neib=3
+direct sum.  Yukawa cases (neq 10, the y* names) hold the shipped fmm()'s
+potentials and the direct sum kernel/Helmholtz_G_dir.cxx.  (The shipped fmm() itself is not run: with its level-2 neib=3
 list it reads past the end of nfj at kernel/m2l.cxx:43, a heap overflow
 under AddressSanitizer.)
 
@@ -38,7 +39,14 @@ CASES = {
     "g2": (400, 450, 2, 0, False, False, 14),
     "g4same": (2500, 2500, 4, 0, True, False, 15),
     "gedge4": (1500, 1400, 4, 0, False, False, 16),
+    # Yukawa (neq 10), test/testh.cxx:37-42 style: lambda 0.1, same points
+    "y2": (1000, 1000, 2, 10, True, False, 17),
+    "y2l1": (1500, 1500, 2, 10, True, False, 18),
 }
+# The shipped Yukawa path runs only at lmax 2: at lmax 3 Helmholtz_m2l indexes
+# lexp1/lexp2 (kernel/Helmholtz_m2l.cxx:111-122) with stacked box numbers
+# beyond the nbne rows it allocates, and crashes.
+LAMBDA = {"y2": 0.1, "y2l1": 1.0}
 # arrays kept in every case; expansion dumps only where listed (size)
 FULL = {"g3", "gnj3", "g2"}
 
@@ -78,7 +86,7 @@ def face_nudged(n, lmax, rng):
     return x
 
 
-def write_input(path, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q):
+def write_input(path, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q, lam=None):
     box = np.array([-PI_F, -PI_F, -PI_F, np.float32(2) * PI_F], np.float32)
     with open(path, "wb") as f:
         f.write(struct.pack("<5i", ni, nj, lmax, neq, int(asis)))
@@ -92,6 +100,8 @@ def write_input(path, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q):
         for k in range(3):
             f.write(np.ascontiguousarray(nj_n[:, k]).tobytes())
         f.write(q.tobytes())
+        if neq == 10:
+            f.write(np.float32(lam).tobytes())
     return box
 
 
@@ -114,9 +124,22 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     ap.add_argument("--only", default=None)
+    ap.add_argument("--exe", default=None, help="another build of ref_driver (e.g. with AddressSanitizer)")
+    ap.add_argument("--out", default=HERE)
     a = ap.parse_args()
     subprocess.run(["make", "-s", "-C", ORACLE, "-f", "Makefile.ref", f"REF={a.ref}", "-j8"], check=True)
-    exe = os.path.join(ORACLE, "_ref", "ref_driver")
+    # The shipped Yukawa P2M writes one element past multipoleh's inr[mpcmp]
+    # (kernel/multipoleh.cxx:10-12 -> kernel/in.cxx:33-35: ribesl fills
+    # b[0..mp]); with the default stack protector that aborts.  Yukawa cases
+    # therefore run a build with AddressSanitizer in recover mode, where the
+    # extra element lands in a redzone and nothing else is overwritten (same
+    # strict floating-point flags, so the arithmetic is unchanged).
+    asan_flags = "-O1 -fno-fast-math -ffp-contract=off -w -fsanitize=address -fsanitize-recover=address"
+    if any(CASES[c][3] == 10 for c in CASES if not a.only or c == a.only):
+        subprocess.run(["make", "-s", "-C", ORACLE, "-f", "Makefile.ref", f"REF={a.ref}", "OUT=_ref/asan",
+                        f"CXXFLAGS={asan_flags}", "-j8"], check=True)
+    asan_exe = os.path.join(ORACLE, "_ref", "asan", "ref_driver")
+    asan_env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=0")
     tmp = os.path.join(ORACLE, "_ref", "tmp")
     os.makedirs(tmp, exist_ok=True)
     for name, (ni, nj, lmax, neq, same, asis, seed) in CASES.items():
@@ -129,8 +152,13 @@ def main():
             xj = face_nudged(nj, lmax, rng)
         fin = os.path.join(tmp, f"{name}.in")
         fout = os.path.join(tmp, f"{name}.out")
-        box = write_input(fin, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q)
-        subprocess.run([exe, fin, fout], check=True, stdout=subprocess.DEVNULL)
+        box = write_input(fin, ni, nj, lmax, neq, asis, xi, ni_n, xj, nj_n, q, LAMBDA.get(name))
+        if neq == 10:
+            subprocess.run([a.exe or asan_exe, fin, fout], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, env=asan_env)
+        else:
+            subprocess.run([a.exe or os.path.join(ORACLE, "_ref", "ref_driver"), fin, fout], check=True,
+                           stdout=subprocess.DEVNULL)
         d = read_dump(fout)
         keep = {}
         for k, v in d.items():
@@ -139,7 +167,7 @@ def main():
             keep[k] = v
         keep.update(in_xi=xi, in_ni=ni_n, in_xj=xj, in_nj=nj_n, in_q=q, in_box=box,
                     in_cfg=np.array([ni, nj, lmax, neq, int(same), int(asis), seed], np.int32))
-        dst = os.path.join(HERE, f"ref_{name}.npz")
+        dst = os.path.join(a.out, f"ref_{name}.npz")
         np.savez_compressed(dst, **keep)
         print(f"{name}: {len(keep)} arrays -> {os.path.relpath(dst, ROOT)} ({os.path.getsize(dst)//1024} KiB)")
         os.remove(fin)

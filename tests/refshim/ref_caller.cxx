@@ -9,6 +9,7 @@
 // usage: ref_caller <input> <output>
 // input:  int ni, nj, lmax, neq, unused; float box[4] (xmin, ymin, zmin, rd);
 //         float xi[3][ni], ni_normals[3][ni], xj[3][nj], nj_normals[3][nj], q[nj]
+//         and for neq 10 one float, lambda
 // output: records {char name[32]; int type (0 int, 1 float); int n; data}
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +24,7 @@ int *nbi, *nbj, *nek, *irank;
 double* tfmm;
 int nbmax, nbne, nbnet, lmax;
 float xmin, ymin, zmin, rd;
+double lambda;
 
 static FILE* out;
 static void rec(const char* name, int type, int n, const void* data) {
@@ -57,6 +59,11 @@ int main(int argc, char** argv) {
   float* rd_j[7] = {xj, yj, zj, gxj, gyj, gzj, vj};
   for (auto a : rd_j)
     if (std::fread(a, 4, nj, in) != (size_t)nj) return 2;
+  if (neq == 10) {
+    float lam;
+    if (std::fread(&lam, 4, 1, in) != 1) return 2;
+    lambda = lam;
+  }
   std::fclose(in);
   xmin = box[0];
   ymin = box[1];
@@ -87,7 +94,7 @@ int main(int argc, char** argv) {
   rec("fmm_vi_sorted", 1, ni, v1.data());
   rec("fmm2_vi_sorted", 1, ni, vi);
   for (int i = 0; i < ni; i++) vi[i] = v1[i];
-  if (neq == 0) fmm(ni, nj, neq);  // gradients of the single-strength call
+  if (neq == 0 || neq == 10) fmm(ni, nj, neq);  // gradients of the single-strength call
   rec("nbi", 0, ni, nbi);
   rec("nbj", 0, nj, nbj);
   unsorti(ni);
@@ -111,8 +118,10 @@ int main(int argc, char** argv) {
     G_dir(0, ni - 1, 0, nj - 1);
   else if (neq == 1)
     Gni_dir(0, ni - 1, 0, nj - 1);
-  else
+  else if (neq == 2)
     Gnj_dir(0, ni - 1, 0, nj - 1);
+  else
+    Helmholtz_G_dir(0, ni - 1, 0, nj - 1);
   dealloc(0);
   rec("dir_vi", 1, ni, vi);
   if (neq == 0) {

@@ -55,6 +55,26 @@ def test_tree_and_lists_bitexact_vs_reference(case):
 
 
 @pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_sort_matches_reference_permutation(case):
+    """fmmx_sort: keys and stable sort of one point set as sort/sorti.cxx:12-26
+    leaves them (nbi, nbj), including points within ulps of box faces."""
+    d = golden(case)
+    lmax = int(d["in_cfg"][2])
+    f = F.FMM(p=4, lmax=lmax, precision="fp64", convention="ref", box=tuple(d["in_box"]))
+    print(f"case={case}", flush=True)
+    for x, perm, leaf in ((d["in_xi"], d["nbi"], d["leaf_nfi"]), (d["in_xj"], d["nbj"], d["leaf_nfj"])):
+        k, pm = f.sort(x)
+        assert np.array_equal(pm, perm)
+        assert np.array_equal(np.unique(k), leaf)
+        assert np.all(np.diff(k) >= 0)
+    # a new domain takes effect at the next call
+    f.set_domain(2, tuple(d["in_box"]))
+    k, _ = f.sort(d["in_xi"])
+    assert k.max() < 64
+    f.close()
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
 def test_fp64_ref_matches_reference_outputs(case):
     d = golden(case)
     f, lmax, neq = _engine(d, keep_lists=False)
@@ -120,8 +140,14 @@ def test_mixed_vs_oracle(neq):
 def test_fmm_error_vs_direct_p10(precision):
     x, q = uniform_cube(100000, 31)
     f = F.FMM(p=10, lmax=4, precision=precision, convention="exact", box=(0, 0, 0, 1))
+    f.set_timing(True)
     f.set_points(x)
     v, g = f.evaluate(q)
+    # every box of the 16^3 leaf grid is occupied: the M2L pair counts of a
+    # full cube, 3,096 + 53,352 + 584,136 (levels 2, 3, 4)
+    st = f.stats()
+    assert st["m2l_pairs"] == 640584 and st["m2l_pairs_leaf"] == 584136, st
+    assert 0 < st["ms_m2l_leaf"] <= st["ms_m2l"]
     idx = np.random.default_rng(1).choice(100000, 1000, replace=False)
     dv, dg = f.direct(x[idx], x, q)
     ov, og = O.direct(x[idx], x, q, conv=O.EXACT)
@@ -243,6 +269,24 @@ def test_deterministic_and_reusable():
     c, _ = f.evaluate(q2)
     d, _ = f.evaluate((q + q2).astype(np.float32))
     assert rel_l2(d, a + c) < 1e-7
+    f.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_shared_tree_equals_separate_trees(precision):
+    """set_points(x) shares one tree between targets and sources; a copy of
+    the same points builds two.  Results are bit-identical, then again after
+    switching between the two forms on one handle."""
+    x, q = uniform_cube(30000, 81)
+    nrm = np.random.default_rng(2).standard_normal((30000, 3)).astype(np.float32)
+    f = F.FMM(p=8, lmax=4, precision=precision, convention="exact", box=(0, 0, 0, 1))
+    out = []
+    for xj in (None, x.copy(), None):
+        f.set_points(x, xj, ni_normals=nrm, nj_normals=nrm)
+        print(f"precision={precision} shared={xj is None}", flush=True)
+        out.append([f.evaluate(q, neq=k)[0] for k in (0, 1, 2)] + [f.evaluate(q)[1]])
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k]) and np.array_equal(out[0][k], out[2][k]), k
     f.close()
 
 

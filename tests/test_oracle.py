@@ -191,3 +191,33 @@ def test_multipole_translation_consistency():
     v, _ = O.fmm(x, x, q, mp=12, lmax=4, conv=O.EXACT, box=(0, 0, 0, 1))
     dv, _ = O.direct(x, x, q, conv=O.EXACT)
     assert rel_l2(v, dv) < 1e-7
+
+
+@pytest.mark.parametrize("case", ["y2", "y2l1"])
+def test_golden_yukawa_direct(case):
+    """kernel/Helmholtz_G_dir.cxx restated: bit for bit with the reference's
+    float outputs; the reference's own Yukawa FMM (plane-wave M2L at mp=9)
+    agrees with it only to about three digits."""
+    d = golden(case)
+    lam = float(d["yukawa"][0])
+    v, _ = O.direct_yukawa(d["in_xi"], d["in_xj"], d["in_q"], lam, conv=O.REF)
+    assert np.array_equal(v.astype(np.float32), d["dir_vi"])
+    assert rel_l2(d["fmm_vi"], d["dir_vi"]) < 1e-2
+
+
+def test_yukawa_direct_closed_form_and_gradient():
+    from scipy.special import spherical_kn
+
+    rng = np.random.default_rng(9)
+    x = rng.random((5, 3)).astype(np.float32)
+    y = rng.random((7, 3)).astype(np.float32)
+    q = rng.random(7).astype(np.float32)
+    lam = 1.3
+    v, g = O.direct_yukawa(x, y, q, lam, conv=O.EXACT)
+    for i in range(5):
+        d = x[i].astype(np.float64) - y.astype(np.float64)
+        r = np.linalg.norm(d, axis=1)
+        assert v[i] == pytest.approx(float(q.astype(np.float64) @ spherical_kn(0, lam * r)), rel=1e-13)
+        # grad k0(lam r) = lam k0'(lam r) d / r, k0' = -k1
+        ref = -(q.astype(np.float64) * lam * spherical_kn(1, lam * r) / r) @ d
+        assert np.allclose(g[i], ref, rtol=1e-12)

@@ -24,6 +24,8 @@ def _write_input(path, d):
             for k in range(3):
                 f.write(np.ascontiguousarray(a[:, k], dtype=np.float32).tobytes())
         f.write(np.asarray(d["in_q"], np.float32).tobytes())
+        if neq == 10:
+            f.write(np.float32(d["yukawa"][0]).tobytes())
 
 
 def _read_output(path):
@@ -57,6 +59,24 @@ def test_caller_without_device_reports_and_exits(tmp_path):
     r, _ = _run(golden("g2"), tmp_path, env={"HIP_VISIBLE_DEVICES": "-1", "ROCR_VISIBLE_DEVICES": "-1"})
     assert r.returncode == 1, (r.returncode, r.stderr)
     assert "fmmx_ref: " in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["y2", "y2l1"])
+def test_reference_caller_yukawa(case, tmp_path):
+    """fmm(ni, nj, 10) and Helmholtz_G_dir through the shim (test/testh.cxx
+    sequence): permutations bit for bit, the direct sum as the reference's,
+    the matvec closer to it than the reference's own three-digit FMM."""
+    _need_caller()
+    d = golden(case)
+    print(f"case={case}", flush=True)
+    r, o = _run(d, tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert np.array_equal(o["nbi"], d["nbi"]) and np.array_equal(o["nbj"], d["nbj"])
+    e_ref = rel_l2(d["fmm_vi"], d["dir_vi"])
+    assert rel_l2(o["fmm_vi"], d["dir_vi"]) < 0.1 * e_ref
+    assert rel_l2(o["fmm2_vi_sorted"], 2 * o["fmm_vi_sorted"]) < 1e-6
+    assert rel_l2(o["dir_vi"], d["dir_vi"]) < 1e-7
 
 
 @pytest.mark.gpu
