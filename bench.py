@@ -62,20 +62,25 @@ def run_engine(F, torch, x, q, p, lmax, precision, steps, warmup):
     for _ in range(warmup):
         f.set_points(x)
         f.evaluate(q)
-    torch.cuda.synchronize()
-    f.set_timing(True)
-    phases = {k: 0.0 for k in ("ms_tree", "ms_p2p", "ms_p2m", "ms_m2m", "ms_m2l", "ms_l2p", "ms_total",
-                               "ms_m2l_leaf")}
+    # headline: one event pair per call, near field overlapped with far field
+    f.set_timing(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         f.set_points(x)
         v, g = f.evaluate(q)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    # per-phase device times (phases serialized by their events), same steps
+    f.set_timing(True)
+    phases = {k: 0.0 for k in ("ms_tree", "ms_p2p", "ms_p2m", "ms_m2m", "ms_m2l", "ms_l2p", "ms_total",
+                               "ms_m2l_leaf")}
+    for _ in range(steps):
+        f.set_points(x)
+        f.evaluate(q)
         st = f.stats()
         for k in phases:
             phases[k] += st[k]
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
     phases = {k: v_ / steps for k, v_ in phases.items()}
     # evaluate-only (tree reused, as in a GMRES iteration)
     f.set_timing(False)

@@ -1,0 +1,287 @@
+"""Boundary-element layer over the FMM matvec: the collocation operator of
+biobem/fmm_matmult.h:9-196 with the geometry of biobem/bem3d.cxx and an
+in-repo restarted GMRES (the reference drives PETSc KSP, bio/main.cxx:391-421).
+
+Surfaces are meshed by 6-node (quadratic) triangles: nodes 0, 1, 2 are the
+vertices, 3, 4, 5 sit on the edges 0-1, 1-2, 2-0 (biobem/bem3d.cxx:359-377).
+Quadrature points carry the interpolated position, the interpolated nodal
+normal and the weight hsg = |dx/dxi x dx/deta| * w (biobem/bem3d.cxx:381-394);
+the element integral of f is 0.5 * sum_j hsg_j f_j.
+
+The operator applied to nodal values u (biobem/fmm_matmult.h):
+    y_l = (1/e1) [ sum_g 0.5 hsg_g (dG/dn_g)(x_l, y_g) u(y_g)
+                 + u_l sum_g 0.5 hsg_g (-(dG/dn_g)(x_l, y_g))
+                 + (e1 + e2)/(e1 - e2) u_l ]
+with G = 1/(4 pi r) and u(y_g) interpolated by the shape functions: the
+double layer with its singularity subtracted.  Both sums are one FMM matvec
+with the Gnj kernel (targets: nodes, sources: quadrature points); the second
+does not depend on u and is computed once.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import FMM, GNJ
+
+# ---- quadrature on the reference triangle (biobem/bem3d.cxx:9-245) ---------
+# Published symmetric rules (Strang-Fix / Dunavant): points (xi, eta) and
+# weights normalised to sum 1.
+_RULES = {
+    1: ([(1 / 3, 1 / 3)], [1.0]),
+    3: ([(1 / 6, 1 / 6), (2 / 3, 1 / 6), (1 / 6, 2 / 3)], [1 / 3] * 3),
+    4: ([(1 / 3, 1 / 3), (0.2, 0.2), (0.6, 0.2), (0.2, 0.6)], [-27 / 48, 25 / 48, 25 / 48, 25 / 48]),
+}
+
+
+def _six(al, be, ga, de, o1, o2):
+    pts = [(de, de), (al, de), (de, al), (be, be), (ga, be), (be, ga)]
+    return pts, [o1] * 3 + [o2] * 3
+
+
+_RULES[6] = _six(0.816847572980459, 0.445948490915965, 0.108103018168070, 0.091576213509771,
+                 0.109951743655322, 0.223381589678011)
+# 7 points: 0.797426985353087 = 1 - 2 * 0.101286507323456 (the reference has
+# 0.797426958353087 at biobem/bem3d.cxx:73, two digits swapped)
+_p7, _w7 = _six(0.797426985353087, 0.470142064105115, 0.059715871789770, 0.101286507323456,
+                0.125939180544827, 0.132394152788506)
+_RULES[7] = (_p7 + [(1 / 3, 1 / 3)], _w7 + [0.225])
+
+
+def gauss_trgl(ngl=3):
+    """(xi, eta, w) of the ngl-point rule; ngl in 1, 3, 4, 6, 7
+    (include/constants.h:33 uses 3)."""
+    if ngl not in _RULES:
+        raise ValueError(f"ngl must be one of {sorted(_RULES)}")
+    pts, w = _RULES[ngl]
+    pts = np.asarray(pts, float)
+    return pts[:, 0].copy(), pts[:, 1].copy(), np.asarray(w, float)
+
+
+# ---- 6-node triangle (biobem/bem3d.cxx:248-323) ------------------------------
+def interp_p(X, al, be, ga, xi, eta):
+    """Shape functions, tangents and unit normal of 6-node triangles.
+    X: (ne, 6, 3) node coordinates; al, be, ga: (ne,) edge-node parameters
+    of nodes 3, 5, 4; xi, eta: scalars.  Returns ph (ne, 6), dxdxi, dxdeta,
+    unit normal (ne, 3 each) and the surface metric hsd (ne,)."""
+    alc, bec, gac = 1 - al, 1 - be, 1 - ga
+    alalc, bebec, gagac = al * alc, be * bec, ga * gac
+    ph2 = xi * (xi - al + eta * (al - ga) / gac) / alc
+    ph3 = eta * (eta - be + xi * (be + ga - 1) / ga) / bec
+    ph4 = xi * (1 - xi - eta) / alalc
+    ph5 = xi * eta / gagac
+    ph6 = eta * (1 - xi - eta) / bebec
+    ph1 = 1 - ph2 - ph3 - ph4 - ph5 - ph6
+    d2 = (2 * xi - al + eta * (al - ga) / gac) / alc
+    d3 = eta * (be + ga - 1) / (ga * bec)
+    d4 = (1 - 2 * xi - eta) / alalc
+    d5 = eta / gagac
+    d6 = -eta / bebec
+    d1 = -d2 - d3 - d4 - d5 - d6
+    e2 = xi * (al - ga) / (alc * gac)
+    e3 = (2 * eta - be + xi * (be + ga - 1) / ga) / bec
+    e4 = -xi / alalc
+    e5 = xi / gagac
+    e6 = (1 - xi - 2 * eta) / bebec
+    e1 = -e2 - e3 - e4 - e5 - e6
+    ph = np.stack([ph1, ph2, ph3, ph4, ph5, ph6], 1)
+    dxi = np.einsum("ek,ekc->ec", np.stack([d1, d2, d3, d4, d5, d6], 1), X)
+    det = np.einsum("ek,ekc->ec", np.stack([e1, e2, e3, e4, e5, e6], 1), X)
+    vn = np.cross(dxi, det)
+    hsd = np.linalg.norm(vn, axis=1)
+    return ph, dxi, det, vn / hsd[:, None], hsd
+
+
+class Mesh:
+    """Nodes (nwn, 3) and 6-node triangles (nwe, 6) with the per-element
+    quadrature data of biobem/bem3d.cxx:325-542 (elm_geom)."""
+
+    def __init__(self, nodes, tri6, ngl=3):
+        self.nodes = np.asarray(nodes, float)
+        self.tri = np.asarray(tri6, np.int64)
+        self.ngl = ngl
+        X = self.nodes[self.tri]  # (ne, 6, 3)
+        dist = lambda a, b: np.linalg.norm(X[:, a] - X[:, b], axis=1)  # noqa: E731
+        # edge-node positions along their edges (biobem/bem3d.cxx:360-365)
+        al = 1 / (1 + dist(3, 1) / dist(3, 0))
+        be = 1 / (1 + dist(5, 2) / dist(5, 0))
+        ga = 1 / (1 + dist(4, 1) / dist(4, 2))
+        xq, eq, wq = gauss_trgl(ngl)
+        ne = len(self.tri)
+        self.ph = np.zeros((ne, ngl, 6))
+        self.hsg = np.zeros((ne, ngl))
+        for j in range(ngl):
+            ph, _, _, _, hsd = interp_p(X, al, be, ga, xq[j], eq[j])
+            self.ph[:, j] = ph
+            self.hsg[:, j] = hsd * wq[j]
+        self.area = 0.5 * self.hsg.sum(1)
+        # nodal normals: average of the element normals at the node
+        # (biobem/bem3d.cxx:396-407, 503-512)
+        vna = np.zeros_like(self.nodes)
+        cnt = np.zeros(len(self.nodes))
+        zero, one = np.zeros(ne), np.ones(ne)
+        node_xi_eta = [(zero, zero), (one, zero), (zero, one), (al, zero), (ga, 1 - ga), (zero, be)]
+        for i, (xi, eta) in enumerate(node_xi_eta):
+            _, _, _, vn, _ = interp_p(X, al, be, ga, xi, eta)
+            np.add.at(vna, self.tri[:, i], vn)
+            np.add.at(cnt, self.tri[:, i], 1)
+        vna /= cnt[:, None]
+        self.normals = vna / np.linalg.norm(vna, axis=1)[:, None]
+        # quadrature points: interpolated positions and (unnormalised) normals
+        # (biobem/fmm_matmult.h:80-98)
+        self.qx = np.einsum("egk,ekc->egc", self.ph, X).reshape(-1, 3)
+        self.qn = np.einsum("egk,ekc->egc", self.ph, self.normals[self.tri]).reshape(-1, 3)
+        self.qw = 0.5 * self.hsg.reshape(-1)
+
+    @property
+    def nwn(self):
+        return len(self.nodes)
+
+    @property
+    def nwe(self):
+        return len(self.tri)
+
+    def interpolate(self, u):
+        """Nodal values -> values at the quadrature points (ne*ngl,)."""
+        return np.einsum("egk,ek->eg", self.ph, np.asarray(u)[self.tri]).reshape(-1)
+
+
+def icosphere(subdiv, radius=1.0, ngl=3):
+    """Sphere meshed by 20*4^subdiv 6-node triangles: an icosahedron refined
+    subdiv times, edge nodes at the projected edge midpoints."""
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    V = [np.array(x, float) / np.linalg.norm(x) for x in v]
+    F = list(f)
+
+    def midpoints(V, F):
+        cache = {}
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                cache[k] = len(V) - 1
+            return cache[k]
+
+        return mid
+
+    for _ in range(subdiv):
+        mid = midpoints(V, F)
+        G = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            G += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = G
+    mid = midpoints(V, F)
+    tri6 = np.array([(a, b, c, mid(a, b), mid(b, c), mid(c, a)) for a, b, c in F], np.int64)
+    return Mesh(np.array(V) * radius, tri6, ngl)
+
+
+class Collocation:
+    """The collocation operator of biobem/fmm_matmult.h:9-196 on a mesh, with
+    inner and outer dielectric constants e1, e2, applied through the FMM."""
+
+    def __init__(self, mesh: Mesh, e1=1.0, e2=80.0, p=10, precision="mixed", lmax=0):
+        self.mesh, self.e1, self.e2 = mesh, float(e1), float(e2)
+        self.f = FMM(p=p, lmax=lmax, precision=precision, convention="exact")
+        self._xi = np.ascontiguousarray(mesh.nodes, np.float32)
+        self._xj = np.ascontiguousarray(mesh.qx, np.float32)
+        self._nj = np.ascontiguousarray(mesh.qn, np.float32)
+        self.f.set_points(self._xi, self._xj, nj_normals=self._nj)
+        # the u-independent sum: sum_g 0.5 hsg (-(dG/dn_g)) at each node
+        k1, _ = self.f.evaluate((-mesh.qw).astype(np.float32), neq=GNJ)
+        self.k1 = k1
+        self.diag = (self.e1 + self.e2) / (self.e1 - self.e2)
+
+    def matvec(self, u):
+        u = np.asarray(u, np.float64)
+        s = (self.mesh.qw * self.mesh.interpolate(u)).astype(np.float32)
+        v, _ = self.f.evaluate(s, neq=GNJ)
+        return (v + (self.k1 + self.diag) * u) / self.e1
+
+    def precondition(self, x):
+        """biobem/fmm_matmult.h:198-214: scaling by 2 e1 (e1 - e2)/(e1 + e2)."""
+        return np.asarray(x) / (self.e1 + self.e2) * (2 * self.e1 * (self.e1 - self.e2))
+
+    def close(self):
+        self.f.close()
+
+
+def gmres(matvec, b, x0=None, precond=None, rtol=1e-6, restart=50, maxiter=500):
+    """Right-preconditioned restarted GMRES, FP64 arithmetic, modified
+    Gram-Schmidt with one reorthogonalisation pass.  Returns (x, residual
+    norms relative to |b|, iterations)."""
+    b = np.asarray(b, np.float64)
+    M = precond or (lambda v: v)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
+    nb = np.linalg.norm(b)
+    if nb == 0:
+        return x, [0.0], 0
+    hist, it = [], 0
+    r = b - matvec(x)
+    beta = np.linalg.norm(r)
+    hist.append(beta / nb)
+    while hist[-1] > rtol and it < maxiter:
+        m = min(restart, maxiter - it)
+        V = np.zeros((m + 1, len(b)))
+        H = np.zeros((m + 1, m))
+        V[0] = r / beta
+        g = np.zeros(m + 1)
+        g[0] = beta
+        cs, sn = np.zeros(m), np.zeros(m)
+        k = 0
+        for k in range(m):
+            w = matvec(M(V[k]))
+            for _ in range(2):  # MGS, then once more
+                for i in range(k + 1):
+                    h = V[i] @ w
+                    H[i, k] += h
+                    w = w - h * V[i]
+            H[k + 1, k] = np.linalg.norm(w)
+            if H[k + 1, k] > 0:
+                V[k + 1] = w / H[k + 1, k]
+            for i in range(k):
+                t = cs[i] * H[i, k] + sn[i] * H[i + 1, k]
+                H[i + 1, k] = -sn[i] * H[i, k] + cs[i] * H[i + 1, k]
+                H[i, k] = t
+            d = np.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = H[k, k] / d, H[k + 1, k] / d
+            H[k, k] = d
+            H[k + 1, k] = 0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            it += 1
+            hist.append(abs(g[k + 1]) / nb)
+            if hist[-1] <= rtol:
+                break
+        kk = k + 1
+        y = np.linalg.solve(np.triu(H[:kk, :kk]), g[:kk])
+        x = x + M(V[:kk].T @ y)
+        r = b - matvec(x)
+        beta = np.linalg.norm(r)
+        hist[-1] = beta / nb  # the true residual
+        if beta == 0:
+            break
+    return x, hist, it
+
+
+def solve_refined(op_low, op_high, b, rtol=1e-10, inner_rtol=1e-5, restart=50, maxiter=400, sweeps=10):
+    """Iterative refinement: residuals with the FP64 operator op_high, each
+    correction solved by GMRES on the mixed-precision op_low."""
+    x = np.zeros_like(np.asarray(b, np.float64))
+    nb = np.linalg.norm(b)
+    hist = []
+    for _ in range(sweeps):
+        r = b - op_high.matvec(x)
+        hist.append(np.linalg.norm(r) / nb)
+        if hist[-1] <= rtol:
+            break
+        d, _, _ = gmres(op_low.matvec, r, precond=op_low.precondition, rtol=inner_rtol, restart=restart,
+                        maxiter=maxiter)
+        x = x + d
+    return x, hist

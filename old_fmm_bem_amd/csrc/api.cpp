@@ -238,6 +238,7 @@ int upload(DevBuf& dst, const std::vector<double>& src, hipStream_t s) {
 }  // namespace
 
 Engine::~Engine() {
+  if (aux) (void)hipStreamDestroy(aux);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
 }
@@ -262,6 +263,7 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   fp64 = c->precision == FMMX_FP64;
   user_box = c->rd > 0;
   for (auto& e : ev) FMMX_HIP(hipEventCreate(&e));
+  FMMX_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
   tab = *get_tables(p, c->convention);
   // device tables: table-ratio corrections and the M2L offset table
   std::vector<double> corr(tab.corr);
@@ -476,8 +478,22 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   e = 3;
   k_zero_f64<<<1, 64, 0, stream>>>(14, pair_ctr.as<double>() + 2);  // M2L pair counters per level
   if (nj > 0) k_gather_q<<<nblk(nj), TPB, 0, stream>>>(nj, sj.perm.as<uint32_t>(), q, qs.as<float>());
+  // Without per-phase timing the near field (vector ALU) runs on the second
+  // stream beside the far field (MFMA); L2P, which adds to its results, waits.
+  const bool overlap = !per_phase && aux && ni > 0;
   if (ni > 0) {
-    if ((rc = run_p2p(neq))) return rc;
+    if (overlap) {
+      FMMX_HIP(hipEventRecord(ev[10], stream));
+      FMMX_HIP(hipStreamWaitEvent(aux, ev[10], 0));
+      hipStream_t main = stream;
+      stream = aux;
+      rc = run_p2p(neq);
+      stream = main;
+      if (rc) return rc;
+      FMMX_HIP(hipEventRecord(ev[11], aux));
+    } else if ((rc = run_p2p(neq))) {
+      return rc;
+    }
     if ((rc = mark())) return rc;  // ev3: after P2P
     if (far && (rc = run_p2m(neq))) return rc;
     if ((rc = mark())) return rc;  // ev4: after P2M
@@ -490,6 +506,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
       if ((rc = run_m2l(l, false))) return rc;
     }
     if ((rc = mark())) return rc;  // ev6: after M2L+L2L
+    if (overlap) FMMX_HIP(hipStreamWaitEvent(stream, ev[11], 0));
     if (far && (rc = run_l2p(neq))) return rc;
     if ((rc = mark())) return rc;  // ev7: after L2P
   }

@@ -78,6 +78,7 @@ struct Engine {
   fmmx_config cfg{};
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;  // second stream: the near field beside the far field
   int p = 0, P2 = 0, lmax = 0;
   // multipoles are stored padded for the M2L B operand: real slot c lives at
   // (c / KS) * KSP + c % KS, KS = ceil(P2/4), KSP = KS rounded up to 4; VS = 4*KSP

@@ -272,6 +272,26 @@ def test_deterministic_and_reusable():
     f.close()
 
 
+def test_overlapped_and_serialized_phases_agree():
+    """Without per-phase timing the near field runs on a second stream beside
+    the far field; with it the phases run in order.  Same bits either way,
+    for every kernel."""
+    x, q = uniform_cube(50000, 82)
+    nrm = np.random.default_rng(3).standard_normal((50000, 3)).astype(np.float32)
+    f = F.FMM(p=10, lmax=4, precision="mixed", convention="exact", box=(0, 0, 0, 1), lam=2.0)
+    f.set_points(x, x, ni_normals=nrm, nj_normals=nrm)
+    for neq in (F.G, F.GNI, F.GNJ, F.YUKAWA):
+        print(f"neq={neq}", flush=True)
+        f.set_timing(False)
+        a = f.evaluate(q, neq=neq)
+        f.set_timing(True)
+        b = f.evaluate(q, neq=neq)
+        assert np.array_equal(a[0], b[0])
+        if a[1] is not None:
+            assert np.array_equal(a[1], b[1])
+    f.close()
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_shared_tree_equals_separate_trees(precision):
     """set_points(x) shares one tree between targets and sources; a copy of
