@@ -239,6 +239,7 @@ int upload(DevBuf& dst, const std::vector<double>& src, hipStream_t s) {
 
 Engine::~Engine() {
   if (aux) (void)hipStreamDestroy(aux);
+  if (aux2) (void)hipStreamDestroy(aux2);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
 }
@@ -264,6 +265,7 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   user_box = c->rd > 0;
   for (auto& e : ev) FMMX_HIP(hipEventCreate(&e));
   FMMX_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+  FMMX_HIP(hipStreamCreateWithFlags(&aux2, hipStreamNonBlocking));
   tab = *get_tables(p, c->convention);
   // device tables: table-ratio corrections and the M2L offset table
   std::vector<double> corr(tab.corr);
@@ -500,10 +502,33 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     for (int l = lmax - 1; far && l >= 2; l--)
       if ((rc = run_m2m(l))) return rc;
     if ((rc = mark())) return rc;  // ev5: after M2M
-    if (far && (rc = run_m2l(2, true))) return rc;
-    for (int l = 3; far && l <= lmax; l++) {
-      if ((rc = run_l2l(l))) return rc;
-      if ((rc = run_m2l(l, false))) return rc;
+    if (far && overlap && lmax >= 3 && m2l_splits(lmax) > 1) {
+      // levels 2..lmax-1 (M2L, L2L) on the third stream while the leaf M2L
+      // computes its partial sums; then the leaf L2L and the reduction that
+      // adds the partials (the serialized order below, bit for bit)
+      FMMX_HIP(hipEventRecord(ev[12], stream));
+      FMMX_HIP(hipStreamWaitEvent(aux2, ev[12], 0));
+      hipStream_t main = stream;
+      stream = aux2;
+      part_buf = &m2l_part2;
+      rc = run_m2l(2, true);
+      for (int l = 3; !rc && l < lmax; l++) {
+        if (!(rc = run_l2l(l))) rc = run_m2l(l, false);
+      }
+      stream = main;
+      part_buf = &m2l_part;
+      if (rc) return rc;
+      FMMX_HIP(hipEventRecord(ev[13], aux2));
+      if ((rc = run_m2l(lmax, false, 1))) return rc;
+      FMMX_HIP(hipStreamWaitEvent(stream, ev[13], 0));
+      if ((rc = run_l2l(lmax))) return rc;
+      if ((rc = run_m2l(lmax, false, 2))) return rc;
+    } else {
+      if (far && (rc = run_m2l(2, true))) return rc;
+      for (int l = 3; far && l <= lmax; l++) {
+        if ((rc = run_l2l(l))) return rc;
+        if ((rc = run_m2l(l, false))) return rc;
+      }
     }
     if ((rc = mark())) return rc;  // ev6: after M2L+L2L
     if (overlap) FMMX_HIP(hipStreamWaitEvent(stream, ev[11], 0));

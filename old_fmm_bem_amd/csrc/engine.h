@@ -78,7 +78,8 @@ struct Engine {
   fmmx_config cfg{};
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t aux = nullptr;  // second stream: the near field beside the far field
+  hipStream_t aux = nullptr;   // second stream: the near field beside the far field
+  hipStream_t aux2 = nullptr;  // third stream: the upper levels' M2L/L2L beside the leaf M2L
   int p = 0, P2 = 0, lmax = 0;
   // multipoles are stored padded for the M2L B operand: real slot c lives at
   // (c / KS) * KSP + c % KS, KS = ceil(P2/4), KSP = KS rounded up to 4; VS = 4*KSP
@@ -100,6 +101,8 @@ struct Engine {
   int cls_cnt_h[MAXL + 1][8] = {};  // host copy of the class counts
   // operators (precision of the evaluation): M2M[8], L2L[8], M2L[levels][NOFF]
   DevBuf op_m2m, op_l2l, op_m2l, m2l_part, m2m_part;  // op_m2l in MFMA fragment order
+  DevBuf m2l_part2;               // M2L partial sums of the levels above the leaf (third stream)
+  DevBuf* part_buf = &m2l_part;   // partial-sum buffer of the M2L launches being issued
   bool m2l_per_level = false;
   DevBuf offtab;  // int4 (dx,dy,dz,opidx) per (class, 189)
   DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)
@@ -153,7 +156,8 @@ struct Engine {
   int run_p2p(int neq);
   int run_p2m(int neq);
   int run_m2m(int lev);
-  int run_m2l(int lev, bool init);
+  int m2l_splits(int lev) const;
+  int run_m2l(int lev, bool init, int phase = 0);
   int run_l2l(int lev);
   int run_l2p(int neq);
   int run_direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,

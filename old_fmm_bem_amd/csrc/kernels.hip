@@ -331,42 +331,53 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
     T Sr[NR], Si[NR];
     solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
     T phi = T(0), gx = T(0), gy = T(0), gz = T(0);
-    double hn[P + 1], h0[P];
-    if (YK) {
+    if constexpr (YK) {
+      double hn[P + 1], h0[P];
       ibar_dev<P - 1>(lrb, h0);
       ibar_dev<P>(lrb * sqrt(ux * ux + uy * uy + uz * uz), hn);
-    }
 #pragma unroll
-    for (int n = 0; n < P; n++) {
-      T pn = T(0), gxn = T(0), gyn = T(0), gzn = T(0);
+      for (int n = 0; n < P; n++) {
+        T pn = T(0), gxn = T(0), gyn = T(0), gzn = T(0);
 #pragma unroll
-      for (int m = 0; m <= n; m++) {
-        const int s = sl(n, m);
-        const T lr = Lr[s], li = Li[s];
-        pn += Sr[s] * lr - Si[s] * li;
-        if (n >= 1) {
-          T pr, pi, mr, mi, zr, zi;
-          harm_grad<P, T>(Sr, Si, n, m, pr, pi, mr, mi, zr, zi);
-          // Re(Ap L), Im(Ap L), Re(Am L), Im(Am L)
-          const T apr = pr * lr - pi * li, api = pr * li + pi * lr;
-          const T amr = mr * lr - mi * li, ami = mr * li + mi * lr;
-          gxn += T(0.5) * (apr + amr);
-          gyn += T(0.5) * (api - ami);
-          gzn += zr * lr - zi * li;
+        for (int m = 0; m <= n; m++) {
+          const int s = sl(n, m);
+          const T lr = Lr[s], li = Li[s];
+          pn += Sr[s] * lr - Si[s] * li;
+          if (n >= 1) {
+            T pr, pi, mr, mi, zr, zi;
+            harm_grad<P, T>(Sr, Si, n, m, pr, pi, mr, mi, zr, zi);
+            const T apr = pr * lr - pi * li, api = pr * li + pi * lr;
+            const T amr = mr * lr - mi * li, ami = mr * li + mi * lr;
+            gxn += T(0.5) * (apr + amr);
+            gyn += T(0.5) * (api - ami);
+            gzn += zr * lr - zi * li;
+          }
         }
-      }
-      if (YK) {
         const T h = T(hn[n] / h0[n]), e = T(lrb * lrb * hn[n + 1] / h0[n]) * pn;
         phi += h * pn;
         gx += h * gxn + e * T(ux);
         gy += h * gyn + e * T(uy);
         gz += h * gzn + e * T(uz);
-      } else {
-        phi += pn;
-        gx += gxn;
-        gy += gyn;
-        gz += gzn;
       }
+    } else {
+#pragma unroll
+      for (int n = 0; n < P; n++)
+#pragma unroll
+        for (int m = 0; m <= n; m++) {
+          const int s = sl(n, m);
+          const T lr = Lr[s], li = Li[s];
+          phi += Sr[s] * lr - Si[s] * li;
+          if (n >= 1) {
+            T pr, pi, mr, mi, zr, zi;
+            harm_grad<P, T>(Sr, Si, n, m, pr, pi, mr, mi, zr, zi);
+            // Re(Ap L), Im(Ap L), Re(Am L), Im(Am L)
+            const T apr = pr * lr - pi * li, api = pr * li + pi * lr;
+            const T amr = mr * lr - mi * li, ami = mr * li + mi * lr;
+            gx += T(0.5) * (apr + amr);
+            gy += T(0.5) * (api - ami);
+            gz += zr * lr - zi * li;
+          }
+        }
     }
     const double s1 = c4 * rbinv, s2 = c4 * rbinv * rbinv;
     if (gn) {
@@ -1185,12 +1196,33 @@ int Engine::run_l2l(int lev) {
   });
 }
 
-int Engine::run_m2l(int lev, bool init) {
+// Offset splits of the M2L launch at a level (partial sums when > 1).
+int Engine::m2l_splits(int lev) const {
+  const int nb = si.nb[lev];
+  int split_env = 0;
+  if (const char* e = getenv("FMMX_M2L_SPLIT")) split_env = atoi(e);
+  if (fp64) {
+    const int tiles = (nb + M2L_TN - 1) / M2L_TN + 8;
+    return std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / tiles));
+  }
+  int wpg = 8, nt = 1;
+  if (const char* e = getenv("FMMX_M2L_NT")) nt = atoi(e) == 2 ? 2 : 1;
+  if (const char* e = getenv("FMMX_M2L_WPG")) wpg = atoi(e) == 4 ? 4 : 8;
+  int cmax = 0;
+  for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
+  const int tpc = std::max(1, (cmax + 16 * wpg * nt - 1) / (16 * wpg * nt));
+  return std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / (8 * tpc)));
+}
+
+// phase 0: the M2L launch and (with splits) the reduction into the locals;
+// 1: the launch alone, partial sums only (needs m2l_splits(lev) > 1);
+// 2: the reduction alone, adding the partials of phase 1 to the locals
+int Engine::run_m2l(int lev, bool init, int phase) {
   const int nb = si.nb[lev];
   if (nb == 0) return FMMX_OK;
   const long nl = (long)nb * P2;
   if (sj.nb[lev] == 0) {
-    if (init) {
+    if (init && phase != 1) {
       if (fp64) k_zero_t<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, lexp[lev].as<double>());
       else k_zero_t<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, lexp[lev].as<float>());
     }
@@ -1203,12 +1235,22 @@ int Engine::run_m2l(int lev, bool init) {
   DevBuf& opm2l = yk ? yop_m2l : op_m2l;
   const int tiles = (nb + M2L_TN - 1) / M2L_TN + 8;
   // enough workgroups to cover the chip: split the 189 offsets when tiles are few
-  int split_env = 0;
-  if (const char* e = getenv("FMMX_M2L_SPLIT")) split_env = atoi(e);
-  int nsplit = std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / tiles));
+  int nsplit = m2l_splits(lev);
   const size_t es = fp64 ? 8 : 4;
-  if (fp64 && nsplit > 1) {
-    int rc = m2l_part.ensure((size_t)nsplit * nl * es);
+  if (phase == 2) {
+    if (nsplit < 2) return fail(FMMX_ESTATE, "M2L reduction without partial sums");
+    if (fp64)
+      k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
+                                                                        lexp[lev].as<double>(), init ? 1 : 0);
+    else
+      k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<float>(),
+                                                                       lexp[lev].as<float>(), init ? 1 : 0);
+    FMMX_HIP(hipGetLastError());
+    return FMMX_OK;
+  }
+  if (phase == 1 && nsplit < 2) return fail(FMMX_ESTATE, "M2L partial sums need offset splits");
+  if (nsplit > 1) {
+    int rc = part_buf->ensure((size_t)nsplit * nl * es);
     if (rc) return rc;
   }
   int rc2 = FMMX_OK;
@@ -1219,11 +1261,11 @@ int Engine::run_m2l(int lev, bool init) {
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
       k_m2l_vec<P, double><<<tiles * nsplit, 256, 0, stream>>>(
           lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
-          sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), m2l_part.as<double>(),
+          sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), part_buf->as<double>(),
           opm2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev);
       if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
-      if (nsplit > 1)
-        k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<double>(),
+      if (nsplit > 1 && phase == 0)
+        k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
                                                                           lexp[lev].as<double>(), init ? 1 : 0);
     } else {
       // variant: N-tiles per wave (1|2) and operator prefetch (0|1)
@@ -1235,12 +1277,11 @@ int Engine::run_m2l(int lev, bool init) {
       int cmax = 0;
       for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
       const int tpc = std::max(1, (cmax + 16 * wpg * nt - 1) / (16 * wpg * nt));
-      const int ns2 = std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / (8 * tpc)));
-      if (ns2 > 1 && (rc2 = m2l_part.ensure((size_t)ns2 * nl * es))) return rc2;
+      const int ns2 = nsplit;
 #define FMMX_M2L_LAUNCH(NT_, PA_, W_)                                                                    \
   k_m2l<P, NT_, PA_, W_><<<8 * tpc * ns2, 64 * W_, 0, stream>>>(                                        \
       lev, nb, ns2, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),   \
-      sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), m2l_part.as<float>(),     \
+      sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), part_buf->as<float>(),     \
       opm2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev)
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
@@ -1255,9 +1296,8 @@ int Engine::run_m2l(int lev, bool init) {
       }
 #undef FMMX_M2L_LAUNCH
       if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
-      nsplit = ns2;
-      if (nsplit > 1)
-        k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, m2l_part.as<float>(),
+      if (nsplit > 1 && phase == 0)
+        k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<float>(),
                                                                          lexp[lev].as<float>(), init ? 1 : 0);
     }
     FMMX_HIP(hipGetLastError());

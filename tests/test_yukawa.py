@@ -122,7 +122,7 @@ def test_yukawa_vs_reference(case, precision):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("lam", [0.5, 4.0, 30.0])
-@pytest.mark.parametrize("precision,p,tol", [("fp64", 12, 1e-6), ("fp64", 8, 1e-4), ("mixed", 10, 2e-5)])
+@pytest.mark.parametrize("precision,p,tol", [("fp64", 12, 5e-6), ("fp64", 8, 1e-4), ("mixed", 10, 2e-5)])
 def test_yukawa_vs_direct(lam, precision, p, tol):
     """Unsoftened Yukawa FMM (self pair excluded) against the oracle's FP64
     direct sum, potential and gradient, on 600 sampled targets."""
