@@ -618,6 +618,7 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
           s_ = dense_j[morton_encode(jx, jy, jz, lev)];                                       \
       }                                                                                       \
       if (s_ >= 0 && kq == 0) mypairs++;                                                      \
+      any_ = any_ || s_ >= 0;                                                                 \
       const float4* ms_ = reinterpret_cast<const float4*>(M + (size_t)(s_ >= 0 ? s_ : 0) * ML::VS + kq * ML::KSP); \
       _Pragma("unroll") for (int v4 = 0; v4 < ML::KSP / 4; v4++) {                            \
         const float4 m4 = s_ >= 0 ? ms_[v4] : make_float4(0.f, 0.f, 0.f, 0.f);                \
@@ -636,9 +637,14 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
       if (e < NV) areg[v] = src_[e];                                                          \
     }                                                                                         \
   }
+  // a wave skips the MFMAs of an offset for which none of its 16 targets has
+  // a source box (boundary tiles); the vote keeps the branch wave-uniform
+  bool cur_any = false;
   if (o0 < o1) {
     if (PA) FMMX_LOAD_A(o0);
+    bool any_ = false;
     FMMX_LOAD_B(o0, bcur);
+    cur_any = __ballot(any_) != 0;
   }
   for (int o = o0; o < o1; o++) {
     __syncthreads();  // every wave is done with the previous operator
@@ -654,11 +660,14 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
     }
     __syncthreads();
     T bnext[NT][KS];
+    bool next_any = false;
     if (o + 1 < o1) {
       if (PA) FMMX_LOAD_A(o + 1);
+      bool any_ = false;
       FMMX_LOAD_B(o + 1, bnext);
+      next_any = __ballot(any_) != 0;
     }
-    if (wave_on) {
+    if (wave_on && cur_any) {
 #pragma unroll
       for (int ks = 0; ks < KS; ks++)
 #pragma unroll
@@ -673,6 +682,7 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
       for (int n = 0; n < NT; n++)
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) bcur[n][ks] = bnext[n][ks];
+      cur_any = next_any;
     }
   }
 #undef FMMX_LOAD_A
