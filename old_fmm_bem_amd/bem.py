@@ -21,7 +21,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import FMM, GNJ
+from . import FMM, G, GNI, GNJ
 
 # ---- quadrature on the reference triangle (biobem/bem3d.cxx:9-245) ---------
 # Published symmetric rules (Strang-Fix / Dunavant): points (xi, eta) and
@@ -139,6 +139,18 @@ class Mesh:
     @property
     def nwe(self):
         return len(self.tri)
+
+    def frames(self):
+        """Nodal tangent and binormal (vta, vsa) of biobem/bem3d.cxx:528-541:
+        vsa = (0, n_z, -n_y)/|.| (or (0, 1, 0) when n is within 1e-3 of the
+        x axis), vta = vsa x n."""
+        n = self.normals
+        s = np.sqrt(n[:, 1] ** 2 + n[:, 2] ** 2)
+        ok = s > 1e-3
+        vsa = np.zeros_like(n)
+        vsa[:, 1] = np.where(ok, n[:, 2] / np.where(ok, s, 1), 1.0)
+        vsa[:, 2] = np.where(ok, -n[:, 1] / np.where(ok, s, 1), 0.0)
+        return np.cross(vsa, n), vsa
 
     def interpolate(self, u):
         """Nodal values -> values at the quadrature points (ne*ngl,)."""
@@ -285,3 +297,98 @@ def solve_refined(op_low, op_high, b, rtol=1e-10, inner_rtol=1e-5, restart=50, m
                         maxiter=maxiter)
         x = x + d
     return x, hist
+
+
+# ---- the constant-element application (bio/main.cxx) -------------------------
+# Energy unit: kcal/mol for charges in e and lengths in Angstrom, the constant
+# the reference multiplies by (bio/main.cxx:502, biobem/main.cxx:198).
+KCAL = 332.112
+
+
+class InducedCharge:
+    """The operator of bio/main.cxx:27-53 on flat panels (centroids c, unit
+    normals nrm as bio/main.cxx:257-281 forms them, areas): for panel values x,
+        y_l = sum_m Gni(c_l, c_m) x_m area_m / e1 + (e1 + e2)/(2 e1 (e1 - e2)) x_l,
+    one FMM matvec with the target-normal kernel (neq 1) per application, and
+    the shell preconditioner of bio/main.cxx:55-71."""
+
+    def __init__(self, centroids, normals, area, e1=4.0, e2=20.0, p=10, precision="mixed", convention="exact",
+                 lmax=0):
+        self.c = np.ascontiguousarray(centroids, np.float32)
+        self.nrm = np.ascontiguousarray(normals, np.float32)
+        self.area = np.asarray(area, np.float64)
+        self.e1, self.e2 = float(e1), float(e2)
+        self.f = FMM(p=p, lmax=lmax, precision=precision, convention=convention)
+        self.f.set_points(self.c, ni_normals=self.nrm)
+        self.diag = (self.e1 + self.e2) / (2 * self.e1 * (self.e1 - self.e2))
+
+    def matvec(self, x):
+        x = np.asarray(x, np.float64)
+        v, _ = self.f.evaluate((x * self.area / self.e1).astype(np.float32), neq=GNI)
+        return v + self.diag * x
+
+    def precondition(self, x):
+        return np.asarray(x) / (self.e1 + self.e2) * (2 * self.e1 * (self.e1 - self.e2))
+
+    def close(self):
+        self.f.close()
+
+
+def solvation_energy(vert, face, xq, q, e1=4.0, e2=20.0, p=10, precision="mixed", rtol=1e-5, restart=30,
+                     maxiter=1000):
+    """Electrostatic solvation energy of point charges inside a triangulated
+    surface, the pipeline of bio/main.cxx:256-503 with the FMM of this
+    package in place of fmm():
+      1. panels: centroids, areas, normals (bio/main.cxx:257-281);
+      2. right-hand side un = -4 pi Gni(charges -> centroids) / e1
+         (bio/main.cxx:282-345);
+      3. GMRES(restart 30, rtol 1e-5: the PETSc KSP defaults) on
+         InducedCharge with its shell preconditioner, zero initial guess
+         (bio/main.cxx:346-437);
+      4. reaction potential at the charges from the panel charges
+         si area / e1 (bio/main.cxx:445-501), energy per charge
+         0.5 * 332.112 * q * potential (bio/main.cxx:502).
+    Returns a dict with the total energy, per-charge energies, the solution
+    si, the reaction field gradient at the charges, the panel geometry and the
+    GMRES history."""
+    from .formats import panels
+
+    c, area, nrm = panels(vert, face)
+    xq = np.ascontiguousarray(xq, np.float32)
+    q = np.asarray(q, np.float32)
+    f = FMM(p=p, precision=precision, convention="exact")
+    f.set_points(c, xq, ni_normals=nrm)
+    vi, _ = f.evaluate(q, neq=GNI)
+    un = -4 * np.pi * vi / e1
+    op = InducedCharge(c, nrm, area, e1, e2, p=p, precision=precision)
+    si, hist, it = gmres(op.matvec, un, precond=op.precondition, rtol=rtol, restart=restart, maxiter=maxiter)
+    op.close()
+    f.set_points(xq, c)
+    vr, gr = f.evaluate((si * area / e1).astype(np.float32), neq=G)
+    f.close()
+    per = 0.5 * KCAL * q.astype(np.float64) * vr
+    return {"energy": float(per.sum()), "per_charge": per, "si": si, "grad": gr, "centroids": c, "area": area,
+            "normals": nrm, "history": hist, "iterations": it}
+
+
+def solvation_energy_6node(mesh: Mesh, xq, q, e1=4.0, e2=20.0, p=10, precision="mixed", rtol=1e-5, restart=30,
+                           maxiter=1000):
+    """The 6-node pipeline of biobem/main.cxx:96-200: right-hand side
+    un = -4 pi Gni(charges -> nodes, normals -vna) / e1, GMRES on Collocation,
+    reaction potential at the charges from the quadrature points with
+    strengths 0.5 hsg (sum_k phi_k si_k) / e1, energy 0.5 * 332.112 * q * v."""
+    xq = np.ascontiguousarray(xq, np.float32)
+    q = np.asarray(q, np.float32)
+    nodes = np.ascontiguousarray(mesh.nodes, np.float32)
+    f = FMM(p=p, precision=precision, convention="exact")
+    f.set_points(nodes, xq, ni_normals=np.ascontiguousarray(-mesh.normals, np.float32))
+    vi, _ = f.evaluate(q, neq=GNI)
+    un = -4 * np.pi * vi / e1
+    op = Collocation(mesh, e1=e1, e2=e2, p=p, precision=precision)
+    si, hist, it = gmres(op.matvec, un, precond=op.precondition, rtol=rtol, restart=restart, maxiter=maxiter)
+    op.close()
+    f.set_points(xq, np.ascontiguousarray(mesh.qx, np.float32))
+    vr, _ = f.evaluate((mesh.qw * mesh.interpolate(si) / e1).astype(np.float32), neq=G, gradient=False)
+    f.close()
+    per = 0.5 * KCAL * q.astype(np.float64) * vr
+    return {"energy": float(per.sum()), "per_charge": per, "si": si, "history": hist, "iterations": it}

@@ -80,6 +80,7 @@ int nlevel(int nmax) {
 struct OpSet {
   std::vector<double> m2m, l2l;  // transposed [op][k][row]
   std::vector<double> m2l_frag;  // MFMA fragment order (FP32 engine)
+  std::vector<double> m2l_canon; // fragment order, the 56 offsets with dx, dy, dz >= 0 (EXACT)
   std::vector<double> m2l_kmaj;  // transposed [op][k][row] (FP64 engine)
   int nlev = 1;
 };
@@ -164,6 +165,16 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
       }
     });
   for (auto& x : th) x.join();
+  if (!per_level) {
+    // EXACT: every offset is a reflection of one with non-negative components
+    S->m2l_canon.resize((size_t)NCANON * fr);
+    for (int ci = 0; ci < NCANON; ci++) {
+      int d[3];
+      canon_vector(ci, d);
+      const int oi = offset_index(d[0], d[1], d[2]);
+      std::copy(&S->m2l_frag[(size_t)oi * fr], &S->m2l_frag[(size_t)oi * fr] + fr, &S->m2l_canon[(size_t)ci * fr]);
+    }
+  }
   std::lock_guard<std::mutex> lk(g_cache_mu);
   g_cache[key] = S;
   return S;
@@ -286,6 +297,19 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   }
   if ((rc = offtab.ensure(ot.size() * 4))) return rc;
   FMMX_HIP(hipMemcpy(offtab.p, ot.data(), ot.size() * 4, hipMemcpyHostToDevice));
+  // the same offsets against the reflection-canonical operators: w = canonical
+  // index | reflection bits (x, y, z) << 16
+  canon = !fp64 && c->convention == FMMX_CONV_EXACT;
+  if (const char* e = getenv("FMMX_M2L_CANON")) canon = canon && atoi(e) != 0;
+  for (size_t e = 0; e < ot.size(); e += 4) {
+    const int dx = ot[e], dy = ot[e + 1], dz = ot[e + 2];
+    const int r = (dx < 0 ? 1 : 0) | (dy < 0 ? 2 : 0) | (dz < 0 ? 4 : 0);
+    const int ci = canon_index(std::abs(dx), std::abs(dy), std::abs(dz));
+    if (ci < 0) return fail(FMMX_ESTATE, "M2L offset outside the canonical table");
+    ot[e + 3] = ci | (r << 16);
+  }
+  if ((rc = offtab_c.ensure(ot.size() * 4))) return rc;
+  FMMX_HIP(hipMemcpy(offtab_c.p, ot.data(), ot.size() * 4, hipMemcpyHostToDevice));
   return FMMX_OK;
 }
 
@@ -340,7 +364,7 @@ int Engine::upload_operators() {
       return rc;
   } else {
     if ((rc = upload<float>(op_m2m, S->m2m, stream)) || (rc = upload<float>(op_l2l, S->l2l, stream)) ||
-        (rc = upload<float>(op_m2l, S->m2l_frag, stream)))
+        (rc = upload<float>(op_m2l, canon ? S->m2l_canon : S->m2l_frag, stream)))
       return rc;
   }
   ops_lmax = lmax;

@@ -104,7 +104,9 @@ struct Engine {
   DevBuf m2l_part2;               // M2L partial sums of the levels above the leaf (third stream)
   DevBuf* part_buf = &m2l_part;   // partial-sum buffer of the M2L launches being issued
   bool m2l_per_level = false;
-  DevBuf offtab;  // int4 (dx,dy,dz,opidx) per (class, 189)
+  DevBuf offtab;    // int4 (dx,dy,dz,opidx) per (class, 189)
+  DevBuf offtab_c;  // the same with w = canonical operator | reflection bits << 16
+  bool canon = false;  // FP32 EXACT: M2L from the 56 reflection-canonical operators
   DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)
   int ops_lmax = -1;
   float ops_rd = -1;

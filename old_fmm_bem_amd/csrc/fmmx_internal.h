@@ -55,6 +55,11 @@ void op_m2l(const Tables& T, int dx, int dy, int dz, double rb, double* A);
 // Offset index of (dx,dy,dz) in [-3,3]^3 with max|d| >= 2, 0..NOFF-1.
 int offset_index(int dx, int dy, int dz);
 void offset_vector(int idx, int* d);
+// The NCANON offsets with 0 <= dx, dy, dz <= 3 and max >= 2 (every M2L offset
+// is one of them reflected through coordinate planes).
+constexpr int NCANON = 56;
+int canon_index(int ax, int ay, int az);
+void canon_vector(int idx, int* d);
 
 std::string& last_error();
 int fail(int code, const std::string& msg);

@@ -548,13 +548,39 @@ constexpr int M2L_TN = 64;  // targets per workgroup of the FP64 vector kernel
 // the matrix cores; the next offset's source multipoles always are
 // (software pipelining).  The operator goes through one LDS buffer shared by
 // the 4 waves.
+// Reflection of an M2L offset through the coordinate planes: with reflection
+// bits r = (x, y, z), M2L(reflected offset) = T_r M2L(offset) T_r for the
+// diagonal +-1 matrix T_r on the real slots: x flips Re by (-1)^m and Im by
+// (-1)^(m+1); y flips Im; z flips both by (-1)^(n+m).  Bit c of mask[r] is set
+// where T_r has -1.
+template <int P>
+struct ReflMask {
+  static constexpr int W = (P * P + 31) / 32;
+  uint32_t mask[8][W];
+  constexpr ReflMask() : mask{} {
+    for (int r = 0; r < 8; r++) {
+      int c = 0;
+      for (int im = 0; im < 2; im++)
+        for (int n = 0; n < P; n++)
+          for (int m = im; m <= n; m++, c++) {
+            int neg = 0;
+            if (r & 1) neg ^= (m & 1) ^ im;
+            if (r & 2) neg ^= im;
+            if (r & 4) neg ^= (n + m) & 1;
+            if (neg) mask[r][c >> 5] |= 1u << (c & 31);
+          }
+    }
+  }
+  __host__ __device__ constexpr bool neg(int r, int c) const { return (mask[r][c >> 5] >> (c & 31)) & 1u; }
+};
+
 template <int P, int NT, int PA, int WPG>
 __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
                                              const uint32_t* __restrict__ ord, const uint32_t* __restrict__ ikey,
                                              const int* __restrict__ dense_j, const float* __restrict__ M,
                                              float* __restrict__ L, float* __restrict__ part,
                                              const float* __restrict__ opF, const int4* __restrict__ offtab,
-                                             int init, unsigned long long* __restrict__ npairs) {
+                                             int init, int region, unsigned long long* __restrict__ npairs) {
   using T = float;
   using ML = MLayout<P>;
   constexpr int P2 = P * P, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, FR = KS * MT * 64;
@@ -571,8 +597,23 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
   // parity class's 189 operators.  Work item k = b/8 of that class, split
   // major: the workgroups resident on an XCD at a time share one split's
   // 189/nsplit operators, which then stay in its L2 across tiles.
-  const int cls0 = blockIdx.x & 7, item = blockIdx.x >> 3, tpc = (int)(gridDim.x >> 3) / nsplit;
-  const int split = item / tpc, tile = item % tpc;
+  // With region set, workgroup b runs on XCD b%8 for a block of tiles
+  // instead: XCD x owns tiles [x*tpr, (x+1)*tpr) of every class, so the 8
+  // siblings of a box (one per class) run on one XCD and share their source
+  // multipoles in its L2; items on an XCD go tile-major, then split, then
+  // class.
+  int cls0, split, tile;
+  if (region) {
+    const int xcd = blockIdx.x & 7, item = blockIdx.x >> 3, tpr = (int)(gridDim.x >> 6) / nsplit;
+    cls0 = item & 7;
+    split = (item >> 3) % nsplit;
+    tile = xcd * tpr + (item >> 3) / nsplit;
+  } else {
+    const int item = blockIdx.x >> 3, tpc = (int)(gridDim.x >> 3) / nsplit;
+    cls0 = blockIdx.x & 7;
+    split = item / tpc;
+    tile = item % tpc;
+  }
   if (tid == 0) {
     int start = 0;
     for (int c = 0; c < cls0; c++) start += cls_cnt[c];
@@ -606,6 +647,30 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
     for (int mt = 0; mt < MT; mt++) acc[n][mt] = V4{0.f, 0.f, 0.f, 0.f};
   unsigned long long mypairs = 0;
   float4 areg[PA ? NVT : 1];
+  // Reflected offsets use the operator of the canonical offset with element
+  // (row i, col k) times T_r(i) T_r(k).  The float4s a thread stages sit at
+  // fixed places, so their sign patterns are worked out once: nibble r of
+  // sgn[v] flags the 4 floats of the thread's v-th float4 under reflection r.
+  // Fragment element q = (ks * MT + mt) * 64 + lane holds row 16 mt + lane % 16
+  // and col (lane / 16) * KS + ks.
+  uint32_t sgn[PA ? 1 : NVT];
+  if (!PA) {
+    constexpr ReflMask<P> RM{};
+#pragma unroll
+    for (int v = 0; v < NVT; v++) {
+      sgn[v] = 0;
+      const int e = tid + NTH * v;
+      if (e < NV) {
+        const int q = 4 * e, lane = q & 63, mt = (q >> 6) % MT, ks = (q >> 6) / MT;
+        const int col = (lane >> 4) * KS + ks, row0 = 16 * mt + (lane & 15);
+        for (int r = 1; r < 8; r++) {
+          const bool cn = col < P2 && RM.neg(r, col);
+          for (int jj = 0; jj < 4; jj++)
+            if (row0 + jj < P2 && RM.neg(r, row0 + jj) != cn) sgn[v] |= 1u << (4 * r + jj);
+        }
+      }
+    }
+  }
   T bcur[NT][KS];
 #define FMMX_LOAD_B(o_, dst_)                                                                 \
   {                                                                                           \
@@ -631,7 +696,7 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
   }
 #define FMMX_LOAD_A(o_)                                                                       \
   {                                                                                           \
-    const float4* src_ = reinterpret_cast<const float4*>(opF + (size_t)offtab[cls * NEBM + (o_)].w * FR); \
+    const float4* src_ = reinterpret_cast<const float4*>(opF + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FR); \
     _Pragma("unroll") for (int v = 0; v < NVT; v++) {                                         \
       const int e = tid + NTH * v;                                                            \
       if (e < NV) areg[v] = src_[e];                                                          \
@@ -655,8 +720,21 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
         if (e < NV) reinterpret_cast<float4*>(As)[e] = areg[v];
       }
     } else {
-      const float4* src = reinterpret_cast<const float4*>(opF + (size_t)offtab[cls * NEBM + o].w * FR);
-      for (int e = tid; e < NV; e += NTH) reinterpret_cast<float4*>(As)[e] = src[e];
+      const int w = offtab[cls * NEBM + o].w, refl = w >> 16;
+      const float4* src = reinterpret_cast<const float4*>(opF + (size_t)(w & 0xffff) * FR);
+#pragma unroll
+      for (int v = 0; v < NVT; v++) {
+        const int e = tid + NTH * v;
+        if (e < NV) {
+          float4 x = src[e];
+          const uint32_t nib = sgn[v] >> (4 * refl);
+          x.x = __uint_as_float(__float_as_uint(x.x) ^ ((nib & 1u) << 31));
+          x.y = __uint_as_float(__float_as_uint(x.y) ^ (((nib >> 1) & 1u) << 31));
+          x.z = __uint_as_float(__float_as_uint(x.z) ^ (((nib >> 2) & 1u) << 31));
+          x.w = __uint_as_float(__float_as_uint(x.w) ^ (((nib >> 3) & 1u) << 31));
+          reinterpret_cast<float4*>(As)[e] = x;
+        }
+      }
     }
     __syncthreads();
     T bnext[NT][KS];
@@ -668,14 +746,25 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
       next_any = __ballot(any_) != 0;
     }
     if (wave_on && cur_any) {
+      // operator fragments of step ks+1 are read from LDS while the MFMAs of
+      // step ks run; the scheduling barriers keep that order, so the LDS
+      // latency sits behind MT matrix instructions instead of in front of one
+      T af[2][MT];
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++)
+      for (int mt = 0; mt < MT; mt++) af[0][mt] = As[mt * 64 + l];
 #pragma unroll
-        for (int mt = 0; mt < MT; mt++) {
-          const T a = As[(ks * MT + mt) * 64 + l];
+      for (int ks = 0; ks < KS; ks++) {
+        if (ks + 1 < KS) {
 #pragma unroll
-          for (int n = 0; n < NT; n++) acc[n][mt] = Mfma<T>::fma(a, bcur[n][ks], acc[n][mt]);
+          for (int mt = 0; mt < MT; mt++) af[(ks + 1) & 1][mt] = As[((ks + 1) * MT + mt) * 64 + l];
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+          for (int n = 0; n < NT; n++) acc[n][mt] = Mfma<T>::fma(af[ks & 1][mt], bcur[n][ks], acc[n][mt]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     if (o + 1 < o1) {
 #pragma unroll
@@ -1282,17 +1371,25 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       int nt = 1, pa = 0, wpg = 8;
       if (const char* e = getenv("FMMX_M2L_NT")) nt = atoi(e) == 2 ? 2 : 1;
       if (const char* e = getenv("FMMX_M2L_PA")) pa = atoi(e) ? 1 : 0;
+      // reflection-canonical operators (Laplace, EXACT) are signed while staged
+      const bool use_canon = canon && !yk;
+      if (use_canon) pa = 0;
       if (const char* e = getenv("FMMX_M2L_WPG")) wpg = atoi(e) == 4 ? 4 : 8;
       // per class at most ceil(max class count / TN) tiles; grid = 8 classes x tiles x splits
       int cmax = 0;
       for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
       const int tpc = std::max(1, (cmax + 16 * wpg * nt - 1) / (16 * wpg * nt));
       const int ns2 = nsplit;
+      // a block of tiles per XCD once every XCD gets at least one tile
+      int region = tpc >= 8 ? 1 : 0;
+      if (const char* e = getenv("FMMX_M2L_REGION")) region = region && atoi(e) != 0;
+      const int nwg = region ? 64 * ((tpc + 7) / 8) * ns2 : 8 * tpc * ns2;
 #define FMMX_M2L_LAUNCH(NT_, PA_, W_)                                                                    \
-  k_m2l<P, NT_, PA_, W_><<<8 * tpc * ns2, 64 * W_, 0, stream>>>(                                        \
+  k_m2l<P, NT_, PA_, W_><<<nwg, 64 * W_, 0, stream>>>(                                                  \
       lev, nb, ns2, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),   \
       sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), part_buf->as<float>(),     \
-      opm2l.as<float>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev)
+      opm2l.as<float>() + opoff, (use_canon ? offtab_c : offtab).as<int4>(), init ? 1 : 0, region,      \
+      pair_ctr.as<unsigned long long>() + 2 + lev)
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
       if (wpg == 8) {

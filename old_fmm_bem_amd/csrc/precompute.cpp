@@ -162,6 +162,33 @@ void build_dense(const Tables& T, const double* in_scale, const double* out_scal
 
 }  // namespace
 
+int canon_index(int ax, int ay, int az) {
+  if (ax < 0 || ay < 0 || az < 0 || ax > 3 || ay > 3 || az > 3 || (ax <= 1 && ay <= 1 && az <= 1)) return -1;
+  int idx = 0;
+  for (int a = 0; a <= 3; a++)
+    for (int b = 0; b <= 3; b++)
+      for (int c = 0; c <= 3; c++) {
+        if (a <= 1 && b <= 1 && c <= 1) continue;
+        if (a == ax && b == ay && c == az) return idx;
+        idx++;
+      }
+  return -1;
+}
+
+void canon_vector(int idx, int* d) {
+  for (int a = 0; a <= 3; a++)
+    for (int b = 0; b <= 3; b++)
+      for (int c = 0; c <= 3; c++) {
+        if (a <= 1 && b <= 1 && c <= 1) continue;
+        if (idx-- == 0) {
+          d[0] = a;
+          d[1] = b;
+          d[2] = c;
+          return;
+        }
+      }
+}
+
 int offset_index(int dx, int dy, int dz) {
   int idx = 0;
   for (int x = -3; x <= 3; x++)

@@ -104,6 +104,43 @@ def test_m2l_operator_matches_reference_translation(conv, p):
             assert np.abs(L - Lref).max() <= 1e-13 * np.abs(Lref).max(), (d, rb)
 
 
+def _reflection_signs(p, r):
+    """Diagonal of T_r on the real slots (Re for m >= 0, then Im for m >= 1):
+    reflecting x maps Y_n^m to (-1)^m conj(Y_n^m), y to conj(Y_n^m), z to
+    (-1)^(n+m) Y_n^m."""
+    s = []
+    for im in (0, 1):
+        for n in range(p):
+            for m in range(im, n + 1):
+                v = 1
+                if r & 1:
+                    v *= (-1) ** m * (-1 if im else 1)
+                if r & 2:
+                    v *= -1 if im else 1
+                if r & 4:
+                    v *= (-1) ** (n + m)
+                s.append(v)
+    return np.array(s, float)
+
+
+@pytest.mark.parametrize("p", [4, 10])
+def test_m2l_operator_reflection_symmetry(p):
+    """The FP32 engine stores only the 56 M2L operators with dx, dy, dz >= 0 and
+    signs them while staging: M2L(reflected offset) = T_r M2L(offset) T_r for
+    every one of the 8 reflections of every canonical offset."""
+    for a in range(4):
+        for b in range(4):
+            for c in range(4):
+                if max(a, b, c) < 2:
+                    continue
+                A = F.operator(p, O.EXACT, 2, a, b, c)
+                for r in range(1, 8):
+                    d = (-a if r & 1 else a, -b if r & 2 else b, -c if r & 4 else c)
+                    t = _reflection_signs(p, r)
+                    B = F.operator(p, O.EXACT, 2, *d)
+                    assert np.abs(B - t[:, None] * A * t[None, :]).max() <= 1e-14 * np.abs(A).max(), (d, r)
+
+
 def test_m2m_operator_is_exact_shift():
     """P2M about the parent centre == M2M(P2M about the child centre): the
     truncated multipole shift is exact (EXACT convention)."""

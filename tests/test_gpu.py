@@ -292,6 +292,30 @@ def test_overlapped_and_serialized_phases_agree():
     f.close()
 
 
+@pytest.mark.parametrize("p", [4, 10])
+def test_canonical_m2l_operators_match_full_set(p, monkeypatch):
+    """The mixed EXACT engine keeps the 56 M2L operators with non-negative
+    offsets and signs them per reflection while staging; with
+    FMMX_M2L_CANON=0 it uploads all 316.  Both agree to float rounding of the
+    operators, for every kernel and at every level."""
+    x, q = uniform_cube(60000, 84)
+    nrm = np.random.default_rng(4).standard_normal((60000, 3)).astype(np.float32)
+    out = []
+    for canon in ("0", "1"):
+        monkeypatch.setenv("FMMX_M2L_CANON", canon)
+        f = F.FMM(p=p, lmax=4, precision="mixed", convention="exact", box=(0, 0, 0, 1))
+        f.set_points(x, x, ni_normals=nrm, nj_normals=nrm)
+        print(f"p={p} canon={canon}", flush=True)
+        out.append([f.evaluate(q, neq=k)[0] for k in (0, 1, 2)] + [f.evaluate(q)[1]])
+        f.close()
+    for k in range(4):
+        assert rel_l2(out[1][k], out[0][k]) < 1e-6, k
+    if p == 10:
+        idx = np.arange(0, 60000, 200)
+        dv, dg = O.direct(x[idx], x, q, neq=0, conv=O.EXACT)
+        assert rel_l2(out[1][0][idx], dv) < 2e-5
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_shared_tree_equals_separate_trees(precision):
     """set_points(x) shares one tree between targets and sources; a copy of
