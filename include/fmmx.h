@@ -188,7 +188,15 @@ int fmmx_get_locals(fmmx_handle* h, int lev, double* ax_reim);
  * M~_n = M_n / ((2n+1) i_n(lambda rb)), L~_n = L_n i_n(lambda rb) with
  * multipole basis k_n(lambda r) Y_n^m and local basis i_n(lambda r) Y_n^m:
  * kind 3: M2M from child octant a0; 4: L2L to child octant a0; 5: M2L for
- * offset (a0,a1,a2).  The convention does not enter kinds 3-5. */
+ * offset (a0,a1,a2).  The convention does not enter kinds 3-5.
+ * Helmholtz (FMMX_HELMHOLTZ), where rb is k times the box size (of the child
+ * for 6 and 7): (2 p^2) x (2 p^2) operators on split expansions, the real
+ * slots of the Re part then of the Im part (multipole basis
+ * i h_n(kr) (kr)^(n+1) / (2n-1)!! (rb/r)^(n+1) Y_n^m, local basis
+ * (2n+1)!! j_n(kr) / (kr)^n (r/rb)^n Y_n^m, both times 1/(4 pi rb)): kind 6:
+ * M2M from child octant a0; 7: L2L to child octant a0; 8: M2L for offset
+ * (a0,a1,a2).  A needs (2 p^2)^2 doubles.  No reference counterpart: the
+ * reference has no oscillatory kernel. */
 int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, double rb, double* A);
 
 /* Last error message of the calling thread. */

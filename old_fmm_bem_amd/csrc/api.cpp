@@ -9,6 +9,7 @@
 #include <tuple>
 
 #include "engine.h"
+#include "helmholtz.h"
 #include "yukawa.h"
 #include "tree_util.h"
 
@@ -671,6 +672,20 @@ int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, doubl
     if (kind == 5) {
       if (fmmx::offset_index(a0, a1, a2) < 0 || !(rb > 0)) return fmmx::fail(FMMX_EINVAL, "bad M2L offset or lambda*rb");
       fmmx::yuk_op_m2l(p, rb, a0, a1, a2, A);
+      return FMMX_OK;
+    }
+    if (kind == 6 || kind == 7) {
+      if (a0 < 0 || a0 > 7 || !(rb > 0) || rb > 0.5 * fmmx::HH_KRB_MAX)
+        return fmmx::fail(FMMX_EINVAL, "octant out of range or k*rb outside (0, HH_KRB_MAX/2]");
+      std::vector<double> other((size_t)4 * p * p * p * p);
+      if (kind == 6) fmmx::hh_op_m2m_l2l(p, rb, a0, A, other.data());
+      else fmmx::hh_op_m2m_l2l(p, rb, a0, other.data(), A);
+      return FMMX_OK;
+    }
+    if (kind == 8) {
+      if (fmmx::offset_index(a0, a1, a2) < 0 || !(rb > 0) || rb > fmmx::HH_KRB_MAX)
+        return fmmx::fail(FMMX_EINVAL, "bad M2L offset or k*rb outside (0, HH_KRB_MAX]");
+      fmmx::hh_op_m2l(p, rb, a0, a1, a2, A);
       return FMMX_OK;
     }
     return fmmx::fail(FMMX_EINVAL, "bad kind");
