@@ -128,8 +128,11 @@ class _Arr:
 
 def operator(p, convention, kind, a0, a1=0, a2=0, rb=1.0):
     """Dense real operator the engine applies (host only, no GPU): kind 0 M2M
-    (octant a0), 1 L2L (octant a0), 2 M2L (offset a0,a1,a2 at box size rb)."""
-    A = np.zeros((p * p, p * p))
+    (octant a0), 1 L2L (octant a0), 2 M2L (offset a0,a1,a2 at box size rb);
+    3-5 the same for YUKAWA (rb = lambda * box size), 6-8 for HELMHOLTZ
+    (rb = k * box size, (2 p^2) x (2 p^2) on split expansions)."""
+    d = 2 * p * p if kind in (6, 7, 8) else p * p
+    A = np.zeros((d, d))
     _check(lib().fmmx_operator(p, _CONV[convention], kind, a0, a1, a2, float(rb), A.ctypes.data))
     return A
 

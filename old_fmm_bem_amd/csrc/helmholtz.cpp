@@ -17,9 +17,8 @@
 //
 // Complex charges q = qr + i qi give multipoles M = R + i I where R and I
 // are the multipoles of the real charge sets qr and qi: each satisfies
-// R_n^-m = (-1)^m conj(R_n^m) and is stored in the Laplace engine's real
-// slots.  Locals are split the same way into the expansions of Re(phi) and
-// Im(phi).  Every operator below maps such a split pair (2 p^2 reals) to a
+// R_n^-m = conj(R_n^m) and is stored in the Laplace engine's real slots.
+// Locals are split the same way into the expansions of Re(phi) and Im(phi).  Every operator below maps such a split pair (2 p^2 reals) to a
 // split pair; it is the exact re-expansion of each basis function, obtained
 // by projection onto the harmonics on a sphere around the new centre, as for
 // the screened kernel (yukawa.cpp).
@@ -88,9 +87,9 @@ void split_real(int p, const std::vector<cplx>& C, double* A) {
       for (int m = 0; m <= n; m++)
         for (int im = 0; im < (m > 0 ? 2 : 1); im++) {
           const int col = t * P2 + (im ? slot_im(p, n, m) : slot_re(n, m));
-          // unit real slot -> X_n^m = a, X_n^-m = (-1)^m conj(a); part t = 1
-          // enters the complex multipole times i
-          cplx a = im ? I : cplx(1), am = ((m & 1) ? -1.0 : 1.0) * std::conj(a);
+          // unit real slot -> X_n^m = a, X_n^-m = conj(a) (Y_n^-m = conj(Y_n^m)
+          // in this normalisation); part t = 1 enters the multipole times i
+          cplx a = im ? I : cplx(1), am = std::conj(a);
           if (t == 1) {
             a *= I;
             am *= I;
@@ -104,8 +103,8 @@ void split_real(int p, const std::vector<cplx>& C, double* A) {
                 lp += rp[ci(n, -m)] * am;
                 lm += rm[ci(n, -m)] * am;
               }
-              const double sg = (k & 1) ? -1.0 : 1.0;
-              const cplx ca = 0.5 * (lp + sg * std::conj(lm)), cb = (lp - sg * std::conj(lm)) / (2.0 * I);
+              // Re(phi) and Im(phi) parts: (L_j^k +- conj(L_j^-k)) / (2, 2i)
+              const cplx ca = 0.5 * (lp + std::conj(lm)), cb = (lp - std::conj(lm)) / (2.0 * I);
               A[(size_t)slot_re(j, k) * D + col] = ca.real();
               A[(size_t)(P2 + slot_re(j, k)) * D + col] = cb.real();
               if (k > 0) {
