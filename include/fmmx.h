@@ -40,8 +40,10 @@ enum {
 
 /* kernels: the reference's neq values (fmm/fmm.cxx:47-65); FMMX_YUKAWA is
  * its "Helmholtz" (neq 10): the screened kernel (pi/2) exp(-lambda r) /
- * (lambda r) (kernel/Helmholtz_G_p2p.cxx:22), lambda from fmmx_set_yukawa */
-enum { FMMX_G = 0, FMMX_GNI = 1, FMMX_GNJ = 2, FMMX_YUKAWA = 10 };
+ * (lambda r) (kernel/Helmholtz_G_p2p.cxx:22), lambda from fmmx_set_yukawa.
+ * FMMX_HELMHOLTZ has no reference counterpart: the oscillatory kernel
+ * exp(i k r) / (4 pi r) with complex charges, k from fmmx_set_helmholtz. */
+enum { FMMX_G = 0, FMMX_GNI = 1, FMMX_GNJ = 2, FMMX_YUKAWA = 10, FMMX_HELMHOLTZ = 20 };
 
 /* arithmetic of the evaluation */
 enum {
@@ -104,13 +106,19 @@ int fmmx_set_points(fmmx_handle* h, int ni, const float* xi, const float* yi, co
  * Unlike the reference, outputs are overwritten (not accumulated) and are
  * double.  Factor 1/(4 pi) included as in kernel/G_p2p.cxx:25.
  * FMMX_YUKAWA also returns the gradient (the reference computes the
- * potential only, kernel/Helmholtz_G_p2p.cxx). */
+ * potential only, kernel/Helmholtz_G_p2p.cxx).
+ * FMMX_HELMHOLTZ: qj holds nj complex charges as (re, im) float pairs and vi
+ * receives ni complex potentials as (re, im) double pairs; no gradient (the
+ * g pointers must be NULL).  p must be one of 4, 6, 8, 10, 12 and k times
+ * the level-2 box size at most 5. */
 int fmmx_evaluate(fmmx_handle* h, int neq, const float* qj, double* vi, double* gxi, double* gyi,
                   double* gzi, int flags);
 
 /* Direct sums (kernel/G_dir.cxx:6, Gni_dir.cxx:6, Gnj_dir.cxx:6) on the GPU
  * for ni targets against nj sources, FP64 accumulation, in the handle's
- * convention.  Stateless apart from the handle's device and stream. */
+ * convention.  Stateless apart from the handle's device and stream (and its
+ * lambda or k).  FMMX_HELMHOLTZ takes and returns complex pairs as
+ * fmmx_evaluate does. */
 int fmmx_direct(fmmx_handle* h, int neq, int ni, const float* xi, const float* yi, const float* zi,
                 const float* nxi, const float* nyi, const float* nzi, int nj, const float* xj,
                 const float* yj, const float* zj, const float* nxj, const float* nyj,
@@ -127,6 +135,11 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
  * lambda, include/constants.h:61, test/testh.cxx:42).  Its translation
  * operators are built at the first Yukawa evaluate after a change. */
 int fmmx_set_yukawa(fmmx_handle* h, double lambda);
+
+/* Wavenumber k > 0 of FMMX_HELMHOLTZ.  No reference counterpart (the
+ * reference has no oscillatory kernel).  Its translation operators are built
+ * at the first Helmholtz evaluate after a change. */
+int fmmx_set_helmholtz(fmmx_handle* h, double k);
 
 /* Morton keys of n points at the leaf level and their stable sort
  * (fmm/boxn.cxx:5-31 then sort/csort.cxx:5-17, as sort/sorti.cxx:12-26

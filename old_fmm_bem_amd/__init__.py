@@ -19,9 +19,10 @@ import os
 
 import numpy as np
 
-__all__ = ["FMM", "FMMError", "lib", "operator", "G", "GNI", "GNJ", "MIXED", "FP64", "REF", "EXACT"]
+__all__ = ["FMM", "FMMError", "lib", "operator", "G", "GNI", "GNJ", "YUKAWA", "HELMHOLTZ", "MIXED", "FP64", "REF",
+           "EXACT"]
 
-G, GNI, GNJ, YUKAWA = 0, 1, 2, 10
+G, GNI, GNJ, YUKAWA, HELMHOLTZ = 0, 1, 2, 10, 20
 MIXED, FP64 = 0, 1
 REF, EXACT = 0, 1
 NEBM = 189
@@ -60,6 +61,7 @@ _SYMBOLS = {
     "fmmx_evaluate": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
     "fmmx_direct": (C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_int] + [C.c_void_p] * 11
                     + [C.c_int]),
+    "fmmx_set_helmholtz": (C.c_int, [C.c_void_p, C.c_double]),
     "fmmx_set_yukawa": (C.c_int, [C.c_void_p, C.c_double]),
     "fmmx_set_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
     "fmmx_sort": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -139,7 +141,7 @@ def operator(p, convention, kind, a0, a1=0, a2=0, rb=1.0):
 
 class FMM:
     def __init__(self, p=10, lmax=0, precision="mixed", convention="exact", box=None, device=0, stream=None,
-                 lam=None):
+                 lam=None, k=None):
         cfg = _Config(p, lmax, _PREC[precision], _CONV[convention], 0.0, 0.0, 0.0, 0.0)
         if box is not None:
             cfg.xmin, cfg.ymin, cfg.zmin, cfg.rd = [float(np.float32(b)) for b in box]
@@ -152,6 +154,13 @@ class FMM:
         self._device = False
         if lam is not None:
             self.set_yukawa(lam)
+        if k is not None:
+            self.set_helmholtz(k)
+
+    def set_helmholtz(self, k):
+        """Wavenumber of the HELMHOLTZ kernel exp(i k r) / (4 pi r)."""
+        _check(lib().fmmx_set_helmholtz(self.h, float(k)))
+        return self
 
     def set_yukawa(self, lam):
         """Screening constant of the YUKAWA kernel (pi/2) exp(-lam r)/(lam r)."""
@@ -211,8 +220,11 @@ class FMM:
 
     # -- matvec ---------------------------------------------------------
     def evaluate(self, q, neq=G, gradient=True):
-        """Potential v (ni,) and, for neq=G or YUKAWA, gradient g (ni,3), caller order."""
+        """Potential v (ni,) and, for neq=G or YUKAWA, gradient g (ni,3), caller order.
+        HELMHOLTZ: complex charges in, complex potentials out (no gradient)."""
         dev = self._device
+        if neq == HELMHOLTZ:
+            return self._evaluate_complex(q), None
         qa = _Arr(q, dev)
         want_g = gradient and neq in (G, YUKAWA)
         if dev:
@@ -230,9 +242,48 @@ class FMM:
             return v, g.T
         return v, None
 
+    @staticmethod
+    def _pairs(q, dev):
+        """complex charges -> contiguous (re, im) float32 pairs"""
+        if dev:
+            import torch
+            return torch.view_as_real(q.to(torch.complex64)).contiguous()
+        q = np.asarray(q)
+        return np.ascontiguousarray(np.stack([q.real, q.imag], -1).astype(np.float32))
+
+    def _evaluate_complex(self, q):
+        dev = self._device
+        qp = self._pairs(q, dev)
+        if dev:
+            import torch
+            v = torch.empty((self.ni, 2), dtype=torch.float64, device=q.device)
+            _check(lib().fmmx_evaluate(self.h, HELMHOLTZ, qp.data_ptr(), v.data_ptr(), None, None, None, 1))
+            return torch.view_as_complex(v)
+        v = np.empty((self.ni, 2))
+        _check(lib().fmmx_evaluate(self.h, HELMHOLTZ, qp.ctypes.data, v.ctypes.data, None, None, None, 0))
+        return v[:, 0] + 1j * v[:, 1]
+
     def direct(self, xi, xj, q, neq=G, ni_normals=None, nj_normals=None):
-        """GPU direct sum (FP64 accumulation) in this handle's convention."""
+        """GPU direct sum (FP64 accumulation) in this handle's convention.
+        HELMHOLTZ: complex charges and potentials."""
         dev = _is_torch(xi) and xi.is_cuda
+        if neq == HELMHOLTZ:
+            a = [_Arr(xi[:, k], dev) for k in range(3)]
+            b = [_Arr(xj[:, k], dev) for k in range(3)]
+            qp = self._pairs(q, dev)
+            n = int(xi.shape[0])
+            if dev:
+                import torch
+                v = torch.zeros((n, 2), dtype=torch.float64, device=xi.device)
+                _check(lib().fmmx_direct(self.h, neq, n, *[c.ptr for c in a], None, None, None, int(xj.shape[0]),
+                                         *[c.ptr for c in b], None, None, None, qp.data_ptr(), v.data_ptr(), None, None,
+                                         None, 1))
+                return torch.view_as_complex(v), None
+            v = np.zeros((n, 2))
+            _check(lib().fmmx_direct(self.h, neq, n, *[c.ptr for c in a], None, None, None, int(xj.shape[0]),
+                                     *[c.ptr for c in b], None, None, None, qp.ctypes.data, v.ctypes.data, None, None,
+                                     None, 0))
+            return v[:, 0] + 1j * v[:, 1], None
         a = [_Arr(xi[:, k], dev) for k in range(3)]
         a += [_Arr(None if ni_normals is None else ni_normals[:, k], dev) for k in range(3)]
         b = [_Arr(xj[:, k], dev) for k in range(3)]

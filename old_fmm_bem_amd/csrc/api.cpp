@@ -32,6 +32,28 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ perm, const double
   if (i < n) d[perm[i]] = s[i];
 }
 
+// complex charges given as (re, im) pairs -> key-ordered real and imaginary parts
+__global__ void k_gather_qc(int n, const uint32_t* __restrict__ perm, const float* __restrict__ q,
+                            float* __restrict__ qr, float* __restrict__ qi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t j = perm ? perm[i] : (uint32_t)i;
+    qr[i] = q[2 * (size_t)j];
+    qi[i] = q[2 * (size_t)j + 1];
+  }
+}
+
+// key-ordered real and imaginary parts -> (re, im) pairs in caller order
+__global__ void k_scatter_c(int n, const uint32_t* __restrict__ perm, const double* __restrict__ re,
+                            const double* __restrict__ im, double* __restrict__ d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t j = perm ? perm[i] : (uint32_t)i;
+    d[2 * (size_t)j] = re[i];
+    d[2 * (size_t)j + 1] = im[i];
+  }
+}
+
 __global__ void k_zero_f64(long n, double* __restrict__ a) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
@@ -237,6 +259,62 @@ std::shared_ptr<OpSet> get_yuk_ops(int p, double lam, float rd, int lmax) {
   return S;
 }
 
+// Helmholtz operator sets keyed by (p, k, rd, lmax), on split expansions:
+// M2M per parent level 2..lmax-1 (padded multipole layout of both parts,
+// (2 VS)^2, k-major), L2L per child level 3..lmax ((2 P2)^2, k-major), M2L per
+// level 2..lmax (316 offsets, (2 P2)^2, k-major)
+std::map<std::tuple<int, double, float, int>, std::shared_ptr<OpSet>> g_hh_cache;
+
+std::shared_ptr<OpSet> get_hh_ops(int p, double k, float rd, int lmax) {
+  const int P2 = p * p, D = 2 * P2;
+  auto key = std::make_tuple(p, k, rd, lmax);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_hh_cache.find(key);
+    if (it != g_hh_cache.end()) return it->second;
+  }
+  auto S = std::make_shared<OpSet>();
+  const size_t dq = (size_t)D * D;
+  const int KS = (P2 + 3) / 4, KSP = (KS + 3) / 4 * 4, VS = 4 * KSP, VH = 2 * VS;
+  const size_t vq = (size_t)VH * VH;
+  auto pos = [&](int c) {  // split slot -> padded position of its part
+    const int t = c / P2, r = c % P2;
+    return t * VS + (r / KS) * KSP + r % KS;
+  };
+  const int nt_lev = std::max(1, lmax - 2);
+  S->m2m.assign((size_t)nt_lev * 8 * vq, 0.0);
+  S->l2l.assign((size_t)nt_lev * 8 * dq, 0.0);
+  S->nlev = lmax - 1;
+  S->m2l_kmaj.resize((size_t)S->nlev * NOFF * dq);
+  const int ntr = lmax > 2 ? (lmax - 2) * 8 : 0, njobs = ntr + S->nlev * NOFF;
+  std::vector<std::thread> th;
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      std::vector<double> A(dq), B(dq);
+      for (int j = (int)t; j < njobs; j += (int)nt) {
+        if (j < ntr) {
+          const int li = j / 8, o = j % 8, lc = 3 + li;
+          hh_op_m2m_l2l(p, k * (double)rd / (double)(1 << lc), o, A.data(), B.data());
+          double* M = &S->m2m[((size_t)li * 8 + o) * vq];
+          for (int r = 0; r < D; r++)
+            for (int c = 0; c < D; c++) M[(size_t)pos(c) * VH + pos(r)] = A[(size_t)r * D + c];
+          transpose_into(B.data(), D, &S->l2l[((size_t)li * 8 + o) * dq]);
+        } else {
+          const int jj = j - ntr, li = jj / NOFF, oi = jj % NOFF;
+          int d[3];
+          offset_vector(oi, d);
+          hh_op_m2l(p, k * (double)rd / (double)(1 << (li + 2)), d[0], d[1], d[2], A.data());
+          transpose_into(A.data(), D, &S->m2l_kmaj[((size_t)li * NOFF + oi) * dq]);
+        }
+      }
+    });
+  for (auto& x : th) x.join();
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_hh_cache[key] = S;
+  return S;
+}
+
 template <typename T>
 int upload(DevBuf& dst, const std::vector<double>& src, hipStream_t s) {
   int rc = dst.ensure(src.size() * sizeof(T));
@@ -332,6 +410,31 @@ int Engine::set_domain(int lm, float x0, float y0, float z0, float r) {
 int Engine::set_yukawa(double lambda) {
   if (!(lambda > 0) || !std::isfinite(lambda)) return fail(FMMX_EINVAL, "lambda must be positive and finite");
   lam = lambda;
+  return FMMX_OK;
+}
+
+int Engine::set_helmholtz(double k) {
+  if (!(k > 0) || !std::isfinite(k)) return fail(FMMX_EINVAL, "k must be positive and finite");
+  hk = k;
+  return FMMX_OK;
+}
+
+int Engine::upload_helmholtz_operators() {
+  if (hops_lmax == lmax && hops_rd == rd && hops_k == hk) return FMMX_OK;
+  auto S = get_hh_ops(p, hk, rd, lmax);
+  int rc;
+  if (fp64) {
+    if ((rc = upload<double>(hop_m2m, S->m2m, stream)) || (rc = upload<double>(hop_l2l, S->l2l, stream)) ||
+        (rc = upload<double>(hop_m2l, S->m2l_kmaj, stream)))
+      return rc;
+  } else {
+    if ((rc = upload<float>(hop_m2m, S->m2m, stream)) || (rc = upload<float>(hop_l2l, S->l2l, stream)) ||
+        (rc = upload<float>(hop_m2l, S->m2l_kmaj, stream)))
+      return rc;
+  }
+  hops_lmax = lmax;
+  hops_rd = rd;
+  hops_k = hk;
   return FMMX_OK;
 }
 
@@ -470,9 +573,15 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
 
 int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags) {
   if (!have_points) return fail(FMMX_ESTATE, "evaluate before set_points");
-  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ && neq != FMMX_YUKAWA)
-    return fail(FMMX_EINVAL, "neq must be 0, 1, 2 or 10");
+  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ && neq != FMMX_YUKAWA && neq != FMMX_HELMHOLTZ)
+    return fail(FMMX_EINVAL, "neq must be 0, 1, 2, 10 or 20");
   if (neq == FMMX_YUKAWA && !(lam > 0)) return fail(FMMX_ESTATE, "Yukawa needs lambda > 0 (fmmx_set_yukawa)");
+  if (neq == FMMX_HELMHOLTZ) {
+    if (!(hk > 0)) return fail(FMMX_ESTATE, "Helmholtz needs k > 0 (fmmx_set_helmholtz)");
+    if (gxi || gyi || gzi) return fail(FMMX_EUNSUPPORTED, "Helmholtz returns the potential only");
+    if (p < 4 || p > 12 || p % 2) return fail(FMMX_EUNSUPPORTED, "Helmholtz needs p in {4, 6, 8, 10, 12}");
+    if (hk * rb(2) > HH_KRB_MAX) return fail(FMMX_EUNSUPPORTED, "k times the level-2 box size exceeds 5");
+  }
   if (neq == FMMX_GNI && si.n > 0 && !si.has_normals) return fail(FMMX_EINVAL, "Gni needs target normals");
   if (neq == FMMX_GNJ && sj.n > 0 && !sj.has_normals) return fail(FMMX_EINVAL, "Gnj needs source normals");
   if ((sj.n > 0 && !qj) || (si.n > 0 && !vi)) return fail(FMMX_EINVAL, "q or v pointer is NULL");
@@ -482,18 +591,22 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   const size_t es = fp64 ? 8 : 4;
   int rc = FMMX_OK;
   size_t ni1 = (size_t)(ni > 0 ? ni : 1);
+  yk = neq == FMMX_YUKAWA;
+  hh = neq == FMMX_HELMHOLTZ;
+  const int np = parts();
   if ((rc = qs.ensure((size_t)(nj > 0 ? nj : 1) * 4)) || (rc = vs.ensure(ni1 * 8)) || (rc = gxs.ensure(ni1 * 8)) ||
-      (rc = gys.ensure(ni1 * 8)) || (rc = gzs.ensure(ni1 * 8)))
+      (rc = gys.ensure(ni1 * 8)) || (rc = gzs.ensure(ni1 * 8)) ||
+      (hh && (rc = qs2.ensure((size_t)(nj > 0 ? nj : 1) * 4))))
     return rc;
   for (int l = 2; l <= lmax; l++) {
-    if ((rc = mexp[l].ensure((size_t)std::max(1, sj.nb[l]) * VS * es)) ||
-        (rc = lexp[l].ensure((size_t)std::max(1, si.nb[l]) * P2 * es)))
+    if ((rc = mexp[l].ensure((size_t)std::max(1, sj.nb[l]) * VS * np * es)) ||
+        (rc = lexp[l].ensure((size_t)std::max(1, si.nb[l]) * P2 * np * es)))
       return rc;
   }
-  const float* q = stage(qin, qj, nj, dev, stream, rc);
+  const float* q = stage(qin, qj, hh ? 2 * nj : nj, dev, stream, rc);
   if (rc) return rc;
-  yk = neq == FMMX_YUKAWA;
   if (yk && (rc = upload_yukawa_operators())) return rc;
+  if (hh && (rc = upload_helmholtz_operators())) return rc;
   // Yukawa: no far field where lambda * (leaf box) makes it vanish in FP64
   const bool far = !yk || lam * rb(lmax) <= YUK_LRB_MAX;
   int e = 2;
@@ -504,7 +617,10 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   FMMX_HIP(hipEventRecord(ev[2], stream));
   e = 3;
   k_zero_f64<<<1, 64, 0, stream>>>(14, pair_ctr.as<double>() + 2);  // M2L pair counters per level
-  if (nj > 0) k_gather_q<<<nblk(nj), TPB, 0, stream>>>(nj, sj.perm.as<uint32_t>(), q, qs.as<float>());
+  if (nj > 0) {
+    if (hh) k_gather_qc<<<nblk(nj), TPB, 0, stream>>>(nj, sj.perm.as<uint32_t>(), q, qs.as<float>(), qs2.as<float>());
+    else k_gather_q<<<nblk(nj), TPB, 0, stream>>>(nj, sj.perm.as<uint32_t>(), q, qs.as<float>());
+  }
   // Without per-phase timing the near field (vector ALU) runs on the second
   // stream beside the far field (MFMA); L2P, which adds to its results, waits.
   const bool overlap = !per_phase && aux && ni > 0;
@@ -527,7 +643,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     for (int l = lmax - 1; far && l >= 2; l--)
       if ((rc = run_m2m(l))) return rc;
     if ((rc = mark())) return rc;  // ev5: after M2M
-    if (far && overlap && lmax >= 3 && m2l_splits(lmax) > 1) {
+    if (far && overlap && !hh && lmax >= 3 && m2l_splits(lmax) > 1) {
       // levels 2..lmax-1 (M2L, L2L) on the third stream while the leaf M2L
       // computes its partial sums; then the leaf L2L and the reduction that
       // adds the partials (the serialized order below, bit for bit)
@@ -561,6 +677,21 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     if ((rc = mark())) return rc;  // ev7: after L2P
   }
   // outputs in caller order
+  if (hh) {
+    if (ni > 0) {
+      double* dst = vi;
+      if (!dev) {
+        if ((rc = out[0].ensure(2 * ni1 * 8))) return rc;
+        dst = out[0].as<double>();
+      }
+      k_scatter_c<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), vs.as<double>(), gxs.as<double>(), dst);
+      if (!dev) FMMX_HIP(hipMemcpyAsync(vi, dst, (size_t)ni * 16, hipMemcpyDeviceToHost, stream));
+    }
+    FMMX_HIP(hipEventRecord(ev[15], stream));
+    if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
+    last_neq = neq;
+    return FMMX_OK;
+  }
   const bool g = neq == FMMX_G || neq == FMMX_YUKAWA;
   double* outs[4] = {vi, g ? gxi : nullptr, g ? gyi : nullptr, g ? gzi : nullptr};
   double* srcs[4] = {vs.as<double>(), gxs.as<double>(), gys.as<double>(), gzs.as<double>()};
@@ -585,9 +716,40 @@ int Engine::direct(int neq, int ni, const float* xi, const float* yi, const floa
                    const float* nxj, const float* nyj, const float* nzj, const float* qj, double* vi, double* gxi,
                    double* gyi, double* gzi, int flags) {
   if (ni < 0 || nj < 0) return fail(FMMX_EINVAL, "negative particle count");
-  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ && neq != FMMX_YUKAWA)
-    return fail(FMMX_EINVAL, "neq must be 0, 1, 2 or 10");
+  if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ && neq != FMMX_YUKAWA && neq != FMMX_HELMHOLTZ)
+    return fail(FMMX_EINVAL, "neq must be 0, 1, 2, 10 or 20");
   if (neq == FMMX_YUKAWA && !(lam > 0)) return fail(FMMX_ESTATE, "Yukawa needs lambda > 0 (fmmx_set_yukawa)");
+  if (neq == FMMX_HELMHOLTZ) {
+    if (!(hk > 0)) return fail(FMMX_ESTATE, "Helmholtz needs k > 0 (fmmx_set_helmholtz)");
+    if (gxi || gyi || gzi) return fail(FMMX_EUNSUPPORTED, "Helmholtz returns the potential only");
+    if (ni > 0 && (!xi || !yi || !zi || !vi)) return fail(FMMX_EINVAL, "NULL target pointer");
+    if (nj > 0 && (!xj || !yj || !zj || !qj)) return fail(FMMX_EINVAL, "NULL source pointer");
+    FMMX_HIP(hipSetDevice(device));
+    const bool dev = flags & FMMX_DEVICE_PTRS;
+    int rc = FMMX_OK;
+    const float* a[7] = {xi, yi, zi, xj, yj, zj, qj};
+    const int na[7] = {ni, ni, ni, nj, nj, nj, 2 * nj};
+    const int slot[7] = {0, 1, 2, 6, 7, 8, 12};
+    const float* d[7];
+    for (int k = 0; k < 7; k++) {
+      d[k] = stage(k < 6 ? in[slot[k]] : qin, a[k], na[k], dev, stream, rc);
+      if (rc) return rc;
+    }
+    const size_t nj1 = (size_t)std::max(1, nj), ni1 = (size_t)std::max(1, ni);
+    if ((rc = in[3].ensure(nj1 * 4)) || (rc = in[4].ensure(nj1 * 4)) || (rc = out[0].ensure(ni1 * 8)) ||
+        (rc = out[1].ensure(ni1 * 8)) || (rc = out[2].ensure(2 * ni1 * 8)))
+      return rc;
+    if (nj > 0) k_gather_qc<<<nblk(nj), TPB, 0, stream>>>(nj, nullptr, d[6], in[3].as<float>(), in[4].as<float>());
+    if ((rc = run_direct(neq, ni, d[0], d[1], d[2], nullptr, nullptr, nullptr, nj, d[3], d[4], d[5], in[4].as<float>(),
+                         nullptr, nullptr, in[3].as<float>(), out[0].as<double>(), out[1].as<double>(), nullptr,
+                         nullptr)))
+      return rc;
+    double* dst = dev ? vi : out[2].as<double>();
+    if (ni > 0) k_scatter_c<<<nblk(ni), TPB, 0, stream>>>(ni, nullptr, out[0].as<double>(), out[1].as<double>(), dst);
+    if (!dev && ni > 0) FMMX_HIP(hipMemcpyAsync(vi, dst, (size_t)ni * 16, hipMemcpyDeviceToHost, stream));
+    if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
+    return FMMX_OK;
+  }
   if (neq == FMMX_GNI && ni > 0 && (!nxi || !nyi || !nzi)) return fail(FMMX_EINVAL, "Gni needs target normals");
   if (neq == FMMX_GNJ && nj > 0 && (!nxj || !nyj || !nzj)) return fail(FMMX_EINVAL, "Gnj needs source normals");
   if (ni > 0 && (!xi || !yi || !zi || !vi)) return fail(FMMX_EINVAL, "NULL target pointer");
@@ -751,6 +913,13 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
   FMMX_GUARD({
     if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
     return h->e.set_domain(lmax, xmin, ymin, zmin, rd);
+  })
+}
+
+int fmmx_set_helmholtz(fmmx_handle* h, double k) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.set_helmholtz(k);
   })
 }
 

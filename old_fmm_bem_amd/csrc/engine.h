@@ -123,6 +123,18 @@ struct Engine {
   int yops_lmax = -1;
   float yops_rd = -1;
   double yops_lam = -1;
+  // Helmholtz (FMMX_HELMHOLTZ): wavenumber, imaginary charges (key order) and
+  // the operator sets per level on split expansions (two parts: the fields
+  // of the real and of the imaginary charges, locals of Re and Im of the
+  // potential): M2M (2 VS)^2 by parent level 2..lmax-1, L2L (2 P2)^2 by child
+  // level 3..lmax, M2L (2 P2)^2 k-major by level
+  double hk = 0;
+  bool hh = false;  // the evaluation in progress is Helmholtz
+  DevBuf qs2, hop_m2m, hop_l2l, hop_m2l;
+  int hops_lmax = -1;
+  float hops_rd = -1;
+  double hops_k = -1;
+  int parts() const { return hh ? 2 : 1; }
   // stand-alone Morton sort (fmmx_sort): staging, keys, permutation
   DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm;
   int last_neq = -1;
@@ -146,6 +158,8 @@ struct Engine {
   int set_domain(int lmax, float xmin, float ymin, float zmin, float rd);
   int set_yukawa(double lambda);
   int upload_yukawa_operators();
+  int set_helmholtz(double k);
+  int upload_helmholtz_operators();
 
   // tree.hip
   int sort_points(int n, const float* x, const float* y, const float* z, int* keys, int* perm, int flags);
@@ -173,10 +187,12 @@ struct Engine {
   // translation operators of the evaluation in progress
   template <class T>
   const T* m2m_ops(int parent_lev) const {
+    if (hh) return hop_m2m.as<T>() + (size_t)(parent_lev - 2) * 8 * 4 * VS * VS;
     return yk ? yop_m2m.as<T>() + (size_t)(parent_lev - 2) * 8 * VS * VS : op_m2m.as<T>();
   }
   template <class T>
   const T* l2l_ops(int child_lev) const {
+    if (hh) return hop_l2l.as<T>() + (size_t)(child_lev - 3) * 8 * 4 * P2 * P2;
     return yk ? yop_l2l.as<T>() + (size_t)(child_lev - 3) * 8 * P2 * P2 : op_l2l.as<T>();
   }
   double rb(int lev) const { return (double)rd / (double)(1 << lev); }

@@ -182,6 +182,36 @@ __device__ inline void ibar_dev(double x, double* out) {
   }
 }
 
+// j^_n(x) = (2n+1)!! j_n(x) / x^n for n = 0..N (spherical Bessel function of
+// the first kind, scaled to 1 at x = 0): power series for N+1 and N, then
+// the stable backward recurrence j^_{n-1} = j^_n - x^2 j^_{n+1}/((2n+1)(2n+3))
+template <int N>
+__device__ inline void jhat_dev(double x, double* out) {
+  const double h = -0.5 * x * x;
+  double top[2];
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int n = N + 1 - u;
+    double t = 1, sum = 1, tmax = 1;
+    for (int k = 1; k < 400; k++) {
+      t *= h / (k * (2.0 * n + 2.0 * k + 1.0));
+      sum += t;
+      tmax = fmax(tmax, fabs(t));
+      if (fabs(t) < 1e-18 * tmax) break;
+    }
+    top[u] = sum;
+  }
+  double a = top[0], b = top[1];
+  out[N] = b;
+#pragma unroll
+  for (int n = N; n >= 1; n--) {
+    const double c = b - x * x * a / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
+    a = b;
+    b = c;
+    out[n - 1] = c;
+  }
+}
+
 __device__ __forceinline__ void box_center(uint32_t key, int lev, double xmin, double ymin, double zmin, double rb,
                                            double& cx, double& cy, double& cz) {
   int ix, iy, iz;
@@ -193,9 +223,13 @@ __device__ __forceinline__ void box_center(uint32_t key, int lev, double xmin, d
 
 // ---------------------------------------------------------------------
 // P2M: one wave per source leaf box
-//   YK: Yukawa multipoles, M~_n^m = sum q i~_n(lambda rho)/i~_n(lambda rb)
+//   YK 1: Yukawa multipoles, M~_n^m = sum q i~_n(lambda rho)/i~_n(lambda rb)
 //       (rho/rb)^n conj(Y_n^m): the Laplace harmonics times a radial weight
 //       per degree (lrb = lambda rb)
+//   YK 2: Helmholtz multipoles of one part of the charges (real or
+//       imaginary), M_n^m = sum q j^_n(k rho) (rho/rb)^n conj(Y_n^m)
+//       (lrb = k rb)
+//   Box b's multipole starts at M + b * mstride + moff.
 template <int P, typename T, int YK = 0>
 __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
@@ -203,7 +237,8 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
                                             const float* __restrict__ nx, const float* __restrict__ ny,
                                             const float* __restrict__ nz, const float* __restrict__ q, int gn,
                                             int conv, int lev, double xmin, double ymin, double zmin, double rb,
-                                            const double* __restrict__ corr, T* __restrict__ M, double lrb) {
+                                            const double* __restrict__ corr, T* __restrict__ M, double lrb,
+                                            int mstride, int moff) {
   constexpr int NR = P * (P + 1) / 2, P2 = P * P, NC = (P2 + 63) / 64;
   __shared__ T red[64][P2 + 1];
   const int b = blockIdx.x, lane = threadIdx.x;
@@ -225,12 +260,17 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
       solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
       if (YK) {
         double ib0[P], ibx[P];
-        ibar_dev<P - 1>(lrb, ib0);
-        ibar_dev<P - 1>(lrb * sqrt(ux * ux + uy * uy + uz * uz), ibx);
+        const double lr = lrb * sqrt(ux * ux + uy * uy + uz * uz);
+        if (YK == 2) {
+          jhat_dev<P - 1>(lr, ibx);
+        } else {
+          ibar_dev<P - 1>(lrb, ib0);
+          ibar_dev<P - 1>(lr, ibx);
+        }
         const double qd = (double)q[i];
 #pragma unroll
         for (int n = 0; n < P; n++) {
-          const T w = T(qd * ibx[n] / ib0[n]);
+          const T w = T(YK == 2 ? qd * ibx[n] : qd * ibx[n] / ib0[n]);
 #pragma unroll
           for (int m = 0; m <= n; m++) {
             row[sl(n, m)] = w * Sr[sl(n, m)];
@@ -285,18 +325,21 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const int col = lane + 64 * c;
-    if (col < P2) M[(size_t)b * ML::VS + ML::at(col)] = acc[c];
+    if (col < P2) M[(size_t)b * mstride + moff + ML::at(col)] = acc[c];
   }
   // zero the padding slots (M2M applies its operator to whole stored vectors)
   for (int c = lane; c < ML::VS; c += 64)
-    if (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2) M[(size_t)b * ML::VS + c] = T(0);
+    if (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2) M[(size_t)b * mstride + moff + c] = T(0);
 }
 
 // ---------------------------------------------------------------------
 // L2P: one wave per target leaf box; adds to the P2P results (sorted order)
-//   YK: Yukawa locals, phi = sum_n h_n(r) sum_m L~ S_n^m(u) with
+//   YK 1: Yukawa locals, phi = sum_n h_n(r) sum_m L~ S_n^m(u) with
 //       h_n = i~_n(lambda r)/i~_n(lambda rb); grad adds
 //       lrb^2 i~_{n+1}(lambda r)/i~_n(lambda rb) (sum_m L~ S_n^m) u
+//   YK 2: one part (Re or Im of the field) of Helmholtz locals,
+//       phi = c4/rb sum_n j^_n(k r) sum_m L S_n^m(u), potential only
+//   Box b's local starts at L + b * lstride + loff.
 template <int P, typename T, int YK = 0>
 __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
@@ -306,7 +349,7 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
                                             int conv, int lev, double xmin, double ymin, double zmin, double rb,
                                             const double* __restrict__ corr, double c4, double* __restrict__ vs,
                                             double* __restrict__ gxs, double* __restrict__ gys,
-                                            double* __restrict__ gzs, double lrb) {
+                                            double* __restrict__ gzs, double lrb, int lstride, int loff) {
   constexpr int NR = P * (P + 1) / 2, P2 = P * P;
   __shared__ T Lr[NR], Li[NR];
   const int b = blockIdx.x, lane = threadIdx.x;
@@ -317,8 +360,8 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
     while (sl(n + 1, 0) <= s) n++;
     const int m = s - sl(n, 0);
     const double w = (m == 0 ? 1.0 : 2.0) * (YK ? 1.0 : corr[s]);
-    Lr[s] = T(w * (double)L[(size_t)b * P2 + s]);
-    Li[s] = m == 0 ? T(0) : T(w * (double)L[(size_t)b * P2 + NR + n * (n - 1) / 2 + m - 1]);
+    Lr[s] = T(w * (double)L[(size_t)b * lstride + loff + s]);
+    Li[s] = m == 0 ? T(0) : T(w * (double)L[(size_t)b * lstride + loff + NR + n * (n - 1) / 2 + m - 1]);
   }
   __syncthreads();
   double cx, cy, cz;
@@ -331,7 +374,17 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
     T Sr[NR], Si[NR];
     solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
     T phi = T(0), gx = T(0), gy = T(0), gz = T(0);
-    if constexpr (YK) {
+    if constexpr (YK == 2) {
+      double jn[P];
+      jhat_dev<P - 1>(lrb * sqrt(ux * ux + uy * uy + uz * uz), jn);
+#pragma unroll
+      for (int n = 0; n < P; n++) {
+        T pn = T(0);
+#pragma unroll
+        for (int m = 0; m <= n; m++) pn += Sr[sl(n, m)] * Lr[sl(n, m)] - Si[sl(n, m)] * Li[sl(n, m)];
+        phi += T(jn[n]) * pn;
+      }
+    } else if constexpr (YK) {
       double hn[P + 1], h0[P];
       ibar_dev<P - 1>(lrb, h0);
       ibar_dev<P>(lrb * sqrt(ux * ux + uy * uy + uz * uz), hn);
@@ -413,9 +466,9 @@ __global__ __launch_bounds__(TR_NT) void k_translate(int nout, const int* __rest
                                                      const T* __restrict__ opT) {
   // M2M: blockIdx.y = child octant; this block writes that octant's partial
   // product into out + octant * nout * VS (summed afterwards in fixed order)
-  constexpr int RPT = (VS + 31) / 32;
-  __shared__ T As[TR_KC][VS];
-  __shared__ T Bs[TR_KC][TR_BST];
+  constexpr int RPT = (VS + 31) / 32, KC = VS > 160 ? TR_KC / 2 : TR_KC;  // LDS within 64 KB
+  __shared__ T As[KC][VS];
+  __shared__ T Bs[KC][TR_BST];
   __shared__ int src[TR_TB], dst[TR_TB];
   __shared__ int s_oct;
   const int tid = threadIdx.x;
@@ -476,8 +529,8 @@ __global__ __launch_bounds__(TR_NT) void k_translate(int nout, const int* __rest
     }
     __syncthreads();
     const T* A = opT + (size_t)o * VS * VS;
-    for (int k0 = 0; k0 < VS; k0 += TR_KC) {
-      const int kc = min(TR_KC, VS - k0);
+    for (int k0 = 0; k0 < VS; k0 += KC) {
+      const int kc = min(KC, VS - k0);
       __syncthreads();
       for (int e = tid; e < kc * VS; e += TR_NT) As[e / VS][e % VS] = A[(size_t)k0 * VS + e];
       for (int e = tid; e < kc * TR_TB; e += TR_NT) {
@@ -804,16 +857,20 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
 // and each thread accumulates a (rows x 4 targets) register block.
 constexpr int M2LV_KC = 32, M2LV_BST = M2L_TN + 4;
 
-template <int P, typename T>
+// PARTS 2 (Helmholtz): split expansions, D = 2 p^2 rows and k-indices; the
+// multipole part t of box s sits at s * 2 VS + t * VS in the padded layout.
+template <int P, typename T, int PARTS = 1>
 __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
                                                  const uint32_t* __restrict__ ord, const uint32_t* __restrict__ ikey,
                                                  const int* __restrict__ dense_j, const T* __restrict__ M,
                                                  T* __restrict__ L, T* __restrict__ part, const T* __restrict__ opT,
                                                  const int4* __restrict__ offtab, int init,
                                                  unsigned long long* __restrict__ npairs) {
-  constexpr int P2 = P * P, RPT = (P2 + 15) / 16;
-  __shared__ T As[M2LV_KC][P2];
-  __shared__ T Bs[M2LV_KC][M2LV_BST];
+  using ML = MLayout<P>;
+  constexpr int P2 = P * P, D = PARTS * P2, RPT = (D + 15) / 16;
+  constexpr int KC = D * (int)sizeof(T) > 1200 ? M2LV_KC / 2 : M2LV_KC;  // LDS within 64 KB
+  __shared__ T As[KC][D];
+  __shared__ T Bs[KC][M2LV_BST];
   __shared__ int tpos[M2L_TN][3];
   __shared__ int tord[M2L_TN];
   __shared__ int src[M2L_TN];
@@ -872,14 +929,15 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
       src[tid] = s;
       if (s >= 0) mypairs++;
     }
-    const T* A = opT + (size_t)off.w * P2 * P2;
-    for (int k0 = 0; k0 < P2; k0 += M2LV_KC) {
-      const int kc = min(M2LV_KC, P2 - k0);
+    const T* A = opT + (size_t)off.w * D * D;
+    for (int k0 = 0; k0 < D; k0 += KC) {
+      const int kc = min(KC, D - k0);
       __syncthreads();
-      for (int e = tid; e < kc * P2; e += 256) As[e / P2][e % P2] = A[(size_t)k0 * P2 + e];
+      for (int e = tid; e < kc * D; e += 256) As[e / D][e % D] = A[(size_t)k0 * D + e];
       for (int e = tid; e < kc * M2L_TN; e += 256) {
-        const int t = e / kc, k = e % kc, s = src[t];
-        Bs[k][t] = s >= 0 ? M[(size_t)s * MLayout<P>::VS + MLayout<P>::at(k0 + k)] : T(0);
+        const int t = e / kc, k = k0 + e % kc, s = src[t];
+        const int part = PARTS == 1 ? 0 : k / P2, kk = PARTS == 1 ? k : k % P2;
+        Bs[e % kc][t] = s >= 0 ? M[(size_t)s * PARTS * ML::VS + part * ML::VS + ML::at(kk)] : T(0);
       }
       __syncthreads();
       for (int k = 0; k < kc; k++) {
@@ -889,7 +947,7 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
           const int row = ty + 16 * r;
-          const T av = row < P2 ? As[k][row] : T(0);
+          const T av = row < D ? As[k][row] : T(0);
 #pragma unroll
           for (int c = 0; c < 4; c++) acc[r][c] += av * bv[c];
         }
@@ -903,12 +961,12 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       const int row = ty + 16 * r;
-      if (row >= P2) continue;
+      if (row >= D) continue;
       if (nsplit == 1) {
-        T* dst = L + (size_t)tord[t] * P2 + row;
+        T* dst = L + (size_t)tord[t] * D + row;
         *dst = init ? acc[r][c] : *dst + acc[r][c];
       } else {
-        part[((size_t)split * nb + tord[t]) * P2 + row] = acc[r][c];
+        part[((size_t)split * nb + tord[t]) * D + row] = acc[r][c];
       }
     }
   }
@@ -957,7 +1015,17 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
     if (SELF) inv = r2 > T(0) ? inv : T(0);  // coincident pair excluded (select, no branch)
   }
   const T qi = T(s.w) * inv;
-  if (NEQ == FMMX_YUKAWA) {
+  if (NEQ == FMMX_HELMHOLTZ) {
+    // (qr + i qi) exp(i k r) / r: real part into p, imaginary part into fx
+    // (sn.x carries the imaginary part of the charge, lam the wavenumber)
+    const T r = r2 * inv;
+    T sk, ck;
+    if constexpr (sizeof(T) == 4) sincosf(lam * r, &sk, &ck);
+    else sincos(lam * r, &sk, &ck);
+    const T qim = T(sn.x) * inv;
+    p += qi * ck - qim * sk;
+    fx += qi * sk + qim * ck;
+  } else if (NEQ == FMMX_YUKAWA) {
     // exp(-lambda r)/r (kernel/Helmholtz_G_p2p.cxx:22 without its constant
     // (pi/2)/lambda); gradient -(1 + lambda r) e d / r^3
     const T r = r2 * inv;
@@ -995,9 +1063,12 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
                                             double* __restrict__ gxs, double* __restrict__ gys,
                                             double* __restrict__ gzs, double lamd) {
   constexpr bool GRAD = NEQ == FMMX_G || NEQ == FMMX_YUKAWA;
+  // Helmholtz: nxj holds the imaginary parts of the charges, gxs receives the
+  // imaginary part of the potential
+  constexpr bool HH = NEQ == FMMX_HELMHOLTZ, SN = NEQ == FMMX_GNJ || HH;
   const T lam = T(lamd);
   __shared__ float4 sp[BS];
-  __shared__ float4 sn[NEQ == FMMX_GNJ ? BS : 1];
+  __shared__ float4 sn[SN ? BS : 1];
   const int b = blockIdx.x, tid = threadIdx.x;
   if (b >= nbi) return;
   const int i0 = ifirst[b], i1 = ilast[b], nn = nij[b];
@@ -1025,6 +1096,7 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
         if (js <= j1) {
           sp[tid] = make_float4(xj[js], yj[js], zj[js], q[js]);
           if (NEQ == FMMX_GNJ) sn[tid] = make_float4(nxj[js], nyj[js], nzj[js], 0.f);
+          if (HH) sn[tid] = make_float4(nxj[js], 0.f, 0.f, 0.f);
         }
         __syncthreads();
         const int cnt = min(BS, j1 - jb + 1);
@@ -1035,24 +1107,22 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
           for (; s + 4 <= cnt; s += 4) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
-              pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx,
-                                        fy, fz, lam);
+              pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u], SN ? sn[s + u] : sp[0], p, fx, fy, fz,
+                                        lam);
           }
           for (; s < cnt; s++)
-            pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz,
-                                      lam);
+            pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
         } else {
           for (; s + 4 <= cnt; s += 4) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
-              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], NEQ == FMMX_GNJ ? sn[s + u] : sp[0], p, fx, fy, fz,
-                                 lam);
+              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], SN ? sn[s + u] : sp[0], p, fx, fy, fz, lam);
           }
           for (; s < cnt; s++)
-            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], NEQ == FMMX_GNJ ? sn[s] : sp[0], p, fx, fy, fz, lam);
+            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
         }
         ap += (double)p;
-        if (GRAD) {
+        if (GRAD || HH) {
           ax += (double)fx;
           ay += (double)fy;
           az += (double)fz;
@@ -1061,6 +1131,7 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
     }
     if (act) {
       vs[i] = c4 * ap;
+      if (HH) gxs[i] = c4 * ax;
       if (GRAD) {
         gxs[i] = c4 * ax;
         gys[i] = c4 * ay;
@@ -1082,7 +1153,7 @@ __global__ __launch_bounds__(64) void k_direct(int ni, const float* __restrict__
                                                const float* __restrict__ nzj, int slice, double c4,
                                                double* __restrict__ v, double* __restrict__ gx,
                                                double* __restrict__ gy, double* __restrict__ gz, double lam) {
-  constexpr bool GRAD = NEQ == FMMX_G || NEQ == FMMX_YUKAWA;
+  constexpr bool GRAD = NEQ == FMMX_G || NEQ == FMMX_YUKAWA, HH = NEQ == FMMX_HELMHOLTZ;
   __shared__ float4 sp[64];
   __shared__ float4 sn[64];
   const int i = blockIdx.x * 64 + threadIdx.x;
@@ -1106,6 +1177,7 @@ __global__ __launch_bounds__(64) void k_direct(int ni, const float* __restrict__
     if (js < j1) {
       sp[threadIdx.x] = make_float4(xj[js], yj[js], zj[js], q[js]);
       if (NEQ == FMMX_GNJ) sn[threadIdx.x] = make_float4(nxj[js], nyj[js], nzj[js], 0.f);
+      if (HH) sn[threadIdx.x] = make_float4(nxj[js], 0.f, 0.f, 0.f);  // imaginary charge
     }
     __syncthreads();
     const int cnt = min(64, j1 - jb);
@@ -1114,6 +1186,7 @@ __global__ __launch_bounds__(64) void k_direct(int ni, const float* __restrict__
   }
   if (act) {
     atomicAdd(&v[i], c4 * p);
+    if (HH) atomicAdd(&gx[i], c4 * fx);
     if (GRAD) {
       atomicAdd(&gx[i], c4 * fx);
       atomicAdd(&gy[i], c4 * fy);
@@ -1150,6 +1223,19 @@ int dispatch_p(int p, F&& f) {
   return fail(FMMX_EUNSUPPORTED, "unsupported p");
 }
 
+// the orders the Helmholtz kernels are built for
+template <typename F>
+int dispatch_hp(int p, F&& f) {
+  switch (p) {
+#define FMMX_CASE(P) \
+  case P:            \
+    return f(std::integral_constant<int, P>{});
+    FMMX_CASE(4) FMMX_CASE(6) FMMX_CASE(8) FMMX_CASE(10) FMMX_CASE(12)
+#undef FMMX_CASE
+  }
+  return fail(FMMX_EUNSUPPORTED, "Helmholtz needs p in {4, 6, 8, 10, 12}");
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -1161,15 +1247,18 @@ int Engine::run_p2p(int neq) {
   const double cc = neq == FMMX_YUKAWA ? c_yukawa() : c4();
   const bool ref = cfg.convention == FMMX_CONV_REF;
   const double mean = (double)si.n / nbi;
+  // Helmholtz: the imaginary charges ride in the source-normal slot, k in lambda's
+  const float* nxj = neq == FMMX_HELMHOLTZ ? qs2.as<float>() : sj.nx.as<float>();
+  const double kl = neq == FMMX_HELMHOLTZ ? hk : lam;
   auto go = [&](auto tag_t, auto neq_c, auto conv_c, auto bs_c) -> int {
     using T = typename decltype(tag_t)::type;
     constexpr int NEQ = decltype(neq_c)::value, CONV = decltype(conv_c)::value, BS = decltype(bs_c)::value;
     k_p2p<T, NEQ, CONV, BS><<<nbi, BS, 0, stream>>>(
         nbi, si.bkey[lmax].as<uint32_t>(), sj.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(), si.blast.as<int>(), nij.as<int>(), neij.as<int>(), sj.bfirst[lmax].as<int>(),
         sj.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(),
-        si.nz.as<float>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), qs.as<float>(), sj.nx.as<float>(),
+        si.nz.as<float>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), qs.as<float>(), nxj,
         sj.ny.as<float>(), sj.nz.as<float>(), cc, vs.as<double>(), gxs.as<double>(), gys.as<double>(),
-        gzs.as<double>(), lam);
+        gzs.as<double>(), kl);
     return FMMX_OK;
   };
   auto by_bs = [&](auto t, auto nq, auto cv) -> int {
@@ -1182,6 +1271,7 @@ int Engine::run_p2p(int neq) {
                : by_bs(t, nq, std::integral_constant<int, FMMX_CONV_EXACT>{});
   };
   auto by_neq = [&](auto t) -> int {
+    if (neq == FMMX_HELMHOLTZ) return by_conv(t, std::integral_constant<int, FMMX_HELMHOLTZ>{});
     if (neq == FMMX_YUKAWA) return by_conv(t, std::integral_constant<int, FMMX_YUKAWA>{});
     if (neq == FMMX_G) return by_conv(t, std::integral_constant<int, FMMX_G>{});
     if (neq == FMMX_GNI) return by_conv(t, std::integral_constant<int, FMMX_GNI>{});
@@ -1199,19 +1289,32 @@ int Engine::run_p2m(int neq) {
   const double* corr = corr_d.as<double>();
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    const double lrb = lam * rb(lmax);
-#define FMMX_P2M(T_, YK_)                                                                                          \
+    const double lrb = (hh ? hk : lam) * rb(lmax);
+#define FMMX_P2M(T_, YK_, Q_, STRIDE_, OFF_)                                                                      \
   k_p2m<P, T_, YK_><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),          \
                                            sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), \
-                                           sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), qs.as<float>(),  \
+                                           sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), Q_,              \
                                            neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, \
-                                           mexp[lmax].as<T_>(), lrb)
-    if (fp64) {
-      if (yk) FMMX_P2M(double, 1);
-      else FMMX_P2M(double, 0);
+                                           mexp[lmax].as<T_>(), lrb, STRIDE_, OFF_)
+    if (hh) {
+      // the multipoles of the real and of the imaginary charges, side by side
+      if constexpr (P % 2 == 0 && P >= 4) {
+        if (fp64) {
+          FMMX_P2M(double, 2, qs.as<float>(), 2 * VS, 0);
+          FMMX_P2M(double, 2, qs2.as<float>(), 2 * VS, VS);
+        } else {
+          FMMX_P2M(float, 2, qs.as<float>(), 2 * VS, 0);
+          FMMX_P2M(float, 2, qs2.as<float>(), 2 * VS, VS);
+        }
+      } else {
+        return fail(FMMX_EUNSUPPORTED, "Helmholtz needs p in {4, 6, 8, 10, 12}");
+      }
+    } else if (fp64) {
+      if (yk) FMMX_P2M(double, 1, qs.as<float>(), VS, 0);
+      else FMMX_P2M(double, 0, qs.as<float>(), VS, 0);
     } else {
-      if (yk) FMMX_P2M(float, 1);
-      else FMMX_P2M(float, 0);
+      if (yk) FMMX_P2M(float, 1, qs.as<float>(), VS, 0);
+      else FMMX_P2M(float, 0, qs.as<float>(), VS, 0);
     }
 #undef FMMX_P2M
     FMMX_HIP(hipGetLastError());
@@ -1226,21 +1329,34 @@ int Engine::run_l2p(int neq) {
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     // Yukawa locals carry the kernel's constant: phi = sum L~ h_n S_n(u)
-    const double lrb = lam * rb(lmax), cl = yk ? rb(lmax) : c4();
+    const double lrb = (hh ? hk : lam) * rb(lmax), cl = yk ? rb(lmax) : c4();
     const int grad = neq == FMMX_G || neq == FMMX_YUKAWA;
-#define FMMX_L2P(T_, YK_)                                                                                          \
+#define FMMX_L2P(T_, YK_, V_, STRIDE_, OFF_)                                                                      \
   k_l2p<P, T_, YK_><<<nb, 64, 0, stream>>>(nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(),          \
                                            si.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), \
                                            si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),                 \
                                            lexp[lmax].as<T_>(), neq == FMMX_GNI, grad, cfg.convention, lmax, xmin,  \
-                                           ymin, zmin, rb(lmax), corr, cl, vs.as<double>(), gxs.as<double>(),      \
-                                           gys.as<double>(), gzs.as<double>(), lrb)
-    if (fp64) {
-      if (yk) FMMX_L2P(double, 1);
-      else FMMX_L2P(double, 0);
+                                           ymin, zmin, rb(lmax), corr, cl, V_, gxs.as<double>(), gys.as<double>(),  \
+                                           gzs.as<double>(), lrb, STRIDE_, OFF_)
+    if (hh) {
+      // Re of the potential into vs, Im into gxs
+      if constexpr (P % 2 == 0 && P >= 4) {
+        if (fp64) {
+          FMMX_L2P(double, 2, vs.as<double>(), 2 * P2, 0);
+          FMMX_L2P(double, 2, gxs.as<double>(), 2 * P2, P2);
+        } else {
+          FMMX_L2P(float, 2, vs.as<double>(), 2 * P2, 0);
+          FMMX_L2P(float, 2, gxs.as<double>(), 2 * P2, P2);
+        }
+      } else {
+        return fail(FMMX_EUNSUPPORTED, "Helmholtz needs p in {4, 6, 8, 10, 12}");
+      }
+    } else if (fp64) {
+      if (yk) FMMX_L2P(double, 1, vs.as<double>(), P2, 0);
+      else FMMX_L2P(double, 0, vs.as<double>(), P2, 0);
     } else {
-      if (yk) FMMX_L2P(float, 1);
-      else FMMX_L2P(float, 0);
+      if (yk) FMMX_L2P(float, 1, vs.as<double>(), P2, 0);
+      else FMMX_L2P(float, 0, vs.as<double>(), P2, 0);
     }
 #undef FMMX_L2P
     FMMX_HIP(hipGetLastError());
@@ -1253,9 +1369,29 @@ int Engine::run_m2m(int lev) {
   if (nbp == 0) return FMMX_OK;
   const dim3 grid((nbp + TR_TB - 1) / TR_TB, 8);
   const size_t es = fp64 ? 8 : 4;
-  const long n = (long)nbp * VS;
+  const long n = (long)nbp * VS * parts();
   int rc = m2m_part.ensure((size_t)8 * n * es);
   if (rc) return rc;
+  if (hh) {
+    return dispatch_hp(p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value, V = 2 * MLayout<P>::VS;
+      if (fp64) {
+        k_translate<V, double, 0><<<grid, TR_NT, 0, stream>>>(
+            nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
+            mexp[lev + 1].as<double>(), m2m_part.as<double>(), m2m_ops<double>(lev));
+        k_m2l_reduce<double><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, 8, m2m_part.as<double>(),
+                                                                         mexp[lev].as<double>(), 1);
+      } else {
+        k_translate<V, float, 0><<<grid, TR_NT, 0, stream>>>(
+            nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
+            mexp[lev + 1].as<float>(), m2m_part.as<float>(), m2m_ops<float>(lev));
+        k_m2l_reduce<float><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, 8, m2m_part.as<float>(),
+                                                                        mexp[lev].as<float>(), 1);
+      }
+      FMMX_HIP(hipGetLastError());
+      return FMMX_OK;
+    });
+  }
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value, V = MLayout<P>::VS;
     if (fp64) {
@@ -1280,6 +1416,21 @@ int Engine::run_l2l(int lev) {
   const int nbc = si.nb[lev];
   if (nbc == 0) return FMMX_OK;
   const int grid = (nbc + TR_TB - 1) / TR_TB + 8;
+  if (hh) {
+    return dispatch_hp(p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value, Q = 2 * P * P;
+      if (fp64)
+        k_translate<Q, double, 1><<<grid, TR_NT, 0, stream>>>(
+            nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
+            cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<double>(), lexp[lev].as<double>(), l2l_ops<double>(lev));
+      else
+        k_translate<Q, float, 1><<<grid, TR_NT, 0, stream>>>(
+            nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
+            cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<float>(), lexp[lev].as<float>(), l2l_ops<float>(lev));
+      FMMX_HIP(hipGetLastError());
+      return FMMX_OK;
+    });
+  }
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value, Q = P * P;
     if (fp64)
@@ -1300,7 +1451,7 @@ int Engine::m2l_splits(int lev) const {
   const int nb = si.nb[lev];
   int split_env = 0;
   if (const char* e = getenv("FMMX_M2L_SPLIT")) split_env = atoi(e);
-  if (fp64) {
+  if (fp64 || hh) {
     const int tiles = (nb + M2L_TN - 1) / M2L_TN + 8;
     return std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / tiles));
   }
@@ -1319,7 +1470,7 @@ int Engine::m2l_splits(int lev) const {
 int Engine::run_m2l(int lev, bool init, int phase) {
   const int nb = si.nb[lev];
   if (nb == 0) return FMMX_OK;
-  const long nl = (long)nb * P2;
+  const long nl = (long)nb * P2 * parts();
   if (sj.nb[lev] == 0) {
     if (init && phase != 1) {
       if (fp64) k_zero_t<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, lexp[lev].as<double>());
@@ -1353,6 +1504,34 @@ int Engine::run_m2l(int lev, bool init, int phase) {
     if (rc) return rc;
   }
   int rc2 = FMMX_OK;
+  if (hh) {
+    // split complex expansions: D = 2 p^2, k-major operators per level, on the
+    // vector units in both precisions
+    const size_t hoff = (size_t)(lev - 2) * NOFF * 4 * P2 * P2;
+    return dispatch_hp(p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      if (fp64)
+        k_m2l_vec<P, double, 2><<<tiles * nsplit, 256, 0, stream>>>(
+            lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
+            sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), part_buf->as<double>(),
+            hop_m2l.as<double>() + hoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev);
+      else
+        k_m2l_vec<P, float, 2><<<tiles * nsplit, 256, 0, stream>>>(
+            lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
+            sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), part_buf->as<float>(),
+            hop_m2l.as<float>() + hoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev);
+      if (nsplit > 1 && phase == 0) {
+        if (fp64)
+          k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
+                                                                            lexp[lev].as<double>(), init ? 1 : 0);
+        else
+          k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<float>(),
+                                                                           lexp[lev].as<float>(), init ? 1 : 0);
+      }
+      FMMX_HIP(hipGetLastError());
+      return FMMX_OK;
+    });
+  }
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if (fp64) {
@@ -1419,6 +1598,7 @@ int Engine::run_direct(int neq, int ni, const float* xi, const float* yi, const 
   if (ni == 0) return FMMX_OK;
   long nz4 = ni;
   k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, v);
+  if (neq == FMMX_HELMHOLTZ) k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gx);  // Im part
   if (neq == FMMX_G || neq == FMMX_YUKAWA) {
     k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gx);
     k_zero_d<<<(int)((nz4 + 255) / 256), 256, 0, stream>>>(nz4, gy);
@@ -1433,10 +1613,15 @@ int Engine::run_direct(int neq, int ni, const float* xi, const float* yi, const 
   dim3 grid(tb, ns);
   const bool ref = cfg.convention == FMMX_CONV_REF;
   const double cc = neq == FMMX_YUKAWA ? c_yukawa() : c4();
+  // Helmholtz: nxj carries the imaginary charges, gx receives Im of the potential
+  const double kl = neq == FMMX_HELMHOLTZ ? hk : lam;
 #define FMMX_DIR(NEQ, CONV)                                                                                      \
   k_direct<NEQ, CONV><<<grid, 64, 0, stream>>>(ni, xi, yi, zi, nxi, nyi, nzi, nj, xj, yj, zj, qj, nxj, nyj, nzj, \
-                                               slice, cc, v, gx, gy, gz, lam)
-  if (neq == FMMX_YUKAWA) {
+                                               slice, cc, v, gx, gy, gz, kl)
+  if (neq == FMMX_HELMHOLTZ) {
+    if (ref) FMMX_DIR(FMMX_HELMHOLTZ, FMMX_CONV_REF);
+    else FMMX_DIR(FMMX_HELMHOLTZ, FMMX_CONV_EXACT);
+  } else if (neq == FMMX_YUKAWA) {
     if (ref) FMMX_DIR(FMMX_YUKAWA, FMMX_CONV_REF);
     else FMMX_DIR(FMMX_YUKAWA, FMMX_CONV_EXACT);
   } else if (ref) {

@@ -1098,6 +1098,37 @@ void orc_direct(int neq, int conv, int nthreads, int ni, const float* xi, const 
   }
 }
 
+void orc_direct_helmholtz(int conv, int nthreads, double k, int ni, const float* xi, const float* yi,
+                          const float* zi, int nj, const float* xj, const float* yj, const float* zj,
+                          const float* qr, const float* qi, double* vr, double* vim) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const double c = conv == ORC_CONV_REF ? 0.25 / (double)PIF : 0.25 / M_PI;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < ni; i++) {
+    double sr = 0, si = 0;
+    for (int j = 0; j < nj; j++) {
+      double r2;
+      if (conv == ORC_CONV_REF) {
+        double dx = xi[i] - xj[j], dy = yi[i] - yj[j], dz = zi[i] - zj[j];
+        r2 = dx * dx + dy * dy + dz * dz + EPSF;
+      } else {
+        double dx = (double)xi[i] - xj[j], dy = (double)yi[i] - yj[j], dz = (double)zi[i] - zj[j];
+        r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0) continue;
+      }
+      double r = sqrt(r2), ck = cos(k * r) / r, sk = sin(k * r) / r;
+      sr += qr[j] * ck - qi[j] * sk;
+      si += qr[j] * sk + qi[j] * ck;
+    }
+    vr[i] = c * sr;
+    vim[i] = c * si;
+  }
+}
+
 void orc_direct_yukawa(int conv, int nthreads, double lambda, int ni, const float* xi, const float* yi,
                        const float* zi, int nj, const float* xj, const float* yj, const float* zj,
                        const float* qj, double* vi, double* gxi, double* gyi, double* gzi) {

@@ -104,6 +104,16 @@ void orc_direct_yukawa(int conv, int nthreads, double lambda, int ni, const floa
                        const float* zi, int nj, const float* xj, const float* yj, const float* zj,
                        const float* qj, double* vi, double* gxi, double* gyi, double* gzi);
 
+/* Oscillatory Helmholtz direct sum, no reference counterpart (the
+ * reference's "Helmholtz" files are the screened kernel above):
+ * v_i = sum_j (qr_j + i qi_j) exp(i k r) / (4 pi r) in FP64 from the float
+ * inputs.  REF: r = sqrt(r^2 + eps) with the self pair included; EXACT: r = 0
+ * pairs excluded.  Pinned by closed forms in the tests (a plane-wave
+ * superposition identity and the k -> 0 Laplace limit). */
+void orc_direct_helmholtz(int conv, int nthreads, double k, int ni, const float* xi, const float* yi,
+                          const float* zi, int nj, const float* xj, const float* yj, const float* zj,
+                          const float* qr, const float* qi, double* vr, double* vim);
+
 /* ---- single-box expansion helpers for closed-form checks ---- */
 /* Multipole of point charges about centre c (kernel/G_p2m.cxx), and the
  * local-expansion evaluation at points (kernel/G_l2p.cxx) of given

@@ -63,6 +63,8 @@ def lib():
                                  C.c_int, fp, fp, fp, fp, fp, fp, fp, dp, dp, dp, dp]
         L.orc_direct_yukawa.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, fp, fp, fp, C.c_int, fp, fp, fp, fp,
                                         dp, dp, dp, dp]
+        L.orc_direct_helmholtz.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, fp, fp, fp, C.c_int, fp, fp, fp,
+                                           fp, fp, dp, dp]
         L.orc_p2m_box.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, C.c_double, C.c_double,
                                   C.c_double, dp]
         L.orc_l2p_box.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, C.c_double, C.c_double,
@@ -225,6 +227,21 @@ def direct_yukawa(xi, xj, q, lam, *, conv=REF, nthreads=0):
     lib().orc_direct_yukawa(conv, nthreads, float(lam), ni, *map(_pf, a), nj, *map(_pf, b), _pf(q), _pd(v),
                             _pd(g[0]), _pd(g[1]), _pd(g[2]))
     return v, g.T.copy()
+
+
+def direct_helmholtz(xi, xj, q, k, *, conv=EXACT, nthreads=0):
+    """sum_j q_j exp(i k r) / (4 pi r), complex q: FP64 from float inputs."""
+    xi = np.ascontiguousarray(xi, dtype=np.float32)
+    xj = np.ascontiguousarray(xj, dtype=np.float32)
+    q = np.asarray(q)
+    qr, qi = _f(q.real), _f(q.imag)
+    ni, nj = len(xi), len(xj)
+    a = [_f(xi[:, c]) for c in range(3)]
+    b = [_f(xj[:, c]) for c in range(3)]
+    vr, vi = np.zeros(ni), np.zeros(ni)
+    lib().orc_direct_helmholtz(conv, nthreads, float(k), ni, *map(_pf, a), nj, *map(_pf, b), _pf(qr), _pf(qi),
+                               _pd(vr), _pd(vi))
+    return vr + 1j * vi
 
 
 def precalc(mp, conv=REF):

@@ -185,7 +185,7 @@ def test_operator_rejects_bad_arguments():
     assert F.lib().fmmx_operator(10, 1, 2, 1, 0, 0, 1.0, A.ctypes.data) == 1  # adjacent offset
     assert F.lib().fmmx_operator(10, 1, 2, 4, 0, 0, 1.0, A.ctypes.data) == 1  # beyond [-3,3]
     assert F.lib().fmmx_operator(99, 1, 0, 0, 0, 0, 1.0, A.ctypes.data) == 1  # p out of range
-    assert F.lib().fmmx_operator(10, 1, 7, 0, 0, 0, 1.0, A.ctypes.data) == 1  # bad kind
+    assert F.lib().fmmx_operator(10, 1, 9, 0, 0, 0, 1.0, A.ctypes.data) == 1  # bad kind
     assert "kind" in F.lib().fmmx_last_error().decode()
 
 

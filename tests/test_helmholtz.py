@@ -135,3 +135,135 @@ def test_helmholtz_operator_arguments():
     assert F.lib().fmmx_operator(10, 1, 8, 1, 0, 0, 1.0, A.ctypes.data) == 1  # adjacent offset
     assert F.lib().fmmx_operator(10, 1, 8, 2, 0, 0, 9.0, A.ctypes.data) == 1  # k rb too large
     assert F.lib().fmmx_operator(10, 1, 6, 8, 0, 0, 1.0, A.ctypes.data) == 1  # octant
+
+
+# ---- oracle (no GPU) -------------------------------------------------------------
+from oracle import oracle as O  # noqa: E402
+
+
+def _sphere_quadrature(a, nth=40, nph=80):
+    """Gauss-Legendre x trapezoid nodes on a sphere of radius a, weights
+    summing to 1"""
+    ct, wt = np.polynomial.legendre.leggauss(nth)
+    ph = 2 * np.pi * np.arange(nph) / nph
+    st = np.sqrt(1 - ct ** 2)
+    pts = a * np.stack([np.outer(st, np.cos(ph)), np.outer(st, np.sin(ph)), np.outer(ct, np.ones(nph))], -1)
+    w = np.outer(wt, np.ones(nph)) / (2 * nph)
+    return pts.reshape(-1, 3), w.reshape(-1)
+
+
+@pytest.mark.parametrize("k", [0.7, 4.0])
+def test_oracle_helmholtz_funk_hecke(k):
+    """Sphere mean of exp(ikr)/(4 pi r) over sources on |y| = a seen from
+    |x| < a: j_0(k|x|) exp(i k a) / (4 pi a) (Funk-Hecke with the addition
+    theorem); complex weights rotate the result."""
+    a = 1.3
+    pts, w = _sphere_quadrature(a)
+    x = np.array([[0.2, -0.3, 0.4], [0.0, 0.0, 0.0], [0.5, 0.5, -0.1]], np.float32)
+    q = (w * (0.6 - 0.8j)).astype(np.complex64)
+    v = O.direct_helmholtz(x, pts.astype(np.float32), q, k)
+    r = np.linalg.norm(x.astype(float), axis=1)
+    want = (0.6 - 0.8j) * np.sinc(k * r / np.pi) * np.exp(1j * k * a) / (4 * np.pi * a)
+    assert np.abs(v - want).max() <= 2e-6 * np.abs(want).max()  # float inputs
+
+
+def test_oracle_helmholtz_small_k_is_laplace():
+    """k -> 0: the Laplace direct sums of the real and imaginary charges (the
+    first correction, k sum q / (4 pi), is 1e-8 of the field here)."""
+    x, q = uniform_cube(300, 70)
+    qc = q + 0.5j * q[::-1]
+    v = O.direct_helmholtz(x, x, qc, 1e-7)
+    lr, _ = O.direct(x, x, qc.real.astype(np.float32), conv=O.EXACT)
+    li, _ = O.direct(x, x, qc.imag.astype(np.float32), conv=O.EXACT)
+    assert rel_l2(v.real, lr) < 1e-7 and rel_l2(v.imag, li) < 1e-7
+
+
+# ---- GPU engine ---------------------------------------------------------------
+def _complex_charges(n, seed):
+    rng = np.random.default_rng(seed)
+    return ((rng.random(n) * 2 - 1) + 1j * (rng.random(n) * 2 - 1)).astype(np.complex64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [0.5, 5.0, 15.0])
+@pytest.mark.parametrize("precision,p,tol", [("fp64", 12, 3e-6), ("fp64", 8, 3e-4), ("mixed", 10, 3e-5)])
+def test_helmholtz_vs_direct(k, precision, p, tol):
+    """Complex charges in the unit cube: FMM against the oracle's FP64 direct
+    sum on 500 sampled targets; the GPU direct sum equals the oracle's.
+    Measured: p = 12 FP64 8e-7 / 1.2e-6 / 1.6e-6, p = 8 FP64 3e-5 / 5e-5 /
+    1.4e-4, p = 10 mixed 5e-6 / 7e-6 / 1.2e-5 at k = 0.5 / 5 / 15."""
+    n = 30000
+    x, _ = uniform_cube(n, 72)
+    q = _complex_charges(n, 73)
+    print(f"k={k} precision={precision} p={p}", flush=True)
+    f = F.FMM(p=p, lmax=4, precision=precision, convention="exact", box=(0, 0, 0, 1), k=k)
+    f.set_points(x)
+    v, _ = f.evaluate(q, neq=F.HELMHOLTZ)
+    idx = np.random.default_rng(74).choice(n, 500, replace=False)
+    dv = O.direct_helmholtz(x[idx], x, q, k)
+    err = rel_l2(v[idx], dv)
+    print(f"rel-L2 {err:.3e}", flush=True)
+    assert err < tol
+    gv, _ = f.direct(x[idx], x, q, neq=F.HELMHOLTZ)
+    assert rel_l2(gv, dv) < 1e-12
+    f.close()
+
+
+@pytest.mark.gpu
+def test_helmholtz_small_k_matches_laplace_engine():
+    """k -> 0: the Helmholtz engine on complex charges equals the Laplace
+    engine run on the real and the imaginary parts (FP64, same tree)."""
+    n = 20000
+    x, _ = uniform_cube(n, 75)
+    q = _complex_charges(n, 76)
+    f = F.FMM(p=8, lmax=4, precision="fp64", convention="exact", box=(0, 0, 0, 1), k=1e-7)
+    f.set_points(x)
+    v, _ = f.evaluate(q, neq=F.HELMHOLTZ)
+    a, _ = f.evaluate(q.real.copy(), neq=F.G, gradient=False)
+    b, _ = f.evaluate(q.imag.copy(), neq=F.G, gradient=False)
+    print(f"re {rel_l2(v.real, a):.2e} im {rel_l2(v.imag, b):.2e}", flush=True)
+    assert rel_l2(v.real, a) < 1e-7 and rel_l2(v.imag, b) < 1e-7
+    f.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("mixed", 1e-5)])
+def test_helmholtz_sphere_surface(precision, tol):
+    """The low-frequency configuration at reduced size: 200,000 points on the
+    unit sphere, k = 5 (k D = 10), complex charges, p = 12, against the FP64
+    direct sum on 2,000 targets."""
+    n = 200000
+    rng = np.random.default_rng(77)
+    x = rng.standard_normal((n, 3))
+    x = (x / np.linalg.norm(x, axis=1)[:, None]).astype(np.float32)
+    q = _complex_charges(n, 78)
+    f = F.FMM(p=12, lmax=5, precision=precision, convention="exact", k=5.0)
+    f.set_points(x)
+    v, _ = f.evaluate(q, neq=F.HELMHOLTZ)
+    idx = rng.choice(n, 2000, replace=False)
+    dv = O.direct_helmholtz(x[idx], x, q, 5.0)
+    err = rel_l2(v[idx], dv)
+    print(f"{precision}: rel-L2 {err:.3e}", flush=True)
+    assert err < tol
+    f.close()
+
+
+@pytest.mark.gpu
+def test_helmholtz_argument_errors():
+    x, _ = uniform_cube(1000, 79)
+    q = _complex_charges(1000, 80)
+    f = F.FMM(p=6, lmax=3, precision="fp64", convention="exact", box=(0, 0, 0, 1))
+    f.set_points(x)
+    with pytest.raises(F.FMMError):
+        f.evaluate(q, neq=F.HELMHOLTZ)  # no k
+    with pytest.raises(F.FMMError):
+        f.set_helmholtz(0.0)
+    f.set_helmholtz(30.0)  # k * (level-2 box 0.25) = 7.5 > 5
+    with pytest.raises(F.FMMError):
+        f.evaluate(q, neq=F.HELMHOLTZ)
+    f.close()
+    g = F.FMM(p=7, lmax=3, precision="fp64", convention="exact", box=(0, 0, 0, 1), k=1.0)
+    g.set_points(x)
+    with pytest.raises(F.FMMError):
+        g.evaluate(q, neq=F.HELMHOLTZ)  # odd p
+    g.close()
