@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
+    ap.add_argument("--workload", default="cube", choices=["cube", "helmholtz-sphere", "bem-sphere"],
+                    help="cube: the headline metric; the others time the sphere workloads (one JSON line each)")
     return ap.parse_args()
 
 
@@ -93,6 +95,103 @@ def run_engine(F, torch, x, q, p, lmax, precision, steps, warmup):
     return f, v, g, ms, ms_eval, phases, st
 
 
+def timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, out
+
+
+def sphere_workload(a, F, torch):
+    """The two sphere workloads, each step a full matvec (tree + all phases)
+    with inputs in HBM, mixed and FP64, error against the FP64 direct sum:
+      helmholtz-sphere: N points uniform on the unit sphere, k = 5 (k D = 10),
+        complex charges with parts uniform in [-1, 1], p = 12, 2,000 targets;
+      bem-sphere: the 3 Gauss points of each triangle of an icosphere with
+        81,920 triangles (245,760 points), unit normals, density a random
+        degree <= 8 spherical-harmonic field times the quadrature weights,
+        single layer (G) and double layer (Gnj), p = 10, 1,000 targets."""
+    rng = np.random.default_rng(0)
+    res = {}
+    if a.workload == "helmholtz-sphere":
+        n, p, k, nt = (a.n, 12, 5.0, 2000)
+        xh = rng.standard_normal((n, 3))
+        xh = (xh / np.linalg.norm(xh, axis=1)[:, None]).astype(np.float32)
+        qh = ((2 * rng.random(n) - 1) + 1j * (2 * rng.random(n) - 1)).astype(np.complex64)
+        x = torch.from_numpy(xh).cuda()
+        q = torch.from_numpy(qh).cuda()
+        idx = torch.from_numpy(np.random.default_rng(1).choice(n, nt, replace=False)).cuda()
+        kernels = [("helmholtz", F.HELMHOLTZ, None)]
+        metric = "Helmholtz FMM matvec Mpoints/s, N=1M on the unit sphere, k=5, p=12, complex charges"
+        workload = (f"oscillatory Helmholtz exp(ikr)/(4 pi r) FMM matvec, N={n} uniform on the unit sphere, k=5, "
+                    f"complex q (parts ~U[-1,1]), p=12; step = tree + lists + all phases")
+        lmaxes = [a.lmax] if a.lmax else [5, 6]
+        mk = lambda prec, lm: F.FMM(p=p, lmax=lm, precision=prec, convention="exact", k=k)  # noqa: E731
+        nrm = None
+    else:
+        from old_fmm_bem_amd import bem
+        from scipy.special import sph_harm_y
+
+        m = bem.icosphere(6)
+        xh = m.qx.astype(np.float32)
+        n, p, nt = len(xh), 10, 1000
+        nrmh = (m.qx / np.linalg.norm(m.qx, axis=1)[:, None]).astype(np.float32)
+        th = np.arccos(np.clip(m.qx[:, 2] / np.linalg.norm(m.qx, axis=1), -1, 1))
+        ph = np.arctan2(m.qx[:, 1], m.qx[:, 0])
+        sig = np.zeros(n)
+        for deg in range(9):
+            for o in range(deg + 1):
+                y = sph_harm_y(deg, o, th, ph)
+                sig += rng.standard_normal() * y.real + (rng.standard_normal() * y.imag if o else 0)
+        qh = (m.qw * sig).astype(np.float32)
+        x = torch.from_numpy(xh).cuda()
+        q = torch.from_numpy(qh).cuda()
+        nrm = torch.from_numpy(nrmh).cuda()
+        idx = torch.from_numpy(np.random.default_rng(1).choice(n, nt, replace=False)).cuda()
+        kernels = [("single_layer", F.G, None), ("double_layer", F.GNJ, None)]
+        metric = "BEM single+double layer matvec Mpoints/s, 245,760 Gauss points on an icosphere, p=10"
+        workload = ("Laplace G and Gnj matvecs on the 3 Gauss points of an 81,920-triangle icosphere (245,760 points,"
+                    " unit normals), density = degree<=8 harmonic field x weights, p=10; step = tree + lists + "
+                    "one G and one Gnj evaluation")
+        lmaxes = [a.lmax] if a.lmax else [4, 5]
+        mk = lambda prec, lm: F.FMM(p=p, lmax=lm, precision=prec, convention="exact")  # noqa: E731
+    for prec in ("mixed", "fp64"):
+        best = None
+        for lm in lmaxes:
+            f = mk(prec, lm)
+
+            def step():
+                f.set_points(x, ni_normals=nrm, nj_normals=nrm)
+                return [f.evaluate(q, neq=kk, gradient=False)[0] for _, kk, _ in kernels]
+
+            ms, outs = timed(torch, step, max(2, a.steps // 2), 1)
+            if best is None or ms < best[0]:
+                if best is not None:
+                    best[2].close()
+                best = (ms, lm, f, outs)
+            else:
+                f.close()
+        ms, lm, f, outs = best
+        errs = {}
+        for (name, kk, _), v in zip(kernels, outs):
+            xs = x[idx]
+            dv, _ = f.direct(xs, x, q, neq=kk, ni_normals=None if nrm is None else nrm[idx], nj_normals=nrm)
+            errs[name] = float(torch.linalg.norm(v[idx] - dv) / torch.linalg.norm(dv))
+        res[prec] = {"ms_per_step": ms, "value": n / (ms * 1e3), "lmax": lm, "rel_l2_err": errs}
+        f.close()
+    out = {"metric": metric, "value": res["mixed"]["value"], "unit": "Mpoints/s", "n_gpus": 1,
+           "steps": max(2, a.steps // 2), "warmup": 1, "ms_per_step": res["mixed"]["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic", "config": {"workload": workload, "n": n, "p": p},
+           "mixed": res["mixed"], "fp64": res["fp64"],
+           "speedup_vs_fp64": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -104,6 +203,9 @@ def main():
     import old_fmm_bem_amd as F
 
     assert torch.cuda.is_available(), "needs a HIP device"
+    if a.workload != "cube":
+        sphere_workload(a, F, torch)
+        return
     rng = np.random.default_rng(0)
     xh = rng.random((a.n, 3), dtype=np.float32)
     qh = (2 * rng.random(a.n, dtype=np.float32) - 1).astype(np.float32)
