@@ -136,6 +136,15 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
  * operators are built at the first Yukawa evaluate after a change. */
 int fmmx_set_yukawa(fmmx_handle* h, double lambda);
 
+/* Independent problems in one tree: with level > 0 the points in different
+ * boxes of that level (of the root cube) do not interact at all, near or
+ * far; each box is its own problem and all of them run in the same launches.
+ * Put each problem in its own box of the level (e.g. surface s of a batch in
+ * cell s of the 2^level grid).  level 0 turns it off.  No reference
+ * counterpart: for callers with many small, uneven problems (batched BEM
+ * solves).  Takes effect at the next fmmx_set_points (required). */
+int fmmx_set_groups(fmmx_handle* h, int level);
+
 /* Wavenumber k > 0 of FMMX_HELMHOLTZ.  No reference counterpart (the
  * reference has no oscillatory kernel).  Its translation operators are built
  * at the first Helmholtz evaluate after a change. */

@@ -62,6 +62,7 @@ _SYMBOLS = {
     "fmmx_direct": (C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_int] + [C.c_void_p] * 11
                     + [C.c_int]),
     "fmmx_set_helmholtz": (C.c_int, [C.c_void_p, C.c_double]),
+    "fmmx_set_groups": (C.c_int, [C.c_void_p, C.c_int]),
     "fmmx_set_yukawa": (C.c_int, [C.c_void_p, C.c_double]),
     "fmmx_set_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
     "fmmx_sort": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -156,6 +157,12 @@ class FMM:
             self.set_yukawa(lam)
         if k is not None:
             self.set_helmholtz(k)
+
+    def set_groups(self, level):
+        """Points in different boxes of this level are independent problems
+        (0: off).  Call set_points afterwards."""
+        _check(lib().fmmx_set_groups(self.h, int(level)))
+        return self
 
     def set_helmholtz(self, k):
         """Wavenumber of the HELMHOLTZ kernel exp(i k r) / (4 pi r)."""

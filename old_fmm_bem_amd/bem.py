@@ -392,3 +392,194 @@ def solvation_energy_6node(mesh: Mesh, xq, q, e1=4.0, e2=20.0, p=10, precision="
     f.close()
     per = 0.5 * KCAL * q.astype(np.float64) * vr
     return {"energy": float(per.sum()), "per_charge": per, "si": si, "history": hist, "iterations": it}
+
+
+# ---- batched solves: many independent surfaces in one FMM -------------------------
+def ellipsoid_surface(npanel, axes, bumps=(), seed=0):
+    """Triangulated ellipsoid with about npanel flat panels: the convex hull of
+    a Fibonacci lattice on the unit sphere (2 n - 4 triangles for n points),
+    mapped to semi-axes `axes` and scaled radially by 1 + sum of Gaussian
+    bumps (centre on the unit sphere, amplitude, width).  Faces are listed
+    counter-clockwise from outside, 1-based, as the reference's .face files.
+    Returns (vertices float32 (n, 3), faces (m, 3))."""
+    from scipy.spatial import ConvexHull
+
+    n = max(8, npanel // 2 + 2)
+    k = np.arange(n) + 0.5
+    z = 1 - 2 * k / n
+    r = np.sqrt(1 - z * z)
+    ph = np.pi * (1 + 5 ** 0.5) * k
+    u = np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+    tri = ConvexHull(u).simplices.copy()
+    scale = np.ones(n)
+    for c, amp, w in bumps:
+        scale += amp * np.exp(-np.sum((u - np.asarray(c)) ** 2, 1) / (2 * w * w))
+    v = u * np.asarray(axes, float) * scale[:, None]
+    a, b, c = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
+    out = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) > 0
+    tri[~out] = tri[~out][:, [0, 2, 1]]
+    return v.astype(np.float32), tri + 1
+
+
+class BatchedInducedCharge:
+    """The operator of bio/main.cxx:27-53 for many independent surfaces at
+    once: surface s is moved to cell s of the 2^G grid of the root cube and
+    the FMM's group level G keeps the cells apart, so one evaluation (one set
+    of launches over all panels) applies every surface's operator.  Vectors
+    are concatenated in surface order; seg[s]:seg[s+1] is surface s."""
+
+    def __init__(self, surfaces, e1=4.0, e2=20.0, p=8, precision="mixed", lmax=0, extra_levels=4, device=None):
+        import torch
+
+        from .formats import panels
+
+        ns = len(surfaces)
+        G = max(1, int(np.ceil(np.log2(max(2, ns)) / 3)))
+        cells = 1 << G
+        parts, ext = [], 0.0
+        for vert, face in surfaces:
+            c, area, nrm = panels(vert, face)
+            parts.append((c, area, nrm))
+            ext = max(ext, float(np.abs(np.asarray(vert) - np.asarray(vert).mean(0)).max()))
+        h = 2.2 * ext  # cell size: every surface strictly inside its cell
+        cs, ar, nr, seg = [], [], [], [0]
+        for s, (c, area, nrm) in enumerate(parts):
+            i, j, k = s % cells, (s // cells) % cells, s // (cells * cells)
+            centre = (np.array([i, j, k]) + 0.5) * h
+            cs.append((c - c.mean(0) + centre).astype(np.float32))
+            ar.append(area)
+            nr.append(nrm)
+            seg.append(seg[-1] + len(c))
+        self.G, self.seg = G, np.array(seg)
+        self.e1, self.e2 = float(e1), float(e2)
+        self.dev = torch.device(device or "cuda")  # the panel vectors live on the GPU
+        c = np.concatenate(cs)
+        self.n = len(c)
+        self.area = torch.from_numpy(np.concatenate(ar).astype(np.float64)).to(self.dev)
+        self.seg_id = torch.from_numpy(np.repeat(np.arange(ns), np.diff(self.seg))).to(self.dev)
+        self.nseg = ns
+        self.f = FMM(p=p, lmax=lmax or G + extra_levels, precision=precision, convention="exact",
+                     box=(0.0, 0.0, 0.0, float(h * cells)))
+        self.f.set_groups(G)
+        self._c = torch.from_numpy(c).to(self.dev)
+        self._n = torch.from_numpy(np.concatenate(nr).astype(np.float32)).to(self.dev)
+        self.f.set_points(self._c, ni_normals=self._n)
+        self.diag = (self.e1 + self.e2) / (2 * self.e1 * (self.e1 - self.e2))
+        self.centroids = c
+
+    def matvec(self, x):
+        v, _ = self.f.evaluate((x * self.area / self.e1).float(), neq=GNI)
+        return v + self.diag * x
+
+    def precondition(self, x):
+        return x / (self.e1 + self.e2) * (2 * self.e1 * (self.e1 - self.e2))
+
+    def close(self):
+        self.f.close()
+
+
+def gmres_batched(matvec, b, seg_id, nseg, precond=None, rtol=1e-6, restart=30, maxiter=300):
+    """Independent right-preconditioned restarted GMRES for every segment of
+    a block-diagonal system, in lockstep: one matvec per iteration serves all
+    segments; inner products, Givens rotations and updates are per segment
+    (FP64, modified Gram-Schmidt with one reorthogonalisation pass).  b and
+    the iterates are torch tensors (device-resident for a GPU matvec).
+    Returns (x, per-segment relative residuals, iterations)."""
+    import torch
+
+    dev, dt = b.device, torch.float64
+    M = precond or (lambda v: v)
+
+    def sdot(a, c):
+        return torch.zeros(nseg, dtype=dt, device=dev).index_add_(0, seg_id, a * c)
+
+    nb = torch.sqrt(sdot(b, b))
+    nb = torch.where(nb > 0, nb, torch.ones_like(nb))
+    x = torch.zeros_like(b)
+    it = 0
+    r = b - matvec(x)
+    beta = torch.sqrt(sdot(r, r))
+    res = beta / nb
+    while bool((res > rtol).any()) and it < maxiter:
+        m = min(restart, maxiter - it)
+        V = [r / torch.where(beta > 0, beta, torch.ones_like(beta))[seg_id]]
+        H = torch.zeros(nseg, m + 1, m, dtype=dt, device=dev)
+        g = torch.zeros(nseg, m + 1, dtype=dt, device=dev)
+        g[:, 0] = beta
+        cs = torch.zeros(nseg, m, dtype=dt, device=dev)
+        sn = torch.zeros(nseg, m, dtype=dt, device=dev)
+        kk = 0
+        for k in range(m):
+            w = matvec(M(V[k]))
+            for _ in range(2):
+                for i in range(k + 1):
+                    h = sdot(V[i], w)
+                    H[:, i, k] += h
+                    w = w - h[seg_id] * V[i]
+            hn = torch.sqrt(sdot(w, w))
+            H[:, k + 1, k] = hn
+            V.append(w / torch.where(hn > 0, hn, torch.ones_like(hn))[seg_id])
+            for i in range(k):
+                t = cs[:, i] * H[:, i, k] + sn[:, i] * H[:, i + 1, k]
+                H[:, i + 1, k] = -sn[:, i] * H[:, i, k] + cs[:, i] * H[:, i + 1, k]
+                H[:, i, k] = t
+            d = torch.sqrt(H[:, k, k] ** 2 + H[:, k + 1, k] ** 2)
+            d1 = torch.where(d > 0, d, torch.ones_like(d))
+            cs[:, k] = torch.where(d > 0, H[:, k, k] / d1, torch.ones_like(d))
+            sn[:, k] = torch.where(d > 0, H[:, k + 1, k] / d1, torch.zeros_like(d))
+            H[:, k, k] = d
+            H[:, k + 1, k] = 0
+            g[:, k + 1] = -sn[:, k] * g[:, k]
+            g[:, k] = cs[:, k] * g[:, k]
+            it += 1
+            kk = k + 1
+            if bool((g[:, k + 1].abs() / nb <= rtol).all()):
+                break
+        R = H[:, :kk, :kk]
+        diag = torch.diagonal(R, dim1=1, dim2=2)
+        R = R + torch.diag_embed(torch.where(diag == 0, torch.ones_like(diag), torch.zeros_like(diag)))
+        y = torch.linalg.solve_triangular(R, g[:, :kk, None], upper=True)[:, :, 0]
+        upd = torch.zeros_like(b)
+        for i in range(kk):
+            upd = upd + y[:, i][seg_id] * V[i]
+        x = x + M(upd)
+        r = b - matvec(x)
+        beta = torch.sqrt(sdot(r, r))
+        res = beta / nb
+    return x, res, it
+
+
+def batched_solvation(surfaces, charges, e1=4.0, e2=20.0, p=8, precision="mixed", rtol=1e-6, restart=30,
+                      maxiter=300, extra_levels=4):
+    """Solve the induced-charge equation of bio/main.cxx for every surface of
+    a batch at once (each with its own point charges inside) and return the
+    per-surface solvation energies (kcal/mol), the solution and the GMRES
+    history.  charges[s] = (positions (m, 3), charges (m,)) in the
+    surface's own coordinates."""
+    import torch
+
+    op = BatchedInducedCharge(surfaces, e1, e2, p=p, precision=precision, extra_levels=extra_levels)
+    # right-hand side: -4 pi Gni(charges -> centroids) / e1 per surface, on
+    # the oracle-free GPU direct sum of the engine
+    f = FMM(p=p, precision="fp64", convention="exact")
+    un = np.zeros(op.n)
+    from .formats import panels
+
+    energies = []
+    for s, ((vert, face), (xq, q)) in enumerate(zip(surfaces, charges)):
+        c, area, nrm = panels(vert, face)
+        v, _ = f.direct(c, np.asarray(xq, np.float32), np.asarray(q, np.float32), neq=GNI, ni_normals=nrm)
+        un[op.seg[s]:op.seg[s + 1]] = -4 * np.pi * v / e1
+    b = torch.from_numpy(un).to(op.dev)
+    x, res, it = gmres_batched(op.matvec, b, op.seg_id, op.nseg, precond=op.precondition, rtol=rtol,
+                               restart=restart, maxiter=maxiter)
+    si = x.cpu().numpy()
+    for s, ((vert, face), (xq, q)) in enumerate(zip(surfaces, charges)):
+        c, area, nrm = panels(vert, face)
+        sl = slice(op.seg[s], op.seg[s + 1])
+        vr, _ = f.direct(np.asarray(xq, np.float32), c, (si[sl] * area / e1).astype(np.float32), neq=G)
+        energies.append(float(0.5 * KCAL * np.sum(np.asarray(q, np.float64) * vr)))
+    f.close()
+    op.close()
+    return {"energies": np.array(energies), "si": si, "residuals": res.cpu().numpy(), "iterations": it,
+            "seg": op.seg}

@@ -413,6 +413,13 @@ int Engine::set_yukawa(double lambda) {
   return FMMX_OK;
 }
 
+int Engine::set_groups(int level) {
+  if (level < 0 || level > MAXL) return fail(FMMX_EINVAL, "group level out of range [0, 8]");
+  group_level = level;
+  have_points = false;  // the near lists depend on it
+  return FMMX_OK;
+}
+
 int Engine::set_helmholtz(double k) {
   if (!(k > 0) || !std::isfinite(k)) return fail(FMMX_EINVAL, "k must be positive and finite");
   hk = k;
@@ -643,7 +650,8 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     for (int l = lmax - 1; far && l >= 2; l--)
       if ((rc = run_m2m(l))) return rc;
     if ((rc = mark())) return rc;  // ev5: after M2M
-    if (far && overlap && !hh && lmax >= 3 && m2l_splits(lmax) > 1) {
+    const int l0 = first_m2l_level();
+    if (far && overlap && !hh && lmax > l0 && m2l_splits(lmax) > 1) {
       // levels 2..lmax-1 (M2L, L2L) on the third stream while the leaf M2L
       // computes its partial sums; then the leaf L2L and the reduction that
       // adds the partials (the serialized order below, bit for bit)
@@ -652,8 +660,8 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
       hipStream_t main = stream;
       stream = aux2;
       part_buf = &m2l_part2;
-      rc = run_m2l(2, true);
-      for (int l = 3; !rc && l < lmax; l++) {
+      rc = run_m2l(l0, true);
+      for (int l = l0 + 1; !rc && l < lmax; l++) {
         if (!(rc = run_l2l(l))) rc = run_m2l(l, false);
       }
       stream = main;
@@ -665,8 +673,8 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
       if ((rc = run_l2l(lmax))) return rc;
       if ((rc = run_m2l(lmax, false, 2))) return rc;
     } else {
-      if (far && (rc = run_m2l(2, true))) return rc;
-      for (int l = 3; far && l <= lmax; l++) {
+      if (far && (rc = run_m2l(l0, true))) return rc;
+      for (int l = l0 + 1; far && l <= lmax; l++) {
         if ((rc = run_l2l(l))) return rc;
         if ((rc = run_m2l(l, false))) return rc;
       }
@@ -913,6 +921,13 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
   FMMX_GUARD({
     if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
     return h->e.set_domain(lmax, xmin, ymin, zmin, rd);
+  })
+}
+
+int fmmx_set_groups(fmmx_handle* h, int level) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.set_groups(level);
   })
 }
 

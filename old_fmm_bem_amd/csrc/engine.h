@@ -2,6 +2,7 @@
 // that operate on it.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <memory>
 
@@ -135,6 +136,13 @@ struct Engine {
   float hops_rd = -1;
   double hops_k = -1;
   int parts() const { return hh ? 2 : 1; }
+  // Groups: with group_level > 0 the points in different boxes of that level
+  // are independent problems (no near or far interaction between them); the
+  // far field then starts at the group level.
+  int group_level = 0;
+  int group_shift(int lev) const { return group_level > 0 ? lev - group_level : -1; }
+  int first_m2l_level() const { return std::max(2, group_level); }
+  int set_groups(int level);
   // stand-alone Morton sort (fmmx_sort): staging, keys, permutation
   DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm;
   int last_neq = -1;

@@ -633,7 +633,8 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
                                              const int* __restrict__ dense_j, const float* __restrict__ M,
                                              float* __restrict__ L, float* __restrict__ part,
                                              const float* __restrict__ opF, const int4* __restrict__ offtab,
-                                             int init, int region, unsigned long long* __restrict__ npairs) {
+                                             int init, int region, int gsh,
+                                             unsigned long long* __restrict__ npairs) {
   using T = float;
   using ML = MLayout<P>;
   constexpr int P2 = P * P, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, FR = KS * MT * 64;
@@ -732,7 +733,8 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
       int s_ = -1;                                                                            \
       if (tord[n] >= 0) {                                                                     \
         const int jx = tx[n] - off_.x, jy = ty[n] - off_.y, jz = tz[n] - off_.z;              \
-        if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g)                     \
+        if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g &&                    \
+            (gsh < 0 || !((jx ^ tx[n]) >> gsh | (jy ^ ty[n]) >> gsh | (jz ^ tz[n]) >> gsh)))   \
           s_ = dense_j[morton_encode(jx, jy, jz, lev)];                                       \
       }                                                                                       \
       if (s_ >= 0 && kq == 0) mypairs++;                                                      \
@@ -864,7 +866,7 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
                                                  const uint32_t* __restrict__ ord, const uint32_t* __restrict__ ikey,
                                                  const int* __restrict__ dense_j, const T* __restrict__ M,
                                                  T* __restrict__ L, T* __restrict__ part, const T* __restrict__ opT,
-                                                 const int4* __restrict__ offtab, int init,
+                                                 const int4* __restrict__ offtab, int init, int gsh,
                                                  unsigned long long* __restrict__ npairs) {
   using ML = MLayout<P>;
   constexpr int P2 = P * P, D = PARTS * P2, RPT = (D + 15) / 16;
@@ -923,8 +925,11 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
     if (tid < M2L_TN) {
       int s = -1;
       if (tid < ntile) {
-        const int jx = tpos[tid][0] - off.x, jy = tpos[tid][1] - off.y, jz = tpos[tid][2] - off.z;
-        if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g) s = dense_j[morton_encode(jx, jy, jz, lev)];
+        const int ix = tpos[tid][0], iy = tpos[tid][1], iz = tpos[tid][2];
+        const int jx = ix - off.x, jy = iy - off.y, jz = iz - off.z;
+        if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g &&
+            (gsh < 0 || !((jx ^ ix) >> gsh | (jy ^ iy) >> gsh | (jz ^ iz) >> gsh)))
+          s = dense_j[morton_encode(jx, jy, jz, lev)];
       }
       src[tid] = s;
       if (s >= 0) mypairs++;
@@ -1504,6 +1509,7 @@ int Engine::run_m2l(int lev, bool init, int phase) {
     if (rc) return rc;
   }
   int rc2 = FMMX_OK;
+  const int gsh = group_shift(lev);
   if (hh) {
     // split complex expansions: D = 2 p^2, k-major operators per level, on the
     // vector units in both precisions
@@ -1514,12 +1520,12 @@ int Engine::run_m2l(int lev, bool init, int phase) {
         k_m2l_vec<P, double, 2><<<tiles * nsplit, 256, 0, stream>>>(
             lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
             sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), part_buf->as<double>(),
-            hop_m2l.as<double>() + hoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev);
+            hop_m2l.as<double>() + hoff, offtab.as<int4>(), init ? 1 : 0, gsh, pair_ctr.as<unsigned long long>() + 2 + lev);
       else
         k_m2l_vec<P, float, 2><<<tiles * nsplit, 256, 0, stream>>>(
             lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
             sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), part_buf->as<float>(),
-            hop_m2l.as<float>() + hoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev);
+            hop_m2l.as<float>() + hoff, offtab.as<int4>(), init ? 1 : 0, gsh, pair_ctr.as<unsigned long long>() + 2 + lev);
       if (nsplit > 1 && phase == 0) {
         if (fp64)
           k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
@@ -1540,7 +1546,7 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       k_m2l_vec<P, double><<<tiles * nsplit, 256, 0, stream>>>(
           lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
           sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), part_buf->as<double>(),
-          opm2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0, pair_ctr.as<unsigned long long>() + 2 + lev);
+          opm2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0, gsh, pair_ctr.as<unsigned long long>() + 2 + lev);
       if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
       if (nsplit > 1 && phase == 0)
         k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
@@ -1567,7 +1573,7 @@ int Engine::run_m2l(int lev, bool init, int phase) {
   k_m2l<P, NT_, PA_, W_><<<nwg, 64 * W_, 0, stream>>>(                                                  \
       lev, nb, ns2, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),   \
       sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), part_buf->as<float>(),     \
-      opm2l.as<float>() + opoff, (use_canon ? offtab_c : offtab).as<int4>(), init ? 1 : 0, region,      \
+      opm2l.as<float>() + opoff, (use_canon ? offtab_c : offtab).as<int4>(), init ? 1 : 0, region, gsh, \
       pair_ctr.as<unsigned long long>() + 2 + lev)
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));

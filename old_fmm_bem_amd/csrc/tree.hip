@@ -113,7 +113,7 @@ __global__ void k_dense(int ub, const int* __restrict__ cnt, const uint32_t* __r
 __global__ void k_near_list(int nbi, const uint32_t* __restrict__ bkey, int lev, const int* __restrict__ dense_j,
                             const int* __restrict__ ifirst, const int* __restrict__ ilast,
                             const int* __restrict__ jfirst, const int* __restrict__ jlast, int* __restrict__ nij,
-                            int* __restrict__ neij, unsigned long long* __restrict__ pairs) {
+                            int* __restrict__ neij, unsigned long long* __restrict__ pairs, int gsh) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbi) return;
   int ix, iy, iz, g = 1 << lev, c = 0;
@@ -122,6 +122,8 @@ __global__ void k_near_list(int nbi, const uint32_t* __restrict__ bkey, int lev,
   for (int jx = max(ix - 1, 0); jx <= min(ix + 1, g - 1); jx++)
     for (int jy = max(iy - 1, 0); jy <= min(iy + 1, g - 1); jy++)
       for (int jz = max(iz - 1, 0); jz <= min(iz + 1, g - 1); jz++) {
+        // groups: boxes in different cells of the group level do not interact
+        if (gsh >= 0 && ((jx ^ ix) >> gsh | (jy ^ iy) >> gsh | (jz ^ iz) >> gsh)) continue;
         int jj = dense_j[morton_encode(jx, jy, jz, lev)];
         if (jj >= 0) {
           neij[b * NNEAR + c++] = jj;
@@ -345,7 +347,7 @@ int Engine::build_lists() {
     k_near_list<<<nblk(nbi), TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
                                                si.bfirst[lmax].as<int>(), si.blast.as<int>(),
                                                sj.bfirst[lmax].as<int>(), sj.blast.as<int>(), nij.as<int>(),
-                                               neij.as<int>(), ctr.as<unsigned long long>());
+                                               neij.as<int>(), ctr.as<unsigned long long>(), group_shift(lmax));
   else if (nbi > 0)
     k_fill_int<<<nblk(nbi), TPB, 0, stream>>>(nbi, 0, nij.as<int>());
   return FMMX_OK;
