@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="cube", choices=["cube", "helmholtz-sphere", "bem-sphere"],
+    ap.add_argument("--workload", default="cube", choices=["cube", "helmholtz-sphere", "bem-sphere", "bem-batch"],
                     help="cube: the headline metric; the others time the sphere workloads (one JSON line each)")
     return ap.parse_args()
 
@@ -192,6 +192,53 @@ def sphere_workload(a, F, torch):
     print(json.dumps(out))
 
 
+def bem_batch_workload(a, F, torch):
+    """Batched BEM solves: 256 ellipsoids (semi-axes uniform in [0.5, 2], 20
+    Gaussian bumps of amplitude <= 0.1 each), panel counts log-uniform from
+    2,000 to 200,000, three unit charges inside each; the induced-charge
+    equation of every surface solved by GMRES to rtol 1e-6 with p = 8, all
+    surfaces in one tree (one FMM launch sequence per iteration).  Timed: the
+    whole batched solve (tree, right-hand sides, GMRES, energies) and the
+    matvec alone."""
+    from old_fmm_bem_amd import bem
+
+    rng = np.random.default_rng(0)
+    ns = 256 if a.n == 1_000_000 else max(2, a.n)
+    surfaces, charges = [], []
+    for s in range(ns):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(200000))))
+        axes = rng.uniform(0.5, 2.0, 3)
+        bumps = [(u / np.linalg.norm(u), rng.uniform(0, 0.1), rng.uniform(0.15, 0.4))
+                 for u in rng.standard_normal((20, 3))]
+        surfaces.append(bem.ellipsoid_surface(n, axes, bumps, seed=s))
+        charges.append(((rng.uniform(-0.3, 0.3, (3, 3)) * axes).astype(np.float32), np.ones(3, np.float32)))
+    npan = sum(len(f) for _, f in surfaces)
+    print(f"bem-batch: {ns} surfaces, {npan} panels generated", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = bem.batched_solvation(surfaces, charges, p=8, precision="mixed", rtol=1e-6, extra_levels=5)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    print(f"bem-batch: solved in {t_solve:.1f} s, {r['iterations']} iterations", file=sys.stderr, flush=True)
+    # the matvec alone on the same batch
+    scale = r["scale"]
+    sc = [((np.asarray(v, np.float64) * s_).astype(np.float32), f) for (v, f), s_ in zip(surfaces, scale)]
+    op = bem.BatchedInducedCharge(sc, p=8, precision="mixed", extra_levels=5)
+    x = torch.ones(op.n, dtype=torch.float64, device=op.dev)
+    ms_mv, _ = timed(torch, lambda: op.matvec(x), 5, 2)
+    op.close()
+    out = {"metric": "batched BEM solve: 256 surfaces, ~10M panels total, GMRES rtol 1e-6, p=8 (panels/s per matvec)",
+           "value": npan / (ms_mv * 1e3), "unit": "Mpanels/s", "n_gpus": 1, "steps": 5, "warmup": 2,
+           "ms_per_step": ms_mv, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{ns} ellipsoids with 20 bumps, {npan} panels (log-uniform 2k..200k per "
+                                  "surface), induced-charge BEM, p=8, mixed precision; step = one batched matvec",
+                      "surfaces": ns, "panels": npan},
+           "solve_s": t_solve, "gmres_iterations": r["iterations"], "max_residual": float(r["residuals"].max()),
+           "energy_range_kcal": [float(r["energies"].min()), float(r["energies"].max())]}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -203,6 +250,9 @@ def main():
     import old_fmm_bem_amd as F
 
     assert torch.cuda.is_available(), "needs a HIP device"
+    if a.workload == "bem-batch":
+        bem_batch_workload(a, F, torch)
+        return
     if a.workload != "cube":
         sphere_workload(a, F, torch)
         return

@@ -103,6 +103,24 @@ def lib():
     return _LIB
 
 
+_TORCH_READY = False
+
+
+def _torch_first():
+    """PyTorch's ROCm wheel carries its own HIP runtime.  When this library's
+    runtime (the system one) opens the GPU first, torch's then reports no
+    device; the other order works for both.  So torch, when installed, gets
+    the device before the first handle is created."""
+    global _TORCH_READY
+    if not _TORCH_READY:
+        _TORCH_READY = True
+        try:
+            import torch
+        except ImportError:
+            return
+        torch.cuda.is_available()
+
+
 def _check(rc):
     if rc != 0:
         raise FMMError(f"libfmmx error {rc}: {lib().fmmx_last_error().decode()}")
@@ -149,6 +167,7 @@ class FMM:
         self.p, self.lmax_cfg = p, lmax
         self.precision, self.convention = _PREC[precision], _CONV[convention]
         h = C.c_void_p()
+        _torch_first()
         _check(lib().fmmx_create(C.byref(cfg), device, stream, C.byref(h)))
         self.h = h
         self.ni = self.nj = 0

@@ -550,14 +550,27 @@ def gmres_batched(matvec, b, seg_id, nseg, precond=None, rtol=1e-6, restart=30, 
 
 
 def batched_solvation(surfaces, charges, e1=4.0, e2=20.0, p=8, precision="mixed", rtol=1e-6, restart=30,
-                      maxiter=300, extra_levels=4):
+                      maxiter=300, extra_levels=4, normalize=True):
     """Solve the induced-charge equation of bio/main.cxx for every surface of
     a batch at once (each with its own point charges inside) and return the
     per-surface solvation energies (kcal/mol), the solution and the GMRES
     history.  charges[s] = (positions (m, 3), charges (m,)) in the
-    surface's own coordinates."""
+    surface's own coordinates.
+
+    normalize: scale every surface (and its charges) by a_s to a common
+    size, so that each fills its cell of the shared tree.  The operator
+    (area times the normal-derivative kernel) is scale-free, the right-hand
+    side scales as 1/a^2 and the reaction potential as 1/a, so the energy of
+    the scaled problem times a_s is the surface's energy; the returned si is
+    the scaled problem's."""
     import torch
 
+    scale = np.ones(len(surfaces))
+    if normalize:
+        ext = [float(np.abs(np.asarray(v) - np.asarray(v).mean(0)).max()) for v, _ in surfaces]
+        scale = np.max(ext) / np.array(ext)
+        surfaces = [((np.asarray(v, np.float64) * a).astype(np.float32), f) for (v, f), a in zip(surfaces, scale)]
+        charges = [((np.asarray(x, np.float64) * a).astype(np.float32), q) for (x, q), a in zip(charges, scale)]
     op = BatchedInducedCharge(surfaces, e1, e2, p=p, precision=precision, extra_levels=extra_levels)
     # right-hand side: -4 pi Gni(charges -> centroids) / e1 per surface, on
     # the oracle-free GPU direct sum of the engine
@@ -578,8 +591,8 @@ def batched_solvation(surfaces, charges, e1=4.0, e2=20.0, p=8, precision="mixed"
         c, area, nrm = panels(vert, face)
         sl = slice(op.seg[s], op.seg[s + 1])
         vr, _ = f.direct(np.asarray(xq, np.float32), c, (si[sl] * area / e1).astype(np.float32), neq=G)
-        energies.append(float(0.5 * KCAL * np.sum(np.asarray(q, np.float64) * vr)))
+        energies.append(float(0.5 * KCAL * np.sum(np.asarray(q, np.float64) * vr)) * scale[s])
     f.close()
     op.close()
     return {"energies": np.array(energies), "si": si, "residuals": res.cpu().numpy(), "iterations": it,
-            "seg": op.seg}
+            "seg": op.seg, "scale": scale}
