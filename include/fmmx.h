@@ -136,6 +136,18 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
  * operators are built at the first Yukawa evaluate after a change. */
 int fmmx_set_yukawa(fmmx_handle* h, double lambda);
 
+/* One evaluation captured as a HIP graph: fmmx_graph_capture runs
+ * fmmx_evaluate(h, neq, qj, vi, gxi, gyi, gzi, FMMX_DEVICE_PTRS) once
+ * eagerly (sizing buffers, uploading operators), then records the same call
+ * on the handle's stream; fmmx_graph_launch replays it asynchronously on
+ * that stream, reading qj and writing vi (and g) again: write new strengths
+ * into qj between replays.  Device pointers only; per-phase timing must be
+ * off.  fmmx_set_points, fmmx_set_groups and a new capture drop the graph.
+ * No reference counterpart: for iterative solvers that repeat one matvec. */
+int fmmx_graph_capture(fmmx_handle* h, int neq, const float* qj, double* vi, double* gxi, double* gyi,
+                       double* gzi);
+int fmmx_graph_launch(fmmx_handle* h);
+
 /* Independent problems in one tree: with level > 0 the points in different
  * boxes of that level (of the root cube) do not interact at all, near or
  * far; each box is its own problem and all of them run in the same launches.

@@ -63,6 +63,8 @@ _SYMBOLS = {
                     + [C.c_int]),
     "fmmx_set_helmholtz": (C.c_int, [C.c_void_p, C.c_double]),
     "fmmx_set_groups": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmmx_graph_capture": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5),
+    "fmmx_graph_launch": (C.c_int, [C.c_void_p]),
     "fmmx_set_yukawa": (C.c_int, [C.c_void_p, C.c_double]),
     "fmmx_set_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
     "fmmx_sort": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -176,6 +178,26 @@ class FMM:
             self.set_yukawa(lam)
         if k is not None:
             self.set_helmholtz(k)
+
+    def capture(self, q, neq=G, gradient=True):
+        """Capture one evaluation on device tensors as a HIP graph; returns the
+        output tensors (v, g) that every replay() rewrites from the current
+        contents of q (a float32 CUDA tensor the caller updates in place)."""
+        import torch
+
+        if not self._device:
+            raise FMMError("capture needs device tensors (set_points with CUDA tensors)")
+        want_g = gradient and neq in (G, YUKAWA)
+        self._gq = q
+        self._gv = torch.empty(self.ni, dtype=torch.float64, device=q.device)
+        self._gg = torch.empty((3, self.ni), dtype=torch.float64, device=q.device) if want_g else None
+        gp = [self._gg[k].data_ptr() for k in range(3)] if want_g else [None] * 3
+        _check(lib().fmmx_graph_capture(self.h, neq, q.data_ptr(), self._gv.data_ptr(), *gp))
+        return self._gv, (self._gg.T if want_g else None)
+
+    def replay(self):
+        """Replay the captured evaluation (asynchronous on the handle's stream)."""
+        _check(lib().fmmx_graph_launch(self.h))
 
     def set_groups(self, level):
         """Points in different boxes of this level are independent problems

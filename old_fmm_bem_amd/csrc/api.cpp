@@ -327,7 +327,71 @@ int upload(DevBuf& dst, const std::vector<double>& src, hipStream_t s) {
 
 }  // namespace
 
+void Engine::drop_graph() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  gexec = nullptr;
+  graph = nullptr;
+}
+
+int Engine::graph_capture(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi) {
+  if (per_phase) return fail(FMMX_ESTATE, "graph capture needs per-phase timing off (fmmx_set_timing 0)");
+  drop_graph();
+  // eager first: sizes every buffer and uploads the operators, so that the
+  // captured evaluation allocates and copies nothing
+  int rc = evaluate(neq, qj, vi, gxi, gyi, gzi, FMMX_DEVICE_PTRS);
+  if (rc) return rc;
+  FMMX_HIP(hipStreamSynchronize(stream));
+  // capture on a stream of the handle's own (the caller's stream may be the
+  // null stream, which cannot be captured); replays are ordered with the
+  // caller's stream by events
+  if (!gstream) {
+    FMMX_HIP(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
+    FMMX_HIP(hipEventCreateWithFlags(&gev[0], hipEventDisableTiming));
+    FMMX_HIP(hipEventCreateWithFlags(&gev[1], hipEventDisableTiming));
+  }
+  hipStream_t caller = stream;
+  stream = gstream;
+  hipError_t eb = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+  if (eb != hipSuccess) {
+    stream = caller;
+    return fail(FMMX_EDEVICE, std::string("hipStreamBeginCapture: ") + hipGetErrorString(eb));
+  }
+  capturing = true;
+  rc = evaluate(neq, qj, vi, gxi, gyi, gzi, FMMX_DEVICE_PTRS);
+  capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(stream, &g);
+  stream = caller;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess || !g) return fail(FMMX_EDEVICE, std::string("stream capture failed: ") + hipGetErrorString(ec));
+  graph = g;
+  if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    drop_graph();
+    return fail(FMMX_EDEVICE, "hipGraphInstantiate failed");
+  }
+  return FMMX_OK;
+}
+
+int Engine::graph_launch() {
+  if (!gexec) return fail(FMMX_ESTATE, "no captured evaluation (fmmx_graph_capture)");
+  FMMX_HIP(hipSetDevice(device));
+  FMMX_HIP(hipEventRecord(gev[0], stream));
+  FMMX_HIP(hipStreamWaitEvent(gstream, gev[0], 0));
+  FMMX_HIP(hipGraphLaunch(gexec, gstream));
+  FMMX_HIP(hipEventRecord(gev[1], gstream));
+  FMMX_HIP(hipStreamWaitEvent(stream, gev[1], 0));
+  return FMMX_OK;
+}
+
 Engine::~Engine() {
+  drop_graph();
+  if (gstream) (void)hipStreamDestroy(gstream);
+  for (auto& e : gev)
+    if (e) (void)hipEventDestroy(e);
   if (aux) (void)hipStreamDestroy(aux);
   if (aux2) (void)hipStreamDestroy(aux2);
   for (auto& e : ev)
@@ -415,6 +479,7 @@ int Engine::set_yukawa(double lambda) {
 
 int Engine::set_groups(int level) {
   if (level < 0 || level > MAXL) return fail(FMMX_EINVAL, "group level out of range [0, 8]");
+  drop_graph();
   group_level = level;
   have_points = false;  // the near lists depend on it
   return FMMX_OK;
@@ -498,6 +563,7 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
                        const float* nzi, int nj, const float* xj, const float* yj, const float* zj, const float* nxj,
                        const float* nyj, const float* nzj, int flags) {
   if (ni < 0 || nj < 0) return fail(FMMX_EINVAL, "negative particle count");
+  drop_graph();  // a captured evaluation refers to this tree's buffers
   if ((ni > 0 && (!xi || !yi || !zi)) || (nj > 0 && (!xj || !yj || !zj)))
     return fail(FMMX_EINVAL, "coordinate pointer is NULL");
   FMMX_HIP(hipSetDevice(device));
@@ -621,7 +687,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     if (per_phase) FMMX_HIP(hipEventRecord(ev[e++], stream));
     return FMMX_OK;
   };
-  FMMX_HIP(hipEventRecord(ev[2], stream));
+  if (!capturing) FMMX_HIP(hipEventRecord(ev[2], stream));
   e = 3;
   k_zero_f64<<<1, 64, 0, stream>>>(14, pair_ctr.as<double>() + 2);  // M2L pair counters per level
   if (nj > 0) {
@@ -695,7 +761,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
       k_scatter_c<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), vs.as<double>(), gxs.as<double>(), dst);
       if (!dev) FMMX_HIP(hipMemcpyAsync(vi, dst, (size_t)ni * 16, hipMemcpyDeviceToHost, stream));
     }
-    FMMX_HIP(hipEventRecord(ev[15], stream));
+    if (!capturing) FMMX_HIP(hipEventRecord(ev[15], stream));
     if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
     last_neq = neq;
     return FMMX_OK;
@@ -713,7 +779,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     k_scatter<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), srcs[k], dst);
     if (!dev) FMMX_HIP(hipMemcpyAsync(outs[k], dst, (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
   }
-  FMMX_HIP(hipEventRecord(ev[15], stream));
+  if (!capturing) FMMX_HIP(hipEventRecord(ev[15], stream));
   if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
   last_neq = neq;
   return FMMX_OK;
@@ -921,6 +987,21 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
   FMMX_GUARD({
     if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
     return h->e.set_domain(lmax, xmin, ymin, zmin, rd);
+  })
+}
+
+int fmmx_graph_capture(fmmx_handle* h, int neq, const float* qj, double* vi, double* gxi, double* gyi,
+                       double* gzi) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.graph_capture(neq, qj, vi, gxi, gyi, gzi);
+  })
+}
+
+int fmmx_graph_launch(fmmx_handle* h) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    return h->e.graph_launch();
   })
 }
 

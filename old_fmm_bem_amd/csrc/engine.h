@@ -143,6 +143,15 @@ struct Engine {
   int group_shift(int lev) const { return group_level > 0 ? lev - group_level : -1; }
   int first_m2l_level() const { return std::max(2, group_level); }
   int set_groups(int level);
+  // one evaluation captured as a HIP graph (device pointers), replayed on demand
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t gstream = nullptr;  // capture and replay stream
+  hipEvent_t gev[2] = {};         // orders replays after / before the caller's stream
+  bool capturing = false;
+  void drop_graph();
+  int graph_capture(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi);
+  int graph_launch();
   // stand-alone Morton sort (fmmx_sort): staging, keys, permutation
   DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm;
   int last_neq = -1;
