@@ -64,6 +64,10 @@ def run_engine(F, torch, x, q, p, lmax, precision, steps, warmup):
     for _ in range(warmup):
         f.set_points(x)
         f.evaluate(q)
+    # M2L launch shapes and offset splits per level, timed by whole
+    # evaluations on this tree (untimed set-up)
+    f.set_points(x)
+    f.tuning = f.autotune(q)
     # headline: one event pair per call, near field overlapped with far field
     f.set_timing(False)
     torch.cuda.synchronize()
@@ -359,6 +363,7 @@ def main():
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms) if fp64 else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "m2l_tuning": {str(k): v for k, v in f.tuning.items()},
     }
     f.close()
     print(json.dumps(out))

@@ -136,6 +136,17 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
  * operators are built at the first Yukawa evaluate after a change. */
 int fmmx_set_yukawa(fmmx_handle* h, double lambda);
 
+/* M2L launch variant at level lev (2..8) for this handle: N-tiles per wave
+ * (1|2), operator staging (0 LDS, 1 register prefetch, 2 double buffer, up
+ * to p = 11), waves per workgroup (4|8|16, 16 with staging 2), offset splits
+ * (1..16); 0 (-1 for staging) keeps the default.  The FP64 and Helmholtz
+ * kernels use the split count only.  fmmx_get_m2l_variant returns the four
+ * values in effect (after set_points).  The tuner that picks them by timing
+ * whole evaluations is old_fmm_bem_amd.FMM.autotune.  No reference
+ * counterpart. */
+int fmmx_set_m2l_variant(fmmx_handle* h, int lev, int nt, int staging, int wpg, int nsplit);
+int fmmx_get_m2l_variant(fmmx_handle* h, int lev, int* v4);
+
 /* One evaluation captured as a HIP graph: fmmx_graph_capture runs
  * fmmx_evaluate(h, neq, qj, vi, gxi, gyi, gzi, FMMX_DEVICE_PTRS) once
  * eagerly (sizing buffers, uploading operators), then records the same call

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 
 import numpy as np
 
@@ -65,6 +66,8 @@ _SYMBOLS = {
     "fmmx_set_groups": (C.c_int, [C.c_void_p, C.c_int]),
     "fmmx_graph_capture": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5),
     "fmmx_graph_launch": (C.c_int, [C.c_void_p]),
+    "fmmx_set_m2l_variant": (C.c_int, [C.c_void_p] + [C.c_int] * 5),
+    "fmmx_get_m2l_variant": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fmmx_set_yukawa": (C.c_int, [C.c_void_p, C.c_double]),
     "fmmx_set_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
     "fmmx_sort": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -194,6 +197,51 @@ class FMM:
         gp = [self._gg[k].data_ptr() for k in range(3)] if want_g else [None] * 3
         _check(lib().fmmx_graph_capture(self.h, neq, q.data_ptr(), self._gv.data_ptr(), *gp))
         return self._gv, (self._gg.T if want_g else None)
+
+    def m2l_variant(self, lev):
+        v = np.zeros(4, np.int32)
+        _check(lib().fmmx_get_m2l_variant(self.h, lev, v.ctypes.data))
+        return tuple(int(a) for a in v)
+
+    def set_m2l_variant(self, lev, nt=0, staging=-1, wpg=0, nsplit=0):
+        _check(lib().fmmx_set_m2l_variant(self.h, lev, nt, staging, wpg, nsplit))
+
+    def autotune(self, q, neq=G, reps=3):
+        """Pick the M2L launch variant of every level by timing whole
+        evaluations of q on the current points (so that the choice accounts
+        for the other phases running beside the M2L): levels from the leaf
+        up, each trying the MFMA staging/workgroup shapes and split counts
+        around the default, the others held.  Returns {level: (n-tiles,
+        staging, waves per workgroup, splits)}."""
+        import torch
+
+        def timed():
+            self.evaluate(q, neq=neq)
+            self.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                self.evaluate(q, neq=neq)
+            self.synchronize()
+            return time.perf_counter() - t0
+
+        del torch
+        lmax = self.lmax
+        shapes = [(1, 0, 8)] if self.precision == FP64 else \
+            [(1, 0, 8), (1, 0, 4), (1, 1, 8)] + ([(1, 2, 16), (1, 2, 8)] if self.p <= 11 else [])
+        for lev in range(lmax, 1, -1):
+            if self.nboxes("i", lev) == 0:
+                continue
+            best, tbest = None, float("inf")
+            for nt, st, wpg in shapes:
+                self.set_m2l_variant(lev, nt, st, wpg, 0)
+                base = self.m2l_variant(lev)[3]
+                for ns in sorted({base, max(1, base // 2), min(16, base * 2), max(1, base // 4), min(16, base * 4)}):
+                    self.set_m2l_variant(lev, nt, st, wpg, ns)
+                    t = timed()
+                    if t < tbest:
+                        best, tbest = (nt, st, wpg, ns), t
+            self.set_m2l_variant(lev, *best)
+        return {lev: self.m2l_variant(lev) for lev in range(2, lmax + 1)}
 
     def replay(self):
         """Replay the captured evaluation (asynchronous on the handle's stream)."""

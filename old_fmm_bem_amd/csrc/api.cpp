@@ -998,6 +998,36 @@ int fmmx_graph_capture(fmmx_handle* h, int neq, const float* qj, double* vi, dou
   })
 }
 
+int fmmx_set_m2l_variant(fmmx_handle* h, int lev, int nt, int staging, int wpg, int nsplit) {
+  FMMX_GUARD({
+    if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
+    if (lev < 2 || lev > fmmx::MAXL) return fmmx::fail(FMMX_EINVAL, "level out of range [2, 8]");
+    if (nt < 0 || nt > 2 || staging < -1 || staging > 2 || (wpg != 0 && wpg != 4 && wpg != 8 && wpg != 16) ||
+        nsplit < 0 || nsplit > 16)
+      return fmmx::fail(FMMX_EINVAL, "bad M2L variant");
+    auto& t = h->e.tune[lev];
+    t.nt = nt;
+    t.pa = staging;
+    t.wpg = wpg;
+    t.nsplit = nsplit;
+    return FMMX_OK;
+  })
+}
+
+int fmmx_get_m2l_variant(fmmx_handle* h, int lev, int* v4) {
+  FMMX_GUARD({
+    if (!h || !v4) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+    if (lev < 2 || lev > fmmx::MAXL) return fmmx::fail(FMMX_EINVAL, "level out of range [2, 8]");
+    if (!h->e.have_points) return fmmx::fail(FMMX_ESTATE, "the split count depends on the points (set_points)");
+    const auto v = h->e.m2l_variant(lev);
+    v4[0] = v.nt;
+    v4[1] = v.pa;
+    v4[2] = v.wpg;
+    v4[3] = v.nsplit;
+    return FMMX_OK;
+  })
+}
+
 int fmmx_graph_launch(fmmx_handle* h) {
   FMMX_GUARD({
     if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");

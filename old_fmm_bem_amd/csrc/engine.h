@@ -189,7 +189,16 @@ struct Engine {
   int run_p2p(int neq);
   int run_p2m(int neq);
   int run_m2m(int lev);
-  int m2l_splits(int lev) const;
+  // M2L launch variant per level: MFMA kernel N-tiles per wave, operator
+  // staging (0 LDS, 1 register prefetch, 2 double buffer), waves per
+  // workgroup, offset splits.  tune[] holds the autotuned choice (0 fields:
+  // the defaults); FMMX_M2L_* environment variables override both.
+  struct M2LVariant {
+    int nt = 0, pa = -1, wpg = 0, nsplit = 0;
+  };
+  M2LVariant tune[MAXL + 1];
+  M2LVariant m2l_variant(int lev) const;
+  int m2l_splits(int lev) const { return m2l_variant(lev).nsplit; }
   int run_m2l(int lev, bool init, int phase = 0);
   int run_l2l(int lev);
   int run_l2p(int neq);

@@ -643,7 +643,10 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
   constexpr int NV = FR / 4;             // float4 per operator
   constexpr int NVT = (NV + NTH - 1) / NTH;  // float4 per thread
   using V4 = f32x4;
-  __shared__ __align__(16) T As[FR];
+  // PA 2: two operator buffers (the next offset's operator is written to one
+  // while the MFMAs read the other; one barrier per offset)
+  constexpr int NBUF = PA == 2 ? 2 : 1;
+  __shared__ __align__(16) T As[NBUF * FR];
   __shared__ int s_class, s_start, s_n;
   const int tid = threadIdx.x;
   // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
@@ -700,15 +703,15 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
 #pragma unroll
     for (int mt = 0; mt < MT; mt++) acc[n][mt] = V4{0.f, 0.f, 0.f, 0.f};
   unsigned long long mypairs = 0;
-  float4 areg[PA ? NVT : 1];
+  float4 areg[PA ? NVT : 1];  // PA 1: unsigned operator prefetch; PA 2: signed
   // Reflected offsets use the operator of the canonical offset with element
   // (row i, col k) times T_r(i) T_r(k).  The float4s a thread stages sit at
   // fixed places, so their sign patterns are worked out once: nibble r of
   // sgn[v] flags the 4 floats of the thread's v-th float4 under reflection r.
   // Fragment element q = (ks * MT + mt) * 64 + lane holds row 16 mt + lane % 16
   // and col (lane / 16) * KS + ks.
-  uint32_t sgn[PA ? 1 : NVT];
-  if (!PA) {
+  uint32_t sgn[PA == 1 ? 1 : NVT];
+  if (PA != 1) {
     constexpr ReflMask<P> RM{};
 #pragma unroll
     for (int v = 0; v < NVT; v++) {
@@ -757,6 +760,84 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
       if (e < NV) areg[v] = src_[e];                                                          \
     }                                                                                         \
   }
+#define FMMX_LOAD_AS(o_)                                                                      \
+  {                                                                                           \
+    const int w_ = offtab[cls * NEBM + (o_)].w, r_ = w_ >> 16;                                \
+    const float4* src_ = reinterpret_cast<const float4*>(opF + (size_t)(w_ & 0xffff) * FR);   \
+    _Pragma("unroll") for (int v = 0; v < NVT; v++) {                                         \
+      const int e = tid + NTH * v;                                                            \
+      if (e < NV) {                                                                           \
+        float4 x_ = src_[e];                                                                  \
+        const uint32_t nib_ = sgn[v] >> (4 * r_);                                             \
+        x_.x = __uint_as_float(__float_as_uint(x_.x) ^ ((nib_ & 1u) << 31));                  \
+        x_.y = __uint_as_float(__float_as_uint(x_.y) ^ (((nib_ >> 1) & 1u) << 31));           \
+        x_.z = __uint_as_float(__float_as_uint(x_.z) ^ (((nib_ >> 2) & 1u) << 31));           \
+        x_.w = __uint_as_float(__float_as_uint(x_.w) ^ (((nib_ >> 3) & 1u) << 31));           \
+        areg[v] = x_;                                                                         \
+      }                                                                                       \
+    }                                                                                         \
+  }
+#define FMMX_STORE_A(buf_)                                                                    \
+  {                                                                                           \
+    _Pragma("unroll") for (int v = 0; v < NVT; v++) {                                         \
+      const int e = tid + NTH * v;                                                            \
+      if (e < NV) reinterpret_cast<float4*>(As + (buf_) * FR)[e] = areg[v];                   \
+    }                                                                                         \
+  }
+  if constexpr (PA == 2) {
+    // double-buffered operator: load offset o+1's operator (signed) and
+    // source multipoles into registers, run offset o's MFMAs from one LDS
+    // buffer, write o+1's operator into the other, one barrier
+    bool cur_any = false;
+    if (o0 < o1) {
+      FMMX_LOAD_AS(o0);
+      FMMX_STORE_A(0);
+      bool any_ = false;
+      FMMX_LOAD_B(o0, bcur);
+      cur_any = __ballot(any_) != 0;
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int o = o0; o < o1; o++) {
+      T bnext[NT][KS];
+      bool next_any = false;
+      if (o + 1 < o1) {
+        FMMX_LOAD_AS(o + 1);
+        bool any_ = false;
+        FMMX_LOAD_B(o + 1, bnext);
+        next_any = __ballot(any_) != 0;
+      }
+      if (wave_on && cur_any) {
+        const T* Ab = As + buf * FR;
+      T af[2][MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) af[0][mt] = Ab[mt * 64 + l];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        if (ks + 1 < KS) {
+#pragma unroll
+          for (int mt = 0; mt < MT; mt++) af[(ks + 1) & 1][mt] = Ab[((ks + 1) * MT + mt) * 64 + l];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+          for (int n = 0; n < NT; n++) acc[n][mt] = Mfma<T>::fma(af[ks & 1][mt], bcur[n][ks], acc[n][mt]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      }
+      if (o + 1 < o1) FMMX_STORE_A(buf ^ 1);
+      __syncthreads();
+      if (o + 1 < o1) {
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+          for (int ks = 0; ks < KS; ks++) bcur[n][ks] = bnext[n][ks];
+        cur_any = next_any;
+      }
+      buf ^= 1;
+    }
+  } else {
   // a wave skips the MFMAs of an offset for which none of its 16 targets has
   // a source box (boundary tiles); the vote keeps the branch wave-uniform
   bool cur_any = false;
@@ -804,14 +885,15 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
       // operator fragments of step ks+1 are read from LDS while the MFMAs of
       // step ks run; the scheduling barriers keep that order, so the LDS
       // latency sits behind MT matrix instructions instead of in front of one
+      const T* Ab = As;
       T af[2][MT];
 #pragma unroll
-      for (int mt = 0; mt < MT; mt++) af[0][mt] = As[mt * 64 + l];
+      for (int mt = 0; mt < MT; mt++) af[0][mt] = Ab[mt * 64 + l];
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) {
         if (ks + 1 < KS) {
 #pragma unroll
-          for (int mt = 0; mt < MT; mt++) af[(ks + 1) & 1][mt] = As[((ks + 1) * MT + mt) * 64 + l];
+          for (int mt = 0; mt < MT; mt++) af[(ks + 1) & 1][mt] = Ab[((ks + 1) * MT + mt) * 64 + l];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -829,6 +911,9 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
       cur_any = next_any;
     }
   }
+  }
+#undef FMMX_LOAD_AS
+#undef FMMX_STORE_A
 #undef FMMX_LOAD_A
 #undef FMMX_LOAD_B
 #pragma unroll
@@ -1452,21 +1537,30 @@ int Engine::run_l2l(int lev) {
 }
 
 // Offset splits of the M2L launch at a level (partial sums when > 1).
-int Engine::m2l_splits(int lev) const {
-  const int nb = si.nb[lev];
-  int split_env = 0;
-  if (const char* e = getenv("FMMX_M2L_SPLIT")) split_env = atoi(e);
-  if (fp64 || hh) {
-    const int tiles = (nb + M2L_TN - 1) / M2L_TN + 8;
-    return std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / tiles));
+Engine::M2LVariant Engine::m2l_variant(int lev) const {
+  M2LVariant v = tune[lev];
+  if (v.nt <= 0) v.nt = 1;
+  if (v.pa < 0) v.pa = 0;
+  if (v.wpg <= 0) v.wpg = 8;
+  if (const char* e = getenv("FMMX_M2L_NT")) v.nt = atoi(e) == 2 ? 2 : 1;
+  if (const char* e = getenv("FMMX_M2L_PA")) v.pa = std::max(0, std::min(2, atoi(e)));
+  if (const char* e = getenv("FMMX_M2L_WPG")) v.wpg = atoi(e) == 4 ? 4 : atoi(e) == 16 ? 16 : 8;
+  if (v.pa == 2 && (p > 11 || v.nt != 1)) v.pa = 0;  // two operator buffers fit LDS up to p = 11
+  if (v.wpg == 16 && v.pa != 2) v.wpg = 8;
+  if (const char* e = getenv("FMMX_M2L_SPLIT")) v.nsplit = atoi(e);
+  if (v.nsplit <= 0) {
+    // enough workgroups to cover the chip: split the 189 offsets when tiles are few
+    if (fp64 || hh) {
+      v.nsplit = 1024 / ((si.nb[lev] + M2L_TN - 1) / M2L_TN + 8);
+    } else {
+      int cmax = 0;
+      for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
+      const int tpc = std::max(1, (cmax + 16 * v.wpg * v.nt - 1) / (16 * v.wpg * v.nt));
+      v.nsplit = 1024 / (8 * tpc);
+    }
   }
-  int wpg = 8, nt = 1;
-  if (const char* e = getenv("FMMX_M2L_NT")) nt = atoi(e) == 2 ? 2 : 1;
-  if (const char* e = getenv("FMMX_M2L_WPG")) wpg = atoi(e) == 4 ? 4 : 8;
-  int cmax = 0;
-  for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
-  const int tpc = std::max(1, (cmax + 16 * wpg * nt - 1) / (16 * wpg * nt));
-  return std::max(1, std::min(16, split_env > 0 ? split_env : 1024 / (8 * tpc)));
+  v.nsplit = std::max(1, std::min(16, v.nsplit));
+  return v;
 }
 
 // phase 0: the M2L launch and (with splits) the reduction into the locals;
@@ -1552,14 +1646,12 @@ int Engine::run_m2l(int lev, bool init, int phase) {
         k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
                                                                           lexp[lev].as<double>(), init ? 1 : 0);
     } else {
-      // variant: N-tiles per wave (1|2) and operator prefetch (0|1)
-      int nt = 1, pa = 0, wpg = 8;
-      if (const char* e = getenv("FMMX_M2L_NT")) nt = atoi(e) == 2 ? 2 : 1;
-      if (const char* e = getenv("FMMX_M2L_PA")) pa = atoi(e) ? 1 : 0;
+      // variant: N-tiles per wave, operator staging, waves per workgroup
+      const M2LVariant var = m2l_variant(lev);
+      int nt = var.nt, pa = var.pa, wpg = var.wpg;
       // reflection-canonical operators (Laplace, EXACT) are signed while staged
       const bool use_canon = canon && !yk;
-      if (use_canon) pa = 0;
-      if (const char* e = getenv("FMMX_M2L_WPG")) wpg = atoi(e) == 4 ? 4 : 8;
+      if (use_canon && pa == 1) pa = 0;
       // per class at most ceil(max class count / TN) tiles; grid = 8 classes x tiles x splits
       int cmax = 0;
       for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
@@ -1577,7 +1669,13 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       pair_ctr.as<unsigned long long>() + 2 + lev)
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
-      if (wpg == 8) {
+      if (pa == 2 && P > 11) pa = 0;  // two operator buffers fit the 160 KB of LDS up to p = 11
+      if (pa == 2) {
+        if constexpr (P <= 11) {
+          if (wpg == 16) FMMX_M2L_LAUNCH(1, 2, 16);
+          else FMMX_M2L_LAUNCH(1, 2, 8);
+        }
+      } else if (wpg == 8) {
         if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 8);
         else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 8);
         else FMMX_M2L_LAUNCH(2, 0, 8);

@@ -317,6 +317,23 @@ def test_canonical_m2l_operators_match_full_set(p, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_autotune_keeps_results(precision):
+    """The autotuner only picks launch shapes and offset splits: results
+    after tuning agree with the untuned ones to summation-order rounding."""
+    x, q = uniform_cube(200000, 86)
+    f = F.FMM(p=10, lmax=5, precision=precision, convention="exact", box=(0, 0, 0, 1))
+    f.set_points(x)
+    v0, g0 = f.evaluate(q)
+    ch = f.autotune(q, reps=1)
+    print(f"precision={precision} choices={ch}", flush=True)
+    assert set(ch) == {2, 3, 4, 5} and all(1 <= c[3] <= 16 for c in ch.values())
+    v1, g1 = f.evaluate(q)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert rel_l2(v1, v0) < tol and rel_l2(g1, g0) < tol
+    f.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_shared_tree_equals_separate_trees(precision):
     """set_points(x) shares one tree between targets and sources; a copy of
     the same points builds two.  Results are bit-identical, then again after
