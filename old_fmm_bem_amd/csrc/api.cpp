@@ -26,10 +26,18 @@ __global__ void k_gather_q(int n, const uint32_t* __restrict__ perm, const float
   if (i < n) qs[i] = q[perm[i]];
 }
 
-__global__ void k_scatter(int n, const uint32_t* __restrict__ perm, const double* __restrict__ s,
-                          double* __restrict__ d) {
+// up to four key-ordered outputs back to caller order in one pass (NULL: skip)
+__global__ void k_scatter4(int n, const uint32_t* __restrict__ perm, const double* __restrict__ s0,
+                           const double* __restrict__ s1, const double* __restrict__ s2, const double* __restrict__ s3,
+                           double* __restrict__ d0, double* __restrict__ d1, double* __restrict__ d2,
+                           double* __restrict__ d3) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[perm[i]] = s[i];
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+  if (d0) d0[j] = s0[i];
+  if (d1) d1[j] = s1[i];
+  if (d2) d2[j] = s2[i];
+  if (d3) d3[j] = s3[i];
 }
 
 // complex charges given as (re, im) pairs -> key-ordered real and imaginary parts
@@ -769,16 +777,20 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   const bool g = neq == FMMX_G || neq == FMMX_YUKAWA;
   double* outs[4] = {vi, g ? gxi : nullptr, g ? gyi : nullptr, g ? gzi : nullptr};
   double* srcs[4] = {vs.as<double>(), gxs.as<double>(), gys.as<double>(), gzs.as<double>()};
+  double* dsts[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int k = 0; k < 4; k++) {
     if (!outs[k] || ni == 0) continue;
-    double* dst = outs[k];
+    dsts[k] = outs[k];
     if (!dev) {
       if ((rc = out[k].ensure(ni1 * 8))) return rc;
-      dst = out[k].as<double>();
+      dsts[k] = out[k].as<double>();
     }
-    k_scatter<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), srcs[k], dst);
-    if (!dev) FMMX_HIP(hipMemcpyAsync(outs[k], dst, (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
   }
+  if (ni > 0)
+    k_scatter4<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), srcs[0], srcs[1], srcs[2], srcs[3], dsts[0],
+                                             dsts[1], dsts[2], dsts[3]);
+  for (int k = 0; k < 4; k++)
+    if (!dev && dsts[k]) FMMX_HIP(hipMemcpyAsync(outs[k], dsts[k], (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
   if (!capturing) FMMX_HIP(hipEventRecord(ev[15], stream));
   if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
   last_neq = neq;

@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
 //   YK 2: one part (Re or Im of the field) of Helmholtz locals,
 //       phi = c4/rb sum_n j^_n(k r) sum_m L S_n^m(u), potential only
 //   Box b's local starts at L + b * lstride + loff.
-template <int P, typename T, int YK = 0>
+template <int P, typename T, int YK = 0, int HALF = 0>
 __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
                                             const float* __restrict__ y, const float* __restrict__ z,
@@ -350,12 +350,17 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
                                             const double* __restrict__ corr, double c4, double* __restrict__ vs,
                                             double* __restrict__ gxs, double* __restrict__ gys,
                                             double* __restrict__ gzs, double lrb, int lstride, int loff) {
-  constexpr int NR = P * (P + 1) / 2, P2 = P * P;
-  __shared__ T Lr[NR], Li[NR];
-  const int b = blockIdx.x, lane = threadIdx.x;
-  if (b >= nb) return;
+  // HALF: two leaves per wave (32 lanes each), for leaves of at most about
+  // 32 targets where a whole wave per leaf would idle half its lanes
+  constexpr int NR = P * (P + 1) / 2, P2 = P * P, NL = HALF ? 2 : 1, STEP = HALF ? 32 : 64;
+  __shared__ T Lrs[NL][NR], Lis[NL][NR];
+  const int h = HALF ? (int)(threadIdx.x >> 5) : 0, lane = HALF ? (int)(threadIdx.x & 31) : (int)threadIdx.x;
+  const int b = HALF ? 2 * (int)blockIdx.x + h : (int)blockIdx.x;
+  const bool valid = b < nb;
+  T* Lr = Lrs[h];
+  T* Li = Lis[h];
   // weights: 1 for m = 0, 2 for m > 0 (kernel/G_l2p.cxx:246-260); table ratio in REF
-  for (int s = lane; s < NR; s += 64) {
+  for (int s = lane; valid && s < NR; s += STEP) {
     int n = 0;
     while (sl(n + 1, 0) <= s) n++;
     const int m = s - sl(n, 0);
@@ -364,11 +369,12 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
     Li[s] = m == 0 ? T(0) : T(w * (double)L[(size_t)b * lstride + loff + NR + n * (n - 1) / 2 + m - 1]);
   }
   __syncthreads();
+  if (!valid) return;
   double cx, cy, cz;
   box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
   const double rbinv = 1.0 / rb;
   const int i0 = bfirst[b], i1 = blast[b];
-  for (int i = i0 + lane; i <= i1; i += 64) {
+  for (int i = i0 + lane; i <= i1; i += STEP) {
     double ux, uy, uz;
     eff_point(conv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
     T Sr[NR], Si[NR];
@@ -1421,13 +1427,19 @@ int Engine::run_l2p(int neq) {
     // Yukawa locals carry the kernel's constant: phi = sum L~ h_n S_n(u)
     const double lrb = (hh ? hk : lam) * rb(lmax), cl = yk ? rb(lmax) : c4();
     const int grad = neq == FMMX_G || neq == FMMX_YUKAWA;
-#define FMMX_L2P(T_, YK_, V_, STRIDE_, OFF_)                                                                      \
-  k_l2p<P, T_, YK_><<<nb, 64, 0, stream>>>(nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(),          \
-                                           si.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), \
-                                           si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),                 \
-                                           lexp[lmax].as<T_>(), neq == FMMX_GNI, grad, cfg.convention, lmax, xmin,  \
-                                           ymin, zmin, rb(lmax), corr, cl, V_, gxs.as<double>(), gys.as<double>(),  \
-                                           gzs.as<double>(), lrb, STRIDE_, OFF_)
+    // two leaves per wave when leaves hold at most about 32 targets
+    const bool half = (double)si.n / nb <= 32.0;
+#define FMMX_L2P_H(T_, YK_, H_, V_, STRIDE_, OFF_)                                                                \
+  k_l2p<P, T_, YK_, H_><<<H_ ? (nb + 1) / 2 : nb, 64, 0, stream>>>(                                              \
+      nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(), si.blast.as<int>(), si.x.as<float>(),        \
+      si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),               \
+      lexp[lmax].as<T_>(), neq == FMMX_GNI, grad, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, cl, V_, \
+      gxs.as<double>(), gys.as<double>(), gzs.as<double>(), lrb, STRIDE_, OFF_)
+#define FMMX_L2P(T_, YK_, V_, STRIDE_, OFF_)               \
+  do {                                                     \
+    if (half) FMMX_L2P_H(T_, YK_, 1, V_, STRIDE_, OFF_);   \
+    else FMMX_L2P_H(T_, YK_, 0, V_, STRIDE_, OFF_);        \
+  } while (0)
     if (hh) {
       // Re of the potential into vs, Im into gxs
       if constexpr (P % 2 == 0 && P >= 4) {
@@ -1449,6 +1461,7 @@ int Engine::run_l2p(int neq) {
       else FMMX_L2P(float, 0, vs.as<double>(), P2, 0);
     }
 #undef FMMX_L2P
+#undef FMMX_L2P_H
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
