@@ -230,7 +230,7 @@ __device__ __forceinline__ void box_center(uint32_t key, int lev, double xmin, d
 //       imaginary), M_n^m = sum q j^_n(k rho) (rho/rb)^n conj(Y_n^m)
 //       (lrb = k rb)
 //   Box b's multipole starts at M + b * mstride + moff.
-template <int P, typename T, int YK = 0>
+template <int P, typename T, int YK = 0, int HALF = 0>
 __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
                                             const float* __restrict__ y, const float* __restrict__ z,
@@ -239,19 +239,25 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
                                             int conv, int lev, double xmin, double ymin, double zmin, double rb,
                                             const double* __restrict__ corr, T* __restrict__ M, double lrb,
                                             int mstride, int moff) {
-  constexpr int NR = P * (P + 1) / 2, P2 = P * P, NC = (P2 + 63) / 64;
+  // HALF: two leaves per wave, rows 0-31 for the first and 32-63 for the
+  // second, for leaves of at most about 32 particles
+  constexpr int NR = P * (P + 1) / 2, P2 = P * P, NL = HALF ? 2 : 1, STEP = 64 / NL;
+  constexpr int NC = (NL * P2 + 63) / 64;
   __shared__ T red[64][P2 + 1];
-  const int b = blockIdx.x, lane = threadIdx.x;
-  if (b >= nb) return;
-  double cx, cy, cz;
-  box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
+  const int lane = threadIdx.x, h = HALF ? lane >> 5 : 0, sub = HALF ? lane & 31 : lane;
+  const int b = HALF ? 2 * (int)blockIdx.x + h : (int)blockIdx.x;
+  const bool valid = b < nb;
+  if (!HALF && !valid) return;
+  double cx = 0, cy = 0, cz = 0;
+  if (valid) box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
   const double rbinv = 1.0 / rb;
   T acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) acc[c] = T(0);
-  const int i0 = bfirst[b], i1 = blast[b];
-  for (int base = i0; base <= i1; base += 64) {
-    const int i = base + lane;
+  const int i0 = valid ? bfirst[b] : 0, i1 = valid ? blast[b] : -1;
+  for (int base = 0;; base += STEP) {
+    if (__ballot(i0 + base <= i1) == 0) break;  // one wave per block: uniform
+    const int i = i0 + base + sub;
     T* row = red[lane];
     if (i <= i1) {
       double ux, uy, uz;
@@ -310,26 +316,31 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
       for (int c = 0; c < P2; c++) row[c] = T(0);
     }
     __syncthreads();
+    // column sums: entry idx = (leaf of the wave, column) over that leaf's rows
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      const int col = lane + 64 * c;
-      if (col < P2) {
+      const int idx = lane + 64 * c, hh = idx / P2, col = idx % P2;
+      if (hh < NL) {
         T s = T(0);
-        for (int r = 0; r < 64; r++) s += red[r][col];
+        for (int r = 0; r < STEP; r++) s += red[hh * STEP + r][col];
         acc[c] += s;
       }
     }
     __syncthreads();
   }
   using ML = MLayout<P>;
+  const int b0 = HALF ? 2 * (int)blockIdx.x : (int)blockIdx.x;
 #pragma unroll
   for (int c = 0; c < NC; c++) {
-    const int col = lane + 64 * c;
-    if (col < P2) M[(size_t)b * mstride + moff + ML::at(col)] = acc[c];
+    const int idx = lane + 64 * c, hh = idx / P2, col = idx % P2;
+    if (hh < NL && b0 + hh < nb) M[(size_t)(b0 + hh) * mstride + moff + ML::at(col)] = acc[c];
   }
   // zero the padding slots (M2M applies its operator to whole stored vectors)
-  for (int c = lane; c < ML::VS; c += 64)
-    if (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2) M[(size_t)b * mstride + moff + c] = T(0);
+  for (int c2 = lane; c2 < NL * ML::VS; c2 += 64) {
+    const int hh = c2 / ML::VS, c = c2 % ML::VS;
+    if (b0 + hh < nb && (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2))
+      M[(size_t)(b0 + hh) * mstride + moff + c] = T(0);
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -1386,12 +1397,19 @@ int Engine::run_p2m(int neq) {
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     const double lrb = (hh ? hk : lam) * rb(lmax);
-#define FMMX_P2M(T_, YK_, Q_, STRIDE_, OFF_)                                                                      \
-  k_p2m<P, T_, YK_><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),          \
-                                           sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), \
-                                           sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), Q_,              \
-                                           neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, \
-                                           mexp[lmax].as<T_>(), lrb, STRIDE_, OFF_)
+    // two leaves per wave when leaves hold at most about 32 particles
+    const bool half = (double)sj.n / nb <= 32.0;
+#define FMMX_P2M_H(T_, YK_, H_, Q_, STRIDE_, OFF_)                                                                 \
+  k_p2m<P, T_, YK_, H_><<<H_ ? (nb + 1) / 2 : nb, 64, 0, stream>>>(                                               \
+      nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(), sj.blast.as<int>(), sj.x.as<float>(),         \
+      sj.y.as<float>(), sj.z.as<float>(), sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), Q_,            \
+      neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, mexp[lmax].as<T_>(), lrb, STRIDE_, \
+      OFF_)
+#define FMMX_P2M(T_, YK_, Q_, STRIDE_, OFF_)                 \
+  do {                                                       \
+    if (half) FMMX_P2M_H(T_, YK_, 1, Q_, STRIDE_, OFF_);     \
+    else FMMX_P2M_H(T_, YK_, 0, Q_, STRIDE_, OFF_);          \
+  } while (0)
     if (hh) {
       // the multipoles of the real and of the imaginary charges, side by side
       if constexpr (P % 2 == 0 && P >= 4) {
@@ -1413,6 +1431,7 @@ int Engine::run_p2m(int neq) {
       else FMMX_P2M(float, 0, qs.as<float>(), VS, 0);
     }
 #undef FMMX_P2M
+#undef FMMX_P2M_H
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
