@@ -36,10 +36,16 @@ __global__ void k_keys(int n, const float* __restrict__ x, const float* __restri
   idx[i] = (uint32_t)i;
 }
 
-__global__ void k_gather(int n, const uint32_t* __restrict__ perm, const float* __restrict__ a,
-                         float* __restrict__ b) {
+// three coordinate (or normal) columns into key order in one pass
+__global__ void k_gather3(int n, const uint32_t* __restrict__ perm, const float* __restrict__ a0,
+                          const float* __restrict__ a1, const float* __restrict__ a2, float* __restrict__ b0,
+                          float* __restrict__ b1, float* __restrict__ b2) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = a[perm[i]];
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+  b0[i] = a0[j];
+  b1[i] = a1[j];
+  b2[i] = a2[j];
 }
 
 __global__ void k_level_flags(int n, const uint32_t* __restrict__ keys, int shift, int* __restrict__ flag) {
@@ -252,18 +258,17 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
     s.nb.fill(0);
     if (s.has_normals) {
       if ((rc = s.nx.ensure(nn * 4)) || (rc = s.ny.ensure(nn * 4)) || (rc = s.nz.ensure(nn * 4))) return rc;
-      const uint32_t* perm = s.perm.as<uint32_t>();
-      k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nx, s.nx.as<float>());
-      k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, ny, s.ny.as<float>());
-      k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nz, s.nz.as<float>());
+      k_gather3<<<nblk(n), TPB, 0, stream>>>(n, s.perm.as<uint32_t>(), nx, ny, nz, s.nx.as<float>(),
+                                             s.ny.as<float>(), s.nz.as<float>());
     }
     if (dense) {
       const int* cnt = s.cnt.as<int>();
       for (int lev = 2; lev <= lmax; lev++) {
         long nd = 1L << (3 * lev);
+        const int ub = (int)std::min<long>(n, nd);  // boxes at this level
         if ((rc = s.dense[lev].ensure(nd * 4))) return rc;
         k_fill_int<<<nblk(nd), TPB, 0, stream>>>(nd, -1, s.dense[lev].as<int>());
-        k_dense<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, s.bkey[lev].as<uint32_t>(), s.dense[lev].as<int>());
+        k_dense<<<nblk(ub), TPB, 0, stream>>>(ub, cnt + lev, s.bkey[lev].as<uint32_t>(), s.dense[lev].as<int>());
       }
     }
     return FMMX_OK;
@@ -293,14 +298,10 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
                                      s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
                                      stream));
   const uint32_t* perm = s.perm.as<uint32_t>();
-  k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, x, s.x.as<float>());
-  k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, y, s.y.as<float>());
-  k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, z, s.z.as<float>());
-  if (s.has_normals) {
-    k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nx, s.nx.as<float>());
-    k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, ny, s.ny.as<float>());
-    k_gather<<<nblk(n), TPB, 0, stream>>>(n, perm, nz, s.nz.as<float>());
-  }
+  k_gather3<<<nblk(n), TPB, 0, stream>>>(n, perm, x, y, z, s.x.as<float>(), s.y.as<float>(), s.z.as<float>());
+  if (s.has_normals)
+    k_gather3<<<nblk(n), TPB, 0, stream>>>(n, perm, nx, ny, nz, s.nx.as<float>(), s.ny.as<float>(),
+                                           s.nz.as<float>());
   int* cnt = s.cnt.as<int>();
   for (int lev = lmax; lev >= 2; lev--) {
     if ((rc = s.flag[lev].ensure(nn * 4)) || (rc = s.pos[lev].ensure(nn * 4)) ||
@@ -316,19 +317,25 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
                                                  s.pos[lev].as<int>(), s.bkey[lev].as<uint32_t>(),
                                                  s.bfirst[lev].as<int>(), cnt + lev);
   }
+  // box-level kernels run over at most min(n, 8^lev) boxes (the counts stay on
+  // the device)
+  auto nbox = [&](int lev) { return (int)std::min<long>(n, 1L << (3 * lev)); };
   for (int lev = lmax; lev >= 3; lev--) {
-    k_parent<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, s.bfirst[lev].as<int>(), s.pos[lev - 1].as<int>(),
-                                          s.flag[lev - 1].as<int>(), s.parent[lev].as<int>());
-    k_cfirst<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, cnt + lev - 1, s.parent[lev].as<int>(),
-                                          s.cfirst[lev - 1].as<int>());
+    const int ub = nbox(lev);
+    k_parent<<<nblk(ub), TPB, 0, stream>>>(ub, cnt + lev, s.bfirst[lev].as<int>(), s.pos[lev - 1].as<int>(),
+                                           s.flag[lev - 1].as<int>(), s.parent[lev].as<int>());
+    k_cfirst<<<nblk(ub), TPB, 0, stream>>>(ub, cnt + lev, cnt + lev - 1, s.parent[lev].as<int>(),
+                                           s.cfirst[lev - 1].as<int>());
   }
-  k_blast<<<nblk(n), TPB, 0, stream>>>(n, cnt + lmax, s.bfirst[lmax].as<int>(), n, s.blast.as<int>());
+  k_blast<<<nblk(nbox(lmax)), TPB, 0, stream>>>(nbox(lmax), cnt + lmax, s.bfirst[lmax].as<int>(), n,
+                                                s.blast.as<int>());
   if (dense) {
     for (int lev = 2; lev <= lmax; lev++) {
       long nd = 1L << (3 * lev);
       if ((rc = s.dense[lev].ensure(nd * 4))) return rc;
       k_fill_int<<<nblk(nd), TPB, 0, stream>>>(nd, -1, s.dense[lev].as<int>());
-      k_dense<<<nblk(n), TPB, 0, stream>>>(n, cnt + lev, s.bkey[lev].as<uint32_t>(), s.dense[lev].as<int>());
+      k_dense<<<nblk(nbox(lev)), TPB, 0, stream>>>(nbox(lev), cnt + lev, s.bkey[lev].as<uint32_t>(),
+                                                   s.dense[lev].as<int>());
     }
   }
   return FMMX_OK;
