@@ -26,6 +26,35 @@ __global__ void k_gather_q(int n, const uint32_t* __restrict__ perm, const float
   if (i < n) qs[i] = q[perm[i]];
 }
 
+// device counters the host needs after the tree build, gathered into one
+// small buffer so that a single copy brings them back
+struct CountSrc {
+  const int* a[2 + MAXL + 1];  // box counts of both sides, class counts per level
+  const unsigned long long* pairs;
+};
+__global__ void k_pack_counts(CountSrc src, int lmax, int stage, int* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (stage == 0) {
+    // [0, 9): target boxes per level, [9, 18): source boxes per level
+    if (t < 2 * (MAXL + 1)) {
+      const int* c = src.a[t / (MAXL + 1)];
+      out[t] = c ? c[t % (MAXL + 1)] : 0;
+    }
+  } else {
+    // [0, 2): near pair count (u64), then 8 class counts per level
+    if (t == 0) {
+      const unsigned long long v = src.pairs ? *src.pairs : 0ull;
+      out[0] = (int)(v & 0xffffffffu);
+      out[1] = (int)(v >> 32);
+    }
+    for (int e = t; e < 8 * (MAXL + 1); e += blockDim.x) {
+      const int l = e / 8;
+      const int* c = src.a[2 + l];
+      out[2 + e] = (l >= 2 && l <= lmax && c) ? c[e % 8] : 0;
+    }
+  }
+}
+
 // up to four key-ordered outputs back to caller order in one pass (NULL: skip)
 __global__ void k_scatter4(int n, const uint32_t* __restrict__ perm, const double* __restrict__ s0,
                            const double* __restrict__ s1, const double* __restrict__ s2, const double* __restrict__ s3,
@@ -397,6 +426,7 @@ int Engine::graph_launch() {
 
 Engine::~Engine() {
   drop_graph();
+  if (rb_host) (void)hipHostFree(rb_host);
   if (gstream) (void)hipStreamDestroy(gstream);
   for (auto& e : gev)
     if (e) (void)hipEventDestroy(e);
@@ -601,7 +631,8 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
     k_init_mm<<<1, 64, 0, stream>>>(mm.as<int>());
     if (ni > 0) k_minmax<<<nblk(ni), TPB, 0, stream>>>(ni, d[0], d[1], d[2], mm.as<int>());
     if (nj > 0) k_minmax<<<nblk(nj), TPB, 0, stream>>>(nj, d[6], d[7], d[8], mm.as<int>());
-    int h[6];
+    if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, (2 + 8 * (MAXL + 1)) * 4, hipHostMallocDefault));
+    int* h = rb_host;
     FMMX_HIP(hipMemcpyAsync(h, mm.p, 24, hipMemcpyDeviceToHost, stream));
     FMMX_HIP(hipStreamSynchronize(stream));
     float lo[3], hi[3];
@@ -624,26 +655,33 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   // tree of both sides (keys, sort, gathers, per-level boxes)
   if ((rc = build_side(si, ni, d[0], d[1], d[2], d[3], d[4], d[5], false))) return rc;
   if ((rc = build_side(sj, nj, d[6], d[7], d[8], d[9], d[10], d[11], true, same ? &si : nullptr))) return rc;
-  int hc[2][MAXL + 1] = {};
-  if (ni > 0) FMMX_HIP(hipMemcpyAsync(hc[0], si.cnt.p, (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
-  if (nj > 0) FMMX_HIP(hipMemcpyAsync(hc[1], sj.cnt.p, (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
+  // one pinned copy per readback (small pageable copies cost tens of us each)
+  const int nrb = 2 + 8 * (MAXL + 1);
+  if ((rc = rb_dev.ensure(nrb * 4))) return rc;
+  if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, nrb * 4, hipHostMallocDefault));
+  CountSrc cs{};
+  cs.a[0] = ni > 0 ? si.cnt.as<int>() : nullptr;
+  cs.a[1] = nj > 0 ? sj.cnt.as<int>() : nullptr;
+  k_pack_counts<<<1, 64, 0, stream>>>(cs, lmax, 0, rb_dev.as<int>());
+  FMMX_HIP(hipMemcpyAsync(rb_host, rb_dev.p, 2 * (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
   FMMX_HIP(hipStreamSynchronize(stream));
   for (int l = 2; l <= lmax; l++) {
-    si.nb[l] = ni > 0 ? hc[0][l] : 0;
-    sj.nb[l] = nj > 0 ? hc[1][l] : 0;
+    si.nb[l] = rb_host[l];
+    sj.nb[l] = rb_host[MAXL + 1 + l];
   }
   if ((rc = build_lists())) return rc;
   if ((rc = build_m2l_order())) return rc;
   if (keep_lists && (rc = build_far_lists())) return rc;
   FMMX_HIP(hipEventRecord(ev[1], stream));
-  unsigned long long pc = 0;
-  if (si.nb[lmax] > 0) FMMX_HIP(hipMemcpyAsync(&pc, pair_ctr.p, 8, hipMemcpyDeviceToHost, stream));
-  for (int l = 2; l <= lmax; l++) {
-    std::fill(cls_cnt_h[l], cls_cnt_h[l] + 8, 0);
-    if (si.nb[l] > 0) FMMX_HIP(hipMemcpyAsync(cls_cnt_h[l], cls_cnt[l].p, 32, hipMemcpyDeviceToHost, stream));
-  }
+  CountSrc cc{};
+  for (int l = 2; l <= lmax; l++) cc.a[2 + l] = si.nb[l] > 0 ? cls_cnt[l].as<int>() : nullptr;
+  cc.pairs = si.nb[lmax] > 0 ? pair_ctr.as<unsigned long long>() : nullptr;
+  k_pack_counts<<<1, 64, 0, stream>>>(cc, lmax, 1, rb_dev.as<int>());
+  FMMX_HIP(hipMemcpyAsync(rb_host, rb_dev.p, nrb * 4, hipMemcpyDeviceToHost, stream));
   FMMX_HIP(hipStreamSynchronize(stream));
-  p2p_pairs = (int64_t)pc;
+  for (int l = 2; l <= MAXL; l++)
+    for (int c = 0; c < 8; c++) cls_cnt_h[l][c] = l <= lmax ? rb_host[2 + 8 * l + c] : 0;
+  p2p_pairs = (int64_t)(((unsigned long long)(uint32_t)rb_host[1] << 32) | (uint32_t)rb_host[0]);
   float ms = 0;
   FMMX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   stats.ms_tree = ms;

@@ -109,6 +109,8 @@ struct Engine {
   DevBuf offtab_c;  // the same with w = canonical operator | reflection bits << 16
   bool canon = false;  // FP32 EXACT: M2L from the 56 reflection-canonical operators
   DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)
+  DevBuf rb_dev;            // counters gathered for one readback
+  int* rb_host = nullptr;   // pinned host copy of rb_dev
   int ops_lmax = -1;
   float ops_rd = -1;
   // caller-order staging for host pointers
