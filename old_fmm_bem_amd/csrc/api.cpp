@@ -139,6 +139,7 @@ int nlevel(int nmax) {
 // process-wide cache of host operators keyed by (p, conv, lmax, rd)
 struct OpSet {
   std::vector<double> m2m, l2l;  // transposed [op][k][row]
+  std::vector<double> m2m_frag, l2l_frag;  // MFMA fragment order (FP32 engine)
   std::vector<double> m2l_frag;  // MFMA fragment order (FP32 engine)
   std::vector<double> m2l_canon; // fragment order, the 56 offsets with dx, dy, dz >= 0 (EXACT)
   std::vector<double> m2l_kmaj;  // transposed [op][k][row] (FP64 engine)
@@ -195,17 +196,21 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
   const int KS = (P2 + 3) / 4, KSP = (KS + 3) / 4 * 4, VS = 4 * KSP;
   const size_t vq = (size_t)VS * VS;
   S->m2m.assign(8 * vq, 0.0);
+  const size_t fr = (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64;
+  S->m2m_frag.resize(8 * fr);
+  S->l2l_frag.resize(8 * fr);
   for (int o = 0; o < 8; o++) {
     op_m2m(T, o, A.data());
+    fragment_into(A.data(), P2, &S->m2m_frag[o * fr]);
     // padded multipole layout, transposed [k][row]
     for (int r = 0; r < P2; r++)
       for (int k = 0; k < P2; k++)
         S->m2m[o * vq + (size_t)((k / KS) * KSP + k % KS) * VS + (r / KS) * KSP + r % KS] = A[(size_t)r * P2 + k];
     op_l2l(T, o, A.data());
+    fragment_into(A.data(), P2, &S->l2l_frag[o * fr]);
     transpose_into(A.data(), P2, &S->l2l[o * sq]);
   }
   S->nlev = per_level ? lmax - 1 : 1;
-  const size_t fr = (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64;
   S->m2l_frag.resize((size_t)S->nlev * NOFF * fr);
   S->m2l_kmaj.resize((size_t)S->nlev * NOFF * sq);
   const int njobs = S->nlev * NOFF;
@@ -262,6 +267,8 @@ std::shared_ptr<OpSet> get_yuk_ops(int p, double lam, float rd, int lmax) {
   S->l2l.assign((size_t)nt_lev * 8 * sq, 0.0);
   S->nlev = lmax - 1;
   const size_t fr = (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64;
+  S->m2m_frag.assign((size_t)nt_lev * 8 * fr, 0.0);
+  S->l2l_frag.assign((size_t)nt_lev * 8 * fr, 0.0);
   S->m2l_frag.resize((size_t)S->nlev * NOFF * fr);
   S->m2l_kmaj.resize((size_t)S->nlev * NOFF * sq);
   const int ntr = lmax > 2 ? (lmax - 2) * 8 : 0, njobs = ntr + S->nlev * NOFF;
@@ -280,6 +287,8 @@ std::shared_ptr<OpSet> get_yuk_ops(int p, double lam, float rd, int lmax) {
             for (int k = 0; k < P2; k++)
               M[(size_t)((k / KS) * KSP + k % KS) * VS + (r / KS) * KSP + r % KS] = A[(size_t)r * P2 + k];
           transpose_into(B.data(), P2, &S->l2l[((size_t)li * 8 + o) * sq]);
+          fragment_into(A.data(), P2, &S->m2m_frag[((size_t)li * 8 + o) * fr]);
+          fragment_into(B.data(), P2, &S->l2l_frag[((size_t)li * 8 + o) * fr]);
         } else {
           const int jj = j - ntr, li = jj / NOFF, oi = jj % NOFF;
           int d[3];
@@ -557,7 +566,7 @@ int Engine::upload_yukawa_operators() {
         (rc = upload<double>(yop_m2l, S->m2l_kmaj, stream)))
       return rc;
   } else {
-    if ((rc = upload<float>(yop_m2m, S->m2m, stream)) || (rc = upload<float>(yop_l2l, S->l2l, stream)) ||
+    if ((rc = upload<float>(yop_m2m, S->m2m_frag, stream)) || (rc = upload<float>(yop_l2l, S->l2l_frag, stream)) ||
         (rc = upload<float>(yop_m2l, S->m2l_frag, stream)))
       return rc;
   }
@@ -577,7 +586,7 @@ int Engine::upload_operators() {
         (rc = upload<double>(op_m2l, S->m2l_kmaj, stream)))
       return rc;
   } else {
-    if ((rc = upload<float>(op_m2m, S->m2m, stream)) || (rc = upload<float>(op_l2l, S->l2l, stream)) ||
+    if ((rc = upload<float>(op_m2m, S->m2m_frag, stream)) || (rc = upload<float>(op_l2l, S->l2l_frag, stream)) ||
         (rc = upload<float>(op_m2l, canon ? S->m2l_canon : S->m2l_frag, stream)))
       return rc;
   }

@@ -213,15 +213,19 @@ struct Engine {
   // (pi/2)/lambda of kernel/Helmholtz_G_p2p.cxx:22 (float pi in REF)
   double c_yukawa() const { return (cfg.convention == FMMX_CONV_REF ? 0.5 * (double)PIF : 0.5 * M_PI) / lam; }
   // translation operators of the evaluation in progress
+  // (FP32 Laplace and Yukawa: MFMA fragment order, FR floats per octant)
+  size_t frag_size() const { return (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64; }
   template <class T>
   const T* m2m_ops(int parent_lev) const {
     if (hh) return hop_m2m.as<T>() + (size_t)(parent_lev - 2) * 8 * 4 * VS * VS;
-    return yk ? yop_m2m.as<T>() + (size_t)(parent_lev - 2) * 8 * VS * VS : op_m2m.as<T>();
+    const size_t per = fp64 ? (size_t)VS * VS : frag_size();
+    return yk ? yop_m2m.as<T>() + (size_t)(parent_lev - 2) * 8 * per : op_m2m.as<T>();
   }
   template <class T>
   const T* l2l_ops(int child_lev) const {
     if (hh) return hop_l2l.as<T>() + (size_t)(child_lev - 3) * 8 * 4 * P2 * P2;
-    return yk ? yop_l2l.as<T>() + (size_t)(child_lev - 3) * 8 * P2 * P2 : op_l2l.as<T>();
+    const size_t per = fp64 ? (size_t)P2 * P2 : frag_size();
+    return yk ? yop_l2l.as<T>() + (size_t)(child_lev - 3) * 8 * per : op_l2l.as<T>();
   }
   double rb(int lev) const { return (double)rd / (double)(1 << lev); }
 };

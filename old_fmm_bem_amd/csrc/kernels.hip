@@ -60,6 +60,39 @@ constexpr HarmCoef make_coef() {
 __constant__ HarmCoef c_hc = make_coef();
 
 __host__ __device__ constexpr int sl(int n, int m) { return n * (n + 1) / 2 + m; }
+// the same coefficients rounded to float once, read by uniform (scalar) loads
+struct HarmCoefF {
+  float a[MAXP + 2], b[MAXP + 2], c[MAXP + 2][MAXP + 2], d[MAXP + 2][MAXP + 2];
+};
+constexpr HarmCoefF make_coef_f() {
+  const HarmCoef h = make_coef();
+  HarmCoefF f{};
+  for (int m = 0; m <= MAXP + 1; m++) {
+    f.a[m] = (float)h.a[m];
+    f.b[m] = (float)h.b[m];
+    for (int n = 0; n <= MAXP + 1; n++) {
+      f.c[n][m] = (float)h.c[n][m];
+      f.d[n][m] = (float)h.d[n][m];
+    }
+  }
+  return f;
+}
+__constant__ HarmCoefF c_hcf = make_coef_f();
+template <typename T>
+struct HC {
+  static __device__ __forceinline__ T a(int m) { return T(c_hc.a[m]); }
+  static __device__ __forceinline__ T b(int m) { return T(c_hc.b[m]); }
+  static __device__ __forceinline__ T c(int n, int m) { return T(c_hc.c[n][m]); }
+  static __device__ __forceinline__ T d(int n, int m) { return T(c_hc.d[n][m]); }
+};
+template <>
+struct HC<float> {
+  static __device__ __forceinline__ float a(int m) { return c_hcf.a[m]; }
+  static __device__ __forceinline__ float b(int m) { return c_hcf.b[m]; }
+  static __device__ __forceinline__ float c(int n, int m) { return c_hcf.c[n][m]; }
+  static __device__ __forceinline__ float d(int n, int m) { return c_hcf.d[n][m]; }
+};
+
 
 // storage of a multipole slot c in the padded layout read by the M2L B
 // operand: four segments of KSP = roundup4(ceil(P2/4)) floats
@@ -230,7 +263,7 @@ __device__ __forceinline__ void box_center(uint32_t key, int lev, double xmin, d
 //       imaginary), M_n^m = sum q j^_n(k rho) (rho/rb)^n conj(Y_n^m)
 //       (lrb = k rb)
 //   Box b's multipole starts at M + b * mstride + moff.
-template <int P, typename T, int YK = 0, int HALF = 0>
+template <int P, typename T, int YK = 0, int HALF = 0, int CONV = -1, int GN = -1>
 __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
                                             const float* __restrict__ y, const float* __restrict__ z,
@@ -259,9 +292,48 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
     if (__ballot(i0 + base <= i1) == 0) break;  // one wave per block: uniform
     const int i = i0 + base + sub;
     T* row = red[lane];
-    if (i <= i1) {
+    if (YK == 0 && CONV == FMMX_CONV_EXACT && GN == 0 && i <= i1) {
+      // charges in the EXACT convention (unit table ratios): the harmonics
+      // column by column straight into the LDS row; the unrolled recurrence
+      // uses literal coefficients and keeps only its state live
       double ux, uy, uz;
-      eff_point(conv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
+      eff_point(FMMX_CONV_EXACT, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
+      const T tx = T(ux), ty = T(uy), tz = T(uz), r2 = tx * tx + ty * ty + tz * tz, qi = T(q[i]);
+      constexpr HarmCoef hc = make_coef();
+      auto put = [&](int n, int m, T sr, T si) {
+        row[sl(n, m)] = qi * sr;
+        if (m > 0) row[NR + n * (n - 1) / 2 + m - 1] = -qi * si;
+      };
+      T dr = T(1), di = T(0);
+#pragma unroll
+      for (int m = 0; m < P; m++) {
+        if (m > 0) {
+          const T a = -T(hc.a[m]), nr = a * (tx * dr - ty * di), ni = a * (tx * di + ty * dr);
+          dr = nr;
+          di = ni;
+        }
+        put(m, m, dr, di);
+        if (m + 1 < P) {
+          const T bz = T(hc.b[m]) * tz;
+          T pr = dr, pi = di, sr = bz * dr, si = bz * di;
+          put(m + 1, m, sr, si);
+#pragma unroll
+          for (int n = m + 2; n < P; n++) {
+            const T cz = T(hc.c[n - 1][m]) * tz, d = T(hc.d[n - 1][m]) * r2;
+            const T nr = cz * sr - d * pr, ni = cz * si - d * pi;
+            pr = sr;
+            pi = si;
+            sr = nr;
+            si = ni;
+            put(n, m, sr, si);
+          }
+        }
+      }
+    } else if (i <= i1) {
+      const int cv = CONV >= 0 ? CONV : conv;
+      const int gnv = GN >= 0 ? GN : gn;
+      double ux, uy, uz;
+      eff_point(cv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
       T Sr[NR], Si[NR];
       solid_harmonics<P, T>(T(ux), T(uy), T(uz), Sr, Si);
       if (YK) {
@@ -283,7 +355,7 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
             if (m > 0) row[NR + n * (n - 1) / 2 + m - 1] = -w * Si[sl(n, m)];
           }
         }
-      } else if (!gn) {
+      } else if (!gnv) {
         const T qi = T(q[i]);
 #pragma unroll
         for (int n = 0; n < P; n++)
@@ -351,7 +423,7 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
 //   YK 2: one part (Re or Im of the field) of Helmholtz locals,
 //       phi = c4/rb sum_n j^_n(k r) sum_m L S_n^m(u), potential only
 //   Box b's local starts at L + b * lstride + loff.
-template <int P, typename T, int YK = 0, int HALF = 0>
+template <int P, typename T, int YK = 0, int HALF = 0, int CONV = -1>
 __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__ bkey, const int* __restrict__ bfirst,
                                             const int* __restrict__ blast, const float* __restrict__ x,
                                             const float* __restrict__ y, const float* __restrict__ z,
@@ -380,6 +452,122 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
     Li[s] = m == 0 ? T(0) : T(w * (double)L[(size_t)b * lstride + loff + NR + n * (n - 1) / 2 + m - 1]);
   }
   __syncthreads();
+  if constexpr (YK == 0) {
+    // The gradient of sum L_n^m S_n^m is a sum over S_{n-1}^{m'} with
+    // coefficients linear in L (the derivative rules of the harmonics above),
+    // so per leaf the table C[s] = (L, X/2, Y/2, Z) is built once and each
+    // target streams its harmonics column by column against it, keeping only
+    // the recurrence state live:
+    //   gx = Re sum S X/2, gy = Im sum S Y/2, gz = Re sum S Z,
+    //   X = gp(n,m-1) L_n^{m-1} - gm(n,m+1) L_n^{m+1} + [m=1] gp(n,0) conj(L_n^0),
+    //   Y = gp(n,m-1) L_n^{m-1} + gm(n,m+1) L_n^{m+1} + [m=1] gp(n,0) conj(L_n^0),
+    //   Z = gz(n,m) L_n^m, with n one above the degree of S.
+    __shared__ __align__(16) T Cs[NL][NR][8];
+    for (int s = lane; valid && s < NR; s += STEP) {
+      int n = 0;
+      while (sl(n + 1, 0) <= s) n++;
+      const int m = s - sl(n, 0);
+      T* c = Cs[h][s];
+      c[0] = Lr[s];
+      c[1] = Li[s];
+      T xr = 0, xi = 0, yr = 0, yi = 0, zr = 0, zi = 0;
+      if (n <= P - 2) {
+        const int nu = n + 1;
+        if (m >= 1) {
+          const T g = T(c_hc.gp[nu][m - 1]), ar = g * Lr[sl(nu, m - 1)], ai = g * Li[sl(nu, m - 1)];
+          xr += ar;
+          xi += ai;
+          yr += ar;
+          yi += ai;
+        }
+        {
+          const T g = T(c_hc.gm[nu][m + 1]), ar = g * Lr[sl(nu, m + 1)], ai = g * Li[sl(nu, m + 1)];
+          xr -= ar;
+          xi -= ai;
+          yr += ar;
+          yi += ai;
+        }
+        if (m == 1) {
+          const T g = T(c_hc.gp[nu][0]), ar = g * Lr[sl(nu, 0)], ai = -g * Li[sl(nu, 0)];
+          xr += ar;
+          xi += ai;
+          yr += ar;
+          yi += ai;
+        }
+        const T g = T(c_hc.gz[nu][m]);
+        zr = g * Lr[sl(nu, m)];
+        zi = g * Li[sl(nu, m)];
+      }
+      c[2] = T(0.5) * xr;
+      c[3] = T(0.5) * xi;
+      c[4] = T(0.5) * yi;
+      c[5] = T(0.5) * yr;
+      c[6] = zr;
+      c[7] = zi;
+    }
+    __syncthreads();
+    if (!valid) return;
+    double cx, cy, cz;
+    box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
+    const double rbinv = 1.0 / rb;
+    const int i0 = bfirst[b], i1 = blast[b];
+    // a compile-time convention keeps the REF angle code (double acos,
+    // atan, sincos) out of the EXACT kernel's register budget
+    const int cv = CONV >= 0 ? CONV : conv;
+    for (int i = i0 + lane; i <= i1; i += STEP) {
+      double ux, uy, uz;
+      eff_point(cv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
+      const T tx = T(ux), ty = T(uy), tz = T(uz), r2 = tx * tx + ty * ty + tz * tz;
+      T phi = T(0), gx = T(0), gy = T(0), gz = T(0);
+      // the table holds zero gradient entries at degree p-1
+      auto term = [&](T sr, T si, int s) {
+        const T* c = Cs[h][s];
+        phi += sr * c[0] - si * c[1];
+        gx += sr * c[2] - si * c[3];
+        gy += sr * c[4] + si * c[5];
+        gz += sr * c[6] - si * c[7];
+      };
+      // harmonics column by column (runtime loops: only the recurrence state
+      // is live, and the table reads cannot be hoisted out of the target loop)
+      T dr = T(1), di = T(0);  // S_m^m
+#pragma unroll 1
+      for (int m = 0; m < P; m++) {
+        if (m > 0) {
+          const T a = -HC<T>::a(m), nr = a * (tx * dr - ty * di), ni = a * (tx * di + ty * dr);
+          dr = nr;
+          di = ni;
+        }
+        term(dr, di, sl(m, m));
+        if (m + 1 < P) {
+          const T bz = HC<T>::b(m) * tz;
+          T pr = dr, pi = di, sr = bz * dr, si = bz * di;  // S_{n-1}^m, S_n^m
+          term(sr, si, sl(m + 1, m));
+#pragma unroll 1
+          for (int n = m + 2; n < P; n++) {
+            const T cz = HC<T>::c(n - 1, m) * tz, d = HC<T>::d(n - 1, m) * r2;
+            const T nr = cz * sr - d * pr, ni = cz * si - d * pi;
+            pr = sr;
+            pi = si;
+            sr = nr;
+            si = ni;
+            term(sr, si, sl(n, m));
+          }
+        }
+      }
+      const double s1 = c4 * rbinv, s2 = c4 * rbinv * rbinv;
+      if (gn) {
+        vs[i] -= s2 * ((double)nx[i] * (double)gx + (double)ny[i] * (double)gy + (double)nz[i] * (double)gz);
+      } else {
+        vs[i] += s1 * (double)phi;
+        if (grad) {
+          gxs[i] += s2 * (double)gx;
+          gys[i] += s2 * (double)gy;
+          gzs[i] += s2 * (double)gz;
+        }
+      }
+    }
+    return;
+  }
   if (!valid) return;
   double cx, cy, cz;
   box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
@@ -581,16 +769,7 @@ __global__ __launch_bounds__(TR_NT) void k_translate(int nout, const int* __rest
   }
 }
 
-// ---------------------------------------------------------------------
-// M2L on MFMA.  A workgroup (4 waves) owns a tile of 64 targets of one
-// parity class at a level; all of them see the same 189 lattice offsets
-// (fmm/ijbox.cxx:366-401).  Per offset the operator (p^2 x p^2, stored by the
-// host in MFMA fragment order) is staged once in LDS and each wave runs
-// ceil(p^2/16) x ceil(p^2/4) v_mfma 16x16x4 steps for its 16 targets, the
-// source multipoles (gathered through the dense key->box table) feeding the
-// B operand straight from registers.  With nsplit > 1 the offsets are split
-// over workgroups and the partial sums reduced by k_m2l_reduce in fixed
-// order (deterministic).
+// v_mfma 16x16x4 in FP32 and FP64
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 template <typename T>
@@ -610,6 +789,132 @@ struct Mfma<double> {
   }
 };
 
+// ---------------------------------------------------------------------
+// M2M and L2L of the FP32 engine on MFMA (v_mfma_f32_16x16x4f32), operators
+// in the fragment order of the M2L kernel ([ks][mt][lane] holds row
+// 16 mt + lane % 16, column (lane / 16) KS + ks of the p^2 x p^2 operator).
+//   M2M: a workgroup of 8 waves owns 16 parent boxes; wave w multiplies the
+//        octant-w operator with the 16 children of octant w (zero where a
+//        parent has none) and the 8 products are summed in LDS in octant
+//        order, so the result does not depend on scheduling
+//        (kernel/m2m.cxx:115-155).
+//   L2L: a wave owns 16 children of one parity class, which share one octant
+//        and so one operator; each child's locals = operator x its parent's
+//        (kernel/l2l.cxx:203-237).  The children's locals are written, not
+//        added to: L2L initialises a level below the first M2L level.
+template <int P, int L2L>
+__global__ __launch_bounds__(L2L ? 256 : 512) void k_translate_mfma(int nout, const int* __restrict__ cfirst,
+                                                                    const uint32_t* __restrict__ key,
+                                                                    const int* __restrict__ parent,
+                                                                    const int* __restrict__ cls_cnt,
+                                                                    const uint32_t* __restrict__ ord,
+                                                                    const float* __restrict__ in,
+                                                                    float* __restrict__ out,
+                                                                    const float* __restrict__ opF) {
+  using ML = MLayout<P>;
+  constexpr int P2 = P * P, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, FR = KS * MT * 64;
+  using V4 = f32x4;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, j = l & 15, kq = l >> 4;
+  int dst = -1, src = -1, oct = 0;
+  if (L2L) {
+    // wave -> (class, tile) over the class-sorted child list
+    int t = blockIdx.x * 4 + w, start = 0, c = 0;
+    for (; c < 8; c++) {
+      const int cnt = cls_cnt[c], nt = (cnt + 15) / 16;
+      if (t < nt) break;
+      t -= nt;
+      start += cnt;
+    }
+    if (c == 8) return;  // no barrier below in this mode
+    // class bits (x, y, z) -> octant bits (y | x << 1 | z << 2)
+    oct = ((c >> 1) & 1) | ((c & 1) << 1) | (c & 4);
+    const int i = t * 16 + j;
+    if (i < cls_cnt[c]) {
+      dst = (int)ord[start + i];
+      src = parent[dst];
+    }
+  } else {
+    oct = w;
+    const int b = blockIdx.x * 16 + j;
+    if (b < nout) {
+      dst = b;
+      for (int c = cfirst[b]; c < cfirst[b + 1]; c++)
+        if ((int)(key[c] & 7) == oct) src = c;
+    }
+  }
+  V4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) acc[mt] = V4{0.f, 0.f, 0.f, 0.f};
+  if (__ballot(src >= 0) != 0) {
+    float bv[KS];
+    if (L2L) {
+      const float* ps = in + (size_t)(src >= 0 ? src : 0) * P2 + kq * KS;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) bv[ks] = (src >= 0 && kq * KS + ks < P2) ? ps[ks] : 0.f;
+    } else {
+      const float4* ms = reinterpret_cast<const float4*>(in + (size_t)(src >= 0 ? src : 0) * ML::VS + kq * ML::KSP);
+#pragma unroll
+      for (int v4 = 0; v4 < ML::KSP / 4; v4++) {
+        const float4 m4 = src >= 0 ? ms[v4] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (4 * v4 + 0 < KS) bv[4 * v4 + 0] = (kq * KS + 4 * v4 + 0 < P2) ? m4.x : 0.f;
+        if (4 * v4 + 1 < KS) bv[4 * v4 + 1] = (kq * KS + 4 * v4 + 1 < P2) ? m4.y : 0.f;
+        if (4 * v4 + 2 < KS) bv[4 * v4 + 2] = (kq * KS + 4 * v4 + 2 < P2) ? m4.z : 0.f;
+        if (4 * v4 + 3 < KS) bv[4 * v4 + 3] = (kq * KS + 4 * v4 + 3 < P2) ? m4.w : 0.f;
+      }
+    }
+    const float* A = opF + (size_t)oct * FR + l;
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      float af[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) af[mt] = A[(ks * MT + mt) * 64];
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) acc[mt] = Mfma<float>::fma(af[mt], bv[ks], acc[mt]);
+    }
+  }
+  if (L2L) {
+    if (dst < 0) return;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * mt + 4 * kq + r;
+        if (i < P2) out[(size_t)dst * P2 + i] = acc[mt][r];
+      }
+    return;
+  }
+  // M2M: the 8 octant products through LDS, summed in octant order
+  __shared__ float red[8][MT * 4 * 64];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) red[w][(mt * 4 + r) * 64 + l] = acc[mt][r];
+  __syncthreads();
+  for (int e = tid; e < 16 * ML::VS; e += 512) {
+    const int jb = e / ML::VS, pos = e % ML::VS, b = blockIdx.x * 16 + jb;
+    if (b >= nout) continue;
+    // padded position -> slot (pads are written as zeros)
+    const int seg = pos / ML::KSP, ks = pos % ML::KSP, i = seg * KS + ks;
+    float v = 0.f;
+    if (ks < KS && i < P2) {
+      const int q = ((i >> 4) * 4 + (i & 3)) * 64 + ((i >> 2) & 3) * 16 + jb;
+#pragma unroll
+      for (int o = 0; o < 8; o++) v += red[o][q];
+    }
+    out[(size_t)b * ML::VS + pos] = v;
+  }
+}
+
+// ---------------------------------------------------------------------
+// M2L on MFMA.  A workgroup (4 waves) owns a tile of 64 targets of one
+// parity class at a level; all of them see the same 189 lattice offsets
+// (fmm/ijbox.cxx:366-401).  Per offset the operator (p^2 x p^2, stored by the
+// host in MFMA fragment order) is staged once in LDS and each wave runs
+// ceil(p^2/16) x ceil(p^2/4) v_mfma 16x16x4 steps for its 16 targets, the
+// source multipoles (gathered through the dense key->box table) feeding the
+// B operand straight from registers.  With nsplit > 1 the offsets are split
+// over workgroups and the partial sums reduced by k_m2l_reduce in fixed
+// order (deterministic).
 constexpr int M2L_TN = 64;  // targets per workgroup of the FP64 vector kernel
 
 // FP32 MFMA kernel: NT N-tiles (16 targets each) per wave, 4 waves, so a
@@ -1399,12 +1704,24 @@ int Engine::run_p2m(int neq) {
     const double lrb = (hh ? hk : lam) * rb(lmax);
     // two leaves per wave when leaves hold at most about 32 particles
     const bool half = (double)sj.n / nb <= 32.0;
-#define FMMX_P2M_H(T_, YK_, H_, Q_, STRIDE_, OFF_)                                                                 \
-  k_p2m<P, T_, YK_, H_><<<H_ ? (nb + 1) / 2 : nb, 64, 0, stream>>>(                                               \
+#define FMMX_P2M_HC(T_, YK_, H_, C_, G_, Q_, STRIDE_, OFF_)                                                       \
+  k_p2m<P, T_, YK_, H_, C_, G_><<<H_ ? (nb + 1) / 2 : nb, 64, 0, stream>>>(                                       \
       nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(), sj.blast.as<int>(), sj.x.as<float>(),         \
       sj.y.as<float>(), sj.z.as<float>(), sj.nx.as<float>(), sj.ny.as<float>(), sj.nz.as<float>(), Q_,            \
       neq == FMMX_GNJ, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, mexp[lmax].as<T_>(), lrb, STRIDE_, \
       OFF_)
+    // Laplace charges in the EXACT convention: the streaming form
+#define FMMX_P2M_H(T_, YK_, H_, Q_, STRIDE_, OFF_)                                                 \
+  do {                                                                                            \
+    if constexpr (YK_ == 0) {                                                                     \
+      if (neq != FMMX_GNJ && cfg.convention == FMMX_CONV_EXACT)                                   \
+        FMMX_P2M_HC(T_, 0, H_, FMMX_CONV_EXACT, 0, Q_, STRIDE_, OFF_);                            \
+      else                                                                                        \
+        FMMX_P2M_HC(T_, 0, H_, -1, -1, Q_, STRIDE_, OFF_);                                        \
+    } else {                                                                                      \
+      FMMX_P2M_HC(T_, YK_, H_, -1, -1, Q_, STRIDE_, OFF_);                                        \
+    }                                                                                             \
+  } while (0)
 #define FMMX_P2M(T_, YK_, Q_, STRIDE_, OFF_)                 \
   do {                                                       \
     if (half) FMMX_P2M_H(T_, YK_, 1, Q_, STRIDE_, OFF_);     \
@@ -1432,6 +1749,7 @@ int Engine::run_p2m(int neq) {
     }
 #undef FMMX_P2M
 #undef FMMX_P2M_H
+#undef FMMX_P2M_HC
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
@@ -1448,12 +1766,20 @@ int Engine::run_l2p(int neq) {
     const int grad = neq == FMMX_G || neq == FMMX_YUKAWA;
     // two leaves per wave when leaves hold at most about 32 targets
     const bool half = (double)si.n / nb <= 32.0;
-#define FMMX_L2P_H(T_, YK_, H_, V_, STRIDE_, OFF_)                                                                \
-  k_l2p<P, T_, YK_, H_><<<H_ ? (nb + 1) / 2 : nb, 64, 0, stream>>>(                                              \
+#define FMMX_L2P_HC(T_, YK_, H_, C_, V_, STRIDE_, OFF_)                                                           \
+  k_l2p<P, T_, YK_, H_, C_><<<H_ ? (nb + 1) / 2 : nb, 64, 0, stream>>>(                                          \
       nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(), si.blast.as<int>(), si.x.as<float>(),        \
       si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),               \
       lexp[lmax].as<T_>(), neq == FMMX_GNI, grad, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, cl, V_, \
       gxs.as<double>(), gys.as<double>(), gzs.as<double>(), lrb, STRIDE_, OFF_)
+    // Laplace: the convention as a template argument
+#define FMMX_L2P_H(T_, YK_, H_, V_, STRIDE_, OFF_)                                          \
+  do {                                                                                     \
+    if constexpr (YK_ != 0) FMMX_L2P_HC(T_, YK_, H_, -1, V_, STRIDE_, OFF_);               \
+    else if (cfg.convention == FMMX_CONV_REF)                                              \
+      FMMX_L2P_HC(T_, 0, H_, FMMX_CONV_REF, V_, STRIDE_, OFF_);                            \
+    else FMMX_L2P_HC(T_, 0, H_, FMMX_CONV_EXACT, V_, STRIDE_, OFF_);                       \
+  } while (0)
 #define FMMX_L2P(T_, YK_, V_, STRIDE_, OFF_)               \
   do {                                                     \
     if (half) FMMX_L2P_H(T_, YK_, 1, V_, STRIDE_, OFF_);   \
@@ -1481,6 +1807,7 @@ int Engine::run_l2p(int neq) {
     }
 #undef FMMX_L2P
 #undef FMMX_L2P_H
+#undef FMMX_L2P_HC
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
@@ -1489,6 +1816,16 @@ int Engine::run_l2p(int neq) {
 int Engine::run_m2m(int lev) {
   const int nbp = sj.nb[lev];
   if (nbp == 0) return FMMX_OK;
+  if (!fp64 && !hh) {
+    return dispatch_p(p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      k_translate_mfma<P, 0><<<(nbp + 15) / 16, 512, 0, stream>>>(
+          nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
+          mexp[lev + 1].as<float>(), mexp[lev].as<float>(), m2m_ops<float>(lev));
+      FMMX_HIP(hipGetLastError());
+      return FMMX_OK;
+    });
+  }
   const dim3 grid((nbp + TR_TB - 1) / TR_TB, 8);
   const size_t es = fp64 ? 8 : 4;
   const long n = (long)nbp * VS * parts();
@@ -1537,6 +1874,17 @@ int Engine::run_m2m(int lev) {
 int Engine::run_l2l(int lev) {
   const int nbc = si.nb[lev];
   if (nbc == 0) return FMMX_OK;
+  if (!fp64 && !hh) {
+    // one wave per 16 children of a class: at most nbc / 16 + 8 waves
+    return dispatch_p(p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      k_translate_mfma<P, 1><<<(nbc / 16 + 8 + 3) / 4, 256, 0, stream>>>(
+          nbc, nullptr, nullptr, si.parent[lev].as<int>(), cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(),
+          lexp[lev - 1].as<float>(), lexp[lev].as<float>(), l2l_ops<float>(lev));
+      FMMX_HIP(hipGetLastError());
+      return FMMX_OK;
+    });
+  }
   const int grid = (nbc + TR_TB - 1) / TR_TB + 8;
   if (hh) {
     return dispatch_hp(p, [&](auto pc) -> int {
