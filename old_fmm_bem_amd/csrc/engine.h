@@ -98,7 +98,8 @@ struct Engine {
   // far lists per level (only with FMMX_KEEP_LISTS): rows of NEBM
   std::array<DevBuf, MAXL + 1> fnij, fneij;
   // M2L target ordering by parity class and class counts per level
-  std::array<DevBuf, MAXL + 1> cls_key, cls_key2, cls_ord, cls_ord2, cls_cnt;
+  std::array<DevBuf, MAXL + 1> cls_ord, cls_cnt;
+  DevBuf cls_chunks;  // class counts per chunk of boxes
   int cls_cnt_h[MAXL + 1][8] = {};  // host copy of the class counts
   // operators (precision of the evaluation): M2M[8], L2L[8], M2L[levels][NOFF]
   DevBuf op_m2m, op_l2l, op_m2l, m2l_part, m2m_part;  // op_m2l in MFMA fragment order

@@ -171,18 +171,129 @@ __global__ void k_far_list(int nbi, const uint32_t* __restrict__ ikey, int nbj, 
   nij[b] = c;
 }
 
-// parity class (x&1, y&1, z&1) of each target box: M2L work is grouped by
-// class so that every target of a tile shares the same 189 offsets
-__global__ void k_class(int nb, const uint32_t* __restrict__ bkey, int lev, uint32_t* __restrict__ ck,
-                        uint32_t* __restrict__ ord, int* __restrict__ cnt) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  int x, y, z;
-  morton_decode(bkey[b], lev, x, y, z);
-  uint32_t c = (uint32_t)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
-  ck[b] = c;
-  ord[b] = (uint32_t)b;
-  atomicAdd(&cnt[c], 1);
+// Parity class (x & 1 | (y & 1) << 1 | (z & 1) << 2) of each target box:
+// M2L and L2L work is grouped by class, so that every target of a tile shares
+// the same 189 offsets (and one L2L octant).  The class order is a stable
+// partition of the boxes of each level (box order kept within a class), built
+// for all levels in two launches over chunks of CL_CH boxes: class counts per
+// chunk, then each box's position from the counts of the chunks before it and
+// a block scan inside its chunk.  Counts of a thread's CL_PER boxes are packed
+// as 16-bit fields, two classes per 32-bit word.
+constexpr int CL_T = 256, CL_PER = 4, CL_CH = CL_T * CL_PER;
+struct ClassLevels {
+  const uint32_t* key[MAXL + 1];
+  uint32_t* ord[MAXL + 1];
+  int* cnt[MAXL + 1];
+  int nb[MAXL + 1];
+  int chunk0[MAXL + 2];  // first chunk of each level
+  int lmin, lmax;
+  int* ccnt;  // 8 class counts per chunk
+};
+
+__device__ __forceinline__ void class_chunk(const ClassLevels& L, int& lev, int& ch) {
+  lev = L.lmin;
+  while (lev < L.lmax && (int)blockIdx.x >= L.chunk0[lev + 1]) lev++;
+  ch = (int)blockIdx.x - L.chunk0[lev];
+}
+
+// the classes of a thread's boxes (-1 past the end) and their packed counts
+__device__ __forceinline__ void class_load(const ClassLevels& L, int lev, int ch, int* c, uint32_t* w) {
+  const int b0 = ch * CL_CH + (int)threadIdx.x * CL_PER;
+  w[0] = w[1] = w[2] = w[3] = 0;
+#pragma unroll
+  for (int k = 0; k < CL_PER; k++) {
+    c[k] = -1;
+    if (b0 + k < L.nb[lev]) {
+      int x, y, z;
+      morton_decode(L.key[lev][b0 + k], lev, x, y, z);
+      c[k] = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+      w[c[k] >> 1] += 1u << (16 * (c[k] & 1));
+    }
+  }
+}
+
+// block-wide exclusive scan of the packed words; tot receives the block totals
+__device__ __forceinline__ void class_scan(uint32_t* w, uint32_t* tot) {
+  __shared__ uint32_t wsum[CL_T / 64][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t inc[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    inc[q] = w[q];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(inc[q], d, 64);
+      if (lane >= d) inc[q] += v;
+    }
+  }
+  if (lane == 63)
+#pragma unroll
+    for (int q = 0; q < 4; q++) wsum[wv][q] = inc[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    uint32_t before = 0, all = 0;
+    for (int v = 0; v < CL_T / 64; v++) {
+      if (v < wv) before += wsum[v][q];
+      all += wsum[v][q];
+    }
+    w[q] = before + inc[q] - w[q];
+    tot[q] = all;
+  }
+}
+
+__global__ __launch_bounds__(CL_T) void k_class_count(ClassLevels L) {
+  int lev, ch, c[CL_PER];
+  uint32_t w[4], tot[4];
+  class_chunk(L, lev, ch);
+  class_load(L, lev, ch, c, w);
+  class_scan(w, tot);
+  if (threadIdx.x < 8) L.ccnt[blockIdx.x * 8 + threadIdx.x] = (int)((tot[threadIdx.x >> 1] >> (16 * (threadIdx.x & 1))) & 0xffffu);
+}
+
+__global__ __launch_bounds__(CL_T) void k_class_place(ClassLevels L) {
+  int lev, ch, c[CL_PER];
+  uint32_t w[4], tot[4];
+  class_chunk(L, lev, ch);
+  class_load(L, lev, ch, c, w);
+  class_scan(w, tot);
+  // counts of the level's earlier chunks and of all its chunks, per class
+  __shared__ int pre[8], all[8];
+  if (threadIdx.x < 8) pre[threadIdx.x] = all[threadIdx.x] = 0;
+  __syncthreads();
+  const int nch = L.chunk0[lev + 1] - L.chunk0[lev];
+  int p8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = threadIdx.x; j < nch; j += CL_T) {
+    const int* cc = L.ccnt + (size_t)(L.chunk0[lev] + j) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      a8[q] += cc[q];
+      if (j < ch) p8[q] += cc[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    if (a8[q]) atomicAdd(&all[q], a8[q]);
+    if (p8[q]) atomicAdd(&pre[q], p8[q]);
+  }
+  __syncthreads();
+  if (ch == 0 && threadIdx.x < 8) L.cnt[lev][threadIdx.x] = all[threadIdx.x];
+  int base[8], s = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    base[q] = s + pre[q] + (int)((w[q >> 1] >> (16 * (q & 1))) & 0xffffu);
+    s += all[q];
+  }
+  const int b0 = ch * CL_CH + (int)threadIdx.x * CL_PER;
+#pragma unroll
+  for (int k = 0; k < CL_PER; k++) {
+    if (c[k] < 0) continue;
+    int pos = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (q == c[k]) pos = base[q]++;
+    L.ord[lev][pos] = (uint32_t)(b0 + k);
+  }
 }
 
 }  // namespace
@@ -380,27 +491,29 @@ int Engine::build_far_lists() {
 
 int Engine::build_m2l_order() {
   int rc;
+  ClassLevels L{};
+  L.lmin = 2;
+  L.lmax = lmax;
+  int nch = 0;
   for (int lev = 2; lev <= lmax; lev++) {
-    int nb = si.nb[lev];
+    const int nb = si.nb[lev];
     size_t nb1 = (size_t)(nb > 0 ? nb : 1);
-    if ((rc = cls_key[lev].ensure(nb1 * 4)) || (rc = cls_key2[lev].ensure(nb1 * 4)) ||
-        (rc = cls_ord[lev].ensure(nb1 * 4)) || (rc = cls_ord2[lev].ensure(nb1 * 4)) ||
-        (rc = cls_cnt[lev].ensure(8 * 4)))
-      return rc;
-    k_fill_int<<<1, 32, 0, stream>>>(8, 0, cls_cnt[lev].as<int>());
-    if (nb == 0) continue;
-    k_class<<<nblk(nb), TPB, 0, stream>>>(nb, si.bkey[lev].as<uint32_t>(), lev, cls_key2[lev].as<uint32_t>(),
-                                          cls_ord2[lev].as<uint32_t>(), cls_cnt[lev].as<int>());
-    size_t tb = 0;
-    FMMX_HIP(rocprim::radix_sort_pairs(nullptr, tb, cls_key2[lev].as<uint32_t>(), cls_key[lev].as<uint32_t>(),
-                                       cls_ord2[lev].as<uint32_t>(), cls_ord[lev].as<uint32_t>(), (size_t)nb, 0, 3,
-                                       stream));
-    if (tb > scratch.bytes && (rc = scratch.ensure(tb + 256))) return rc;
-    tb = scratch.bytes;
-    FMMX_HIP(rocprim::radix_sort_pairs(scratch.p, tb, cls_key2[lev].as<uint32_t>(), cls_key[lev].as<uint32_t>(),
-                                       cls_ord2[lev].as<uint32_t>(), cls_ord[lev].as<uint32_t>(), (size_t)nb, 0, 3,
-                                       stream));
+    if ((rc = cls_ord[lev].ensure(nb1 * 4)) || (rc = cls_cnt[lev].ensure(8 * 4))) return rc;
+    if (nb == 0) k_fill_int<<<1, 32, 0, stream>>>(8, 0, cls_cnt[lev].as<int>());
+    L.key[lev] = si.bkey[lev].as<uint32_t>();
+    L.ord[lev] = cls_ord[lev].as<uint32_t>();
+    L.cnt[lev] = cls_cnt[lev].as<int>();
+    L.nb[lev] = nb;
+    L.chunk0[lev] = nch;
+    nch += (nb + CL_CH - 1) / CL_CH;
   }
+  L.chunk0[lmax + 1] = nch;
+  if (nch == 0) return FMMX_OK;
+  if ((rc = cls_chunks.ensure((size_t)nch * 8 * 4))) return rc;
+  L.ccnt = cls_chunks.as<int>();
+  k_class_count<<<nch, CL_T, 0, stream>>>(L);
+  k_class_place<<<nch, CL_T, 0, stream>>>(L);
+  FMMX_HIP(hipGetLastError());
   return FMMX_OK;
 }
 
