@@ -13,6 +13,11 @@ namespace fmmx {
 
 namespace {
 
+// radix sorts always by onesweep passes (rocPRIM's default switches to a merge
+// sort up to 2^20 items, ten passes where the 3*lmax-bit keys need two)
+using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                           rocprim::default_config, 0>;
+
 constexpr int TPB = 256;
 inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
 
@@ -327,12 +332,12 @@ int Engine::sort_points(int n, const float* x, const float* y, const float* z, i
   k_keys<<<nblk(n), TPB, 0, stream>>>(n, d[0], d[1], d[2], cfg.xmin, cfg.ymin, cfg.zmin, rbinv, lm,
                                       srt_keys_in.as<uint32_t>(), srt_idx_in.as<uint32_t>());
   size_t tb = 0;
-  FMMX_HIP(rocprim::radix_sort_pairs(nullptr, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
+  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
                                      srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
                                      stream));
   if ((rc = scratch.ensure(tb + 256))) return rc;
   tb = scratch.bytes;
-  FMMX_HIP(rocprim::radix_sort_pairs(scratch.p, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
+  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(scratch.p, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
                                      srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
                                      stream));
   const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
@@ -397,7 +402,7 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rbinv, lmax, s.keys_in.as<uint32_t>(),
                                       s.idx_in.as<uint32_t>());
   size_t tb = 0;
-  FMMX_HIP(rocprim::radix_sort_pairs(nullptr, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
+  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
                                      s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
                                      stream));
   size_t sb = 0;
@@ -405,7 +410,7 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
                                    stream));
   if ((rc = scratch.ensure(std::max(tb, sb) + 256))) return rc;
   tb = scratch.bytes;
-  FMMX_HIP(rocprim::radix_sort_pairs(scratch.p, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
+  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(scratch.p, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
                                      s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
                                      stream));
   const uint32_t* perm = s.perm.as<uint32_t>();
