@@ -231,15 +231,19 @@ class FMM:
         for lev in range(lmax, 1, -1):
             if self.nboxes("i", lev) == 0:
                 continue
-            best, tbest = None, float("inf")
+            best, tbest, seen = None, float("inf"), set()
             for nt, st, wpg in shapes:
                 self.set_m2l_variant(lev, nt, st, wpg, 0)
                 base = self.m2l_variant(lev)[3]
                 for ns in sorted({base, max(1, base // 2), min(16, base * 2), max(1, base // 4), min(16, base * 4)}):
                     self.set_m2l_variant(lev, nt, st, wpg, ns)
+                    eff = tuple(self.m2l_variant(lev))  # the launch that runs (shapes can coincide)
+                    if eff in seen:
+                        continue
+                    seen.add(eff)
                     t = timed()
                     if t < tbest:
-                        best, tbest = (nt, st, wpg, ns), t
+                        best, tbest = eff, t
             self.set_m2l_variant(lev, *best)
         return {lev: self.m2l_variant(lev) for lev in range(2, lmax + 1)}
 

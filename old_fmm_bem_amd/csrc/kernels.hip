@@ -1927,6 +1927,8 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
   if (const char* e = getenv("FMMX_M2L_WPG")) v.wpg = atoi(e) == 4 ? 4 : atoi(e) == 16 ? 16 : 8;
   if (v.pa == 2 && (p > 11 || v.nt != 1)) v.pa = 0;  // two operator buffers fit LDS up to p = 11
   if (v.wpg == 16 && v.pa != 2) v.wpg = 8;
+  // reflection-canonical operators are signed while staged: no register prefetch form
+  if (v.pa == 1 && canon && !yk) v.pa = 0;
   if (const char* e = getenv("FMMX_M2L_SPLIT")) v.nsplit = atoi(e);
   if (v.nsplit <= 0) {
     // enough workgroups to cover the chip: split the 189 offsets when tiles are few
