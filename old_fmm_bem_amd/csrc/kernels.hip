@@ -1460,7 +1460,10 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
   }
 }
 
-template <typename T, int NEQ, int CONV, int BS>
+// TPT lanes per target (2: leaves of at most about 32 targets, one wave per
+// workgroup): the lanes of a target take alternating staged sources and add
+// their sums at the end, so a half-empty leaf does not idle half the wave.
+template <typename T, int NEQ, int CONV, int BS, int TPT = 1>
 __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ jkey,
                                             const int* __restrict__ ifirst, const int* __restrict__ ilast,
                                             const int* __restrict__ nij, const int* __restrict__ neij,
@@ -1481,11 +1484,13 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
   const T lam = T(lamd);
   __shared__ float4 sp[BS];
   __shared__ float4 sn[SN ? BS : 1];
-  const int b = blockIdx.x, tid = threadIdx.x;
+  static_assert(TPT == 1 || (TPT == 2 && BS == 64), "two lanes per target within one wave");
+  constexpr int TB = BS / TPT;  // targets per pass
+  const int b = blockIdx.x, tid = threadIdx.x, tt = tid % TB, part = tid / TB;
   if (b >= nbi) return;
   const int i0 = ifirst[b], i1 = ilast[b], nn = nij[b];
-  for (int base = i0; base <= i1; base += BS) {
-    const int i = base + tid;
+  for (int base = i0; base <= i1; base += TB) {
+    const int i = base + tt;
     const bool act = i <= i1;
     float px = 0, py = 0, pz = 0, qx = 0, qy = 0, qz = 0;
     if (act) {
@@ -1513,24 +1518,25 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
         __syncthreads();
         const int cnt = min(BS, j1 - jb + 1);
         T p = T(0), fx = T(0), fy = T(0), fz = T(0);
-        int s = 0;
+        int s = part;
         // coincident points can only share a box key; other box pairs skip the test
         if (CONV == FMMX_CONV_EXACT && !(jkey[jj] == ikey[b])) {
-          for (; s + 4 <= cnt; s += 4) {
+          for (; s + 3 * TPT < cnt; s += 4 * TPT) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
-              pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u], SN ? sn[s + u] : sp[0], p, fx, fy, fz,
-                                        lam);
+              pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u * TPT], SN ? sn[s + u * TPT] : sp[0], p,
+                                        fx, fy, fz, lam);
           }
-          for (; s < cnt; s++)
+          for (; s < cnt; s += TPT)
             pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
         } else {
-          for (; s + 4 <= cnt; s += 4) {
+          for (; s + 3 * TPT < cnt; s += 4 * TPT) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
-              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u], SN ? sn[s + u] : sp[0], p, fx, fy, fz, lam);
+              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u * TPT], SN ? sn[s + u * TPT] : sp[0], p, fx, fy,
+                                 fz, lam);
           }
-          for (; s < cnt; s++)
+          for (; s < cnt; s += TPT)
             pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
         }
         ap += (double)p;
@@ -1541,7 +1547,16 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
         }
       }
     }
-    if (act) {
+    if (TPT == 2) {
+      // the two lanes of a target are tid and tid + 32 of the one wave
+      ap += __shfl_xor(ap, 32, 64);
+      if (GRAD || HH) {
+        ax += __shfl_xor(ax, 32, 64);
+        ay += __shfl_xor(ay, 32, 64);
+        az += __shfl_xor(az, 32, 64);
+      }
+    }
+    if (act && part == 0) {
       vs[i] = c4 * ap;
       if (HH) gxs[i] = c4 * ax;
       if (GRAD) {
@@ -1662,10 +1677,11 @@ int Engine::run_p2p(int neq) {
   // Helmholtz: the imaginary charges ride in the source-normal slot, k in lambda's
   const float* nxj = neq == FMMX_HELMHOLTZ ? qs2.as<float>() : sj.nx.as<float>();
   const double kl = neq == FMMX_HELMHOLTZ ? hk : lam;
-  auto go = [&](auto tag_t, auto neq_c, auto conv_c, auto bs_c) -> int {
+  auto go = [&](auto tag_t, auto neq_c, auto conv_c, auto bs_c, auto tpt_c) -> int {
     using T = typename decltype(tag_t)::type;
     constexpr int NEQ = decltype(neq_c)::value, CONV = decltype(conv_c)::value, BS = decltype(bs_c)::value;
-    k_p2p<T, NEQ, CONV, BS><<<nbi, BS, 0, stream>>>(
+    constexpr int TPT = decltype(tpt_c)::value;
+    k_p2p<T, NEQ, CONV, BS, TPT><<<nbi, BS, 0, stream>>>(
         nbi, si.bkey[lmax].as<uint32_t>(), sj.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(), si.blast.as<int>(), nij.as<int>(), neij.as<int>(), sj.bfirst[lmax].as<int>(),
         sj.blast.as<int>(), si.x.as<float>(), si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(),
         si.nz.as<float>(), sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), qs.as<float>(), nxj,
@@ -1674,9 +1690,11 @@ int Engine::run_p2p(int neq) {
     return FMMX_OK;
   };
   auto by_bs = [&](auto t, auto nq, auto cv) -> int {
-    if (mean > 96) return go(t, nq, cv, std::integral_constant<int, 256>{});
-    if (mean > 40) return go(t, nq, cv, std::integral_constant<int, 128>{});
-    return go(t, nq, cv, std::integral_constant<int, 64>{});
+    using one = std::integral_constant<int, 1>;
+    if (mean > 96) return go(t, nq, cv, std::integral_constant<int, 256>{}, one{});
+    if (mean > 40) return go(t, nq, cv, std::integral_constant<int, 128>{}, one{});
+    if (mean > 32) return go(t, nq, cv, std::integral_constant<int, 64>{}, one{});
+    return go(t, nq, cv, std::integral_constant<int, 64>{}, std::integral_constant<int, 2>{});
   };
   auto by_conv = [&](auto t, auto nq) -> int {
     return ref ? by_bs(t, nq, std::integral_constant<int, FMMX_CONV_REF>{})
