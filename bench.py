@@ -46,11 +46,13 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profile(n, p, lmax):
-    """HBM bytes per leaf-level M2L launch measured with rocprofv3 PMC counters
-    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size), kept as a
-    text summary under profiles/; None when no summary matches this workload."""
-    path = os.path.join(ROOT, "profiles", "k_m2l_leaf_n1m_p10_pmc.txt")
+def traffic_from_profile(n, p, lmax, all_levels):
+    """HBM bytes per M2L launch (every level in one launch, or the leaf
+    level's) measured with rocprofv3 PMC counters (FETCH_SIZE + WRITE_SIZE,
+    scaled by a plain copy of known size), kept as a text summary under
+    profiles/; None when no summary matches this workload."""
+    name = "k_m2l_all_levels_n1m_p10_pmc.txt" if all_levels else "k_m2l_leaf_n1m_p10_pmc.txt"
+    path = os.path.join(ROOT, "profiles", name)
     if not (os.path.exists(path) and (n, p, lmax) == (1_000_000, 10, 5)):
         return None
     for line in open(path):
@@ -298,20 +300,23 @@ def main():
                 "phases_ms": {k: round(v_, 4) for k, v_ in ph64.items()}}
         f64.close()
 
-    # dominant kernel and its roofline position.  The leaf-level M2L launch
-    # (FP32 MFMA) carries most of the matvec's flops at lmax 5: algorithmic
-    # work 2 p^4 flop per M2L pair (dense p^2 x p^2 real operator), pairs
-    # counted by the kernel itself; duration from HIP events recorded on the
-    # engine's stream around that launch alone.
+    # dominant kernel and its roofline position.  The M2L launch (FP32 MFMA;
+    # one launch over every level, or the leaf level's) carries most of the
+    # matvec's flops: algorithmic work 2 p^4 flop per M2L pair (dense
+    # p^2 x p^2 real operator), pairs counted by the kernel itself; duration
+    # from HIP events recorded on the engine's stream around that launch alone.
     p2 = a.p * a.p
+    launch_pairs = st["m2l_pairs"] if st["m2l_all_levels"] else st["m2l_pairs_leaf"]
     if phases["ms_m2l"] >= phases["ms_p2p"]:
-        flops = 2.0 * p2 * p2 * st["m2l_pairs_leaf"]
+        flops = 2.0 * p2 * p2 * launch_pairs
         achieved = flops / (phases["ms_m2l_leaf"] * 1e-3) / 1e12
-        tr = traffic_from_profile(a.n, a.p, lmax)
+        tr = traffic_from_profile(a.n, a.p, lmax, st["m2l_all_levels"])
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma"], "unit": "TFLOP/s",
                 "frac": achieved / PEAK["fp32_mfma"], "traffic": tr,
-                "kernel": f"k_m2l, leaf level {lmax} (FP32 MFMA 16x16x4)", "ms_per_launch": phases["ms_m2l_leaf"],
-                "work": f"2*p^4 = {2 * p2 * p2} flop x {st['m2l_pairs_leaf']} M2L pairs per launch",
+                "kernel": (f"k_m2l, levels 2-{lmax} in one launch (FP32 MFMA 16x16x4)"
+                           if st["m2l_all_levels"] else f"k_m2l, leaf level {lmax} (FP32 MFMA 16x16x4)"),
+                "ms_per_launch": phases["ms_m2l_leaf"],
+                "work": f"2*p^4 = {2 * p2 * p2} flop x {launch_pairs} M2L pairs per launch",
                 "all_levels_tflops": 2.0 * p2 * p2 * st["m2l_pairs"] / (phases["ms_m2l"] * 1e-3) / 1e12}
     else:
         flops = 20.0 * st["p2p_pairs"]

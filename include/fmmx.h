@@ -193,8 +193,9 @@ typedef struct fmmx_stats {
   int64_t m2l_pairs;     /* number of (target box, source box) M2L pairs */
   int64_t m2l_tiles;     /* M2L work tiles launched */
   int64_t m2l_pairs_leaf; /* M2L pairs at the leaf level */
-  double ms_m2l_leaf;     /* duration of the leaf-level M2L kernel alone
-                             (per-phase timing only) */
+  double ms_m2l_leaf;     /* duration of the M2L kernel launch that covers
+                             the leaf level, alone (per-phase timing only;
+                             every level's when fmmx_get_count what 2 is 1) */
 } fmmx_stats;
 int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st);
 /* Enable/disable per-phase event timing (default off: one event pair). */
@@ -202,7 +203,8 @@ int fmmx_set_timing(fmmx_handle* h, int per_phase);
 
 /* ---- introspection of the tree for parity checks (host outputs) ---- */
 /* lmax in use and box counts: what = 0 lmax, 1 boxes of side at level lev
- * (side 0 = targets i, 1 = sources j) */
+ * (side 0 = targets i, 1 = sources j), 2: 1 if the last evaluate ran the
+ * M2L of every level in one launch (FP32 Laplace and Yukawa), else 0 */
 int fmmx_get_count(fmmx_handle* h, int what, int side, int lev);
 /* sort permutation: sorted position -> caller index (sort/sorti.cxx:39-41) */
 int fmmx_get_perm(fmmx_handle* h, int side, int* perm);

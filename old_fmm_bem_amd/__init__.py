@@ -412,7 +412,10 @@ class FMM:
     def stats(self):
         s = Stats()
         _check(lib().fmmx_get_stats(self.h, C.byref(s)))
-        return s.as_dict()
+        d = s.as_dict()
+        # whether ms_m2l_leaf timed one launch over every level
+        d["m2l_all_levels"] = lib().fmmx_get_count(self.h, 2, 0, 0) == 1
+        return d
 
     # -- introspection ----------------------------------------------------
     @property

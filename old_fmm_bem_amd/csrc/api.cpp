@@ -772,7 +772,15 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
       if ((rc = run_m2m(l))) return rc;
     if ((rc = mark())) return rc;  // ev5: after M2M
     const int l0 = first_m2l_level();
-    if (far && overlap && !hh && lmax > l0 && m2l_splits(lmax) > 1) {
+    last_m2l_all = far && m2l_all_levels();
+    if (last_m2l_all) {
+      // FP32 MFMA: every level's M2L in one launch (the levels above the leaf
+      // fill the leaf launch's tail), then the L2L chain adds each parent's
+      // locals; the same order with and without per-phase timing
+      if ((rc = run_m2l_all(l0))) return rc;
+      for (int l = l0 + 1; l <= lmax; l++)
+        if ((rc = run_l2l(l, true))) return rc;
+    } else if (far && overlap && !hh && lmax > l0 && m2l_splits(lmax) > 1) {
       // levels 2..lmax-1 (M2L, L2L) on the third stream while the leaf M2L
       // computes its partial sums; then the leaf L2L and the reduction that
       // adds the partials (the serialized order below, bit for bit)
@@ -1175,6 +1183,7 @@ int fmmx_get_count(fmmx_handle* h, int what, int side, int lev) {
   if (!h) return -1;
   Engine& e = h->e;
   if (what == 0) return e.lmax;
+  if (what == 2) return e.last_m2l_all ? 1 : 0;
   if (lev < 2 || lev > e.lmax) return -1;
   return side == 0 ? e.si.nb[lev] : e.sj.nb[lev];
 }

@@ -75,6 +75,8 @@ struct Side {
   std::array<int, MAXL + 1> nb{};  // host copy of box counts per level
 };
 
+struct M2LLev;  // one level of an M2L launch (kernels.hip)
+
 struct Engine {
   fmmx_config cfg{};
   int device = 0;
@@ -203,7 +205,15 @@ struct Engine {
   M2LVariant m2l_variant(int lev) const;
   int m2l_splits(int lev) const { return m2l_variant(lev).nsplit; }
   int run_m2l(int lev, bool init, int phase = 0);
-  int run_l2l(int lev);
+  int run_m2l_all(int l0);  // FP32 MFMA: every level in one launch
+  bool last_m2l_all = false;  // the last evaluation took that form
+  bool m2l_all_levels() const {
+    if (fp64 || hh) return false;
+    const char* e = getenv("FMMX_M2L_FUSED");
+    return !e || atoi(e) != 0;
+  }
+  M2LLev m2l_desc(int lev, int nt, int wpg, int nsplit, float* part, int init, int wg0);
+  int run_l2l(int lev, bool add = false);
   int run_l2p(int neq);
   int run_direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
                  const float* nyi, const float* nzi, int nj, const float* xj, const float* yj,

@@ -800,8 +800,8 @@ struct Mfma<double> {
 //        (kernel/m2m.cxx:115-155).
 //   L2L: a wave owns 16 children of one parity class, which share one octant
 //        and so one operator; each child's locals = operator x its parent's
-//        (kernel/l2l.cxx:203-237).  The children's locals are written, not
-//        added to: L2L initialises a level below the first M2L level.
+//        (kernel/l2l.cxx:203-237).  The children's locals are written, or
+//        added to when the level's M2L ran first (add).
 template <int P, int L2L>
 __global__ __launch_bounds__(L2L ? 256 : 512) void k_translate_mfma(int nout, const int* __restrict__ cfirst,
                                                                     const uint32_t* __restrict__ key,
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(L2L ? 256 : 512) void k_translate_mfma(int nout, co
                                                                     const uint32_t* __restrict__ ord,
                                                                     const float* __restrict__ in,
                                                                     float* __restrict__ out,
-                                                                    const float* __restrict__ opF) {
+                                                                    const float* __restrict__ opF, int add) {
   using ML = MLayout<P>;
   constexpr int P2 = P * P, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, FR = KS * MT * 64;
   using V4 = f32x4;
@@ -879,7 +879,10 @@ __global__ __launch_bounds__(L2L ? 256 : 512) void k_translate_mfma(int nout, co
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = 16 * mt + 4 * kq + r;
-        if (i < P2) out[(size_t)dst * P2 + i] = acc[mt][r];
+        if (i < P2) {
+          float* o = out + (size_t)dst * P2 + i;
+          *o = add ? *o + acc[mt][r] : acc[mt][r];
+        }
       }
     return;
   }
@@ -949,14 +952,50 @@ struct ReflMask {
   __host__ __device__ constexpr bool neg(int r, int c) const { return (mask[r][c >> 5] >> (c & 31)) & 1u; }
 };
 
+}  // namespace
+
+// One M2L launch can cover several levels: workgroups [wg0, wg0 + nwg) of
+// the grid run level `lev` with that level's tiles, splits and buffers.
+struct M2LLev {
+  const int* cls_cnt;
+  const uint32_t* ord;
+  const uint32_t* ikey;
+  const int* dense_j;
+  const float* M;
+  float* L;
+  float* part;
+  const float* opF;
+  const int4* offtab;
+  unsigned long long* npairs;
+  int lev, nb, nsplit, init, region, gsh, wg0, nwg;
+};
+struct M2LLaunch {
+  M2LLev lv[MAXL + 1];  // ascending wg0
+  int nlv;
+};
+
+namespace {
+
 template <int P, int NT, int PA, int WPG>
-__global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
-                                             const uint32_t* __restrict__ ord, const uint32_t* __restrict__ ikey,
-                                             const int* __restrict__ dense_j, const float* __restrict__ M,
-                                             float* __restrict__ L, float* __restrict__ part,
-                                             const float* __restrict__ opF, const int4* __restrict__ offtab,
-                                             int init, int region, int gsh,
-                                             unsigned long long* __restrict__ npairs) {
+__global__ __launch_bounds__(64 * WPG) void k_m2l(M2LLaunch A) {
+  // this workgroup's level (uniform selects: no dynamic indexing of the
+  // kernel arguments)
+  M2LLev Q = A.lv[0];
+#pragma unroll
+  for (int k = 1; k < MAXL + 1; k++)
+    if (k < A.nlv && (int)blockIdx.x >= A.lv[k].wg0) Q = A.lv[k];
+  const int lev = Q.lev, nb = Q.nb, nsplit = Q.nsplit, init = Q.init, region = Q.region, gsh = Q.gsh;
+  const int* __restrict__ cls_cnt = Q.cls_cnt;
+  const uint32_t* __restrict__ ord = Q.ord;
+  const uint32_t* __restrict__ ikey = Q.ikey;
+  const int* __restrict__ dense_j = Q.dense_j;
+  const float* __restrict__ M = Q.M;
+  float* __restrict__ L = Q.L;
+  float* __restrict__ part = Q.part;
+  const float* __restrict__ opF = Q.opF;
+  const int4* __restrict__ offtab = Q.offtab;
+  unsigned long long* __restrict__ npairs = Q.npairs;
+  const int bid = (int)blockIdx.x - Q.wg0, nwg = Q.nwg;
   using T = float;
   using ML = MLayout<P>;
   constexpr int P2 = P * P, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, FR = KS * MT * 64;
@@ -982,14 +1021,15 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(int lev, int nb, int nsplit, c
   // multipoles in its L2; items on an XCD go tile-major, then split, then
   // class.
   int cls0, split, tile;
+  // (level ranges start at multiples of 8, so bid % 8 is still the XCD)
   if (region) {
-    const int xcd = blockIdx.x & 7, item = blockIdx.x >> 3, tpr = (int)(gridDim.x >> 6) / nsplit;
+    const int xcd = bid & 7, item = bid >> 3, tpr = (nwg >> 6) / nsplit;
     cls0 = item & 7;
     split = (item >> 3) % nsplit;
     tile = xcd * tpr + (item >> 3) / nsplit;
   } else {
-    const int item = blockIdx.x >> 3, tpc = (int)(gridDim.x >> 3) / nsplit;
-    cls0 = blockIdx.x & 7;
+    const int item = bid >> 3, tpc = (nwg >> 3) / nsplit;
+    cls0 = bid & 7;
     split = item / tpc;
     tile = item % tpc;
   }
@@ -1839,7 +1879,7 @@ int Engine::run_m2m(int lev) {
       constexpr int P = decltype(pc)::value;
       k_translate_mfma<P, 0><<<(nbp + 15) / 16, 512, 0, stream>>>(
           nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
-          mexp[lev + 1].as<float>(), mexp[lev].as<float>(), m2m_ops<float>(lev));
+          mexp[lev + 1].as<float>(), mexp[lev].as<float>(), m2m_ops<float>(lev), 0);
       FMMX_HIP(hipGetLastError());
       return FMMX_OK;
     });
@@ -1889,7 +1929,7 @@ int Engine::run_m2m(int lev) {
   });
 }
 
-int Engine::run_l2l(int lev) {
+int Engine::run_l2l(int lev, bool add) {
   const int nbc = si.nb[lev];
   if (nbc == 0) return FMMX_OK;
   if (!fp64 && !hh) {
@@ -1898,11 +1938,12 @@ int Engine::run_l2l(int lev) {
       constexpr int P = decltype(pc)::value;
       k_translate_mfma<P, 1><<<(nbc / 16 + 8 + 3) / 4, 256, 0, stream>>>(
           nbc, nullptr, nullptr, si.parent[lev].as<int>(), cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(),
-          lexp[lev - 1].as<float>(), lexp[lev].as<float>(), l2l_ops<float>(lev));
+          lexp[lev - 1].as<float>(), lexp[lev].as<float>(), l2l_ops<float>(lev), add ? 1 : 0);
       FMMX_HIP(hipGetLastError());
       return FMMX_OK;
     });
   }
+  if (add) return fail(FMMX_ESTATE, "L2L add form is FP32 Laplace/Yukawa only");
   const int grid = (nbc + TR_TB - 1) / TR_TB + 8;
   if (hh) {
     return dispatch_hp(p, [&](auto pc) -> int {
@@ -1937,6 +1978,13 @@ int Engine::run_l2l(int lev) {
 // Offset splits of the M2L launch at a level (partial sums when > 1).
 Engine::M2LVariant Engine::m2l_variant(int lev) const {
   M2LVariant v = tune[lev];
+  if (lev != lmax && m2l_all_levels()) {
+    // one launch for all levels: the leaf level's shape, this level's splits
+    const M2LVariant lf = m2l_variant(lmax);
+    v.nt = lf.nt;
+    v.pa = lf.pa;
+    v.wpg = lf.wpg;
+  }
   if (v.nt <= 0) v.nt = 1;
   if (v.pa < 0) v.pa = 0;
   if (v.wpg <= 0) v.wpg = 8;
@@ -1966,6 +2014,110 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
 // phase 0: the M2L launch and (with splits) the reduction into the locals;
 // 1: the launch alone, partial sums only (needs m2l_splits(lev) > 1);
 // 2: the reduction alone, adding the partials of phase 1 to the locals
+// FP32 MFMA M2L: the launch parameters of one level, its workgroups from wg0
+// (grid = 8 classes x tiles x splits; a block of tiles per XCD once every XCD
+// gets at least one tile)
+M2LLev Engine::m2l_desc(int lev, int nt, int wpg, int nsplit, float* part, int init, int wg0) {
+  M2LLev d{};
+  const int MT = (P2 + 15) / 16, KS = (P2 + 3) / 4;
+  const size_t FR = (size_t)KS * MT * 64;
+  const size_t opoff = (yk || m2l_per_level) ? (size_t)(lev - 2) * NOFF * FR : 0;
+  // reflection-canonical operators (Laplace, EXACT) are signed while staged
+  const bool use_canon = canon && !yk;
+  int cmax = 0;
+  for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
+  const int tpc = std::max(1, (cmax + 16 * wpg * nt - 1) / (16 * wpg * nt));
+  int region = tpc >= 8 ? 1 : 0;
+  if (const char* e = getenv("FMMX_M2L_REGION")) region = region && atoi(e) != 0;
+  d.cls_cnt = cls_cnt[lev].as<int>();
+  d.ord = cls_ord[lev].as<uint32_t>();
+  d.ikey = si.bkey[lev].as<uint32_t>();
+  d.dense_j = sj.dense[lev].as<int>();
+  d.M = mexp[lev].as<float>();
+  d.L = lexp[lev].as<float>();
+  d.part = part;
+  d.opF = (yk ? yop_m2l : op_m2l).as<float>() + opoff;
+  d.offtab = (use_canon ? offtab_c : offtab).as<int4>();
+  d.npairs = pair_ctr.as<unsigned long long>() + 2 + lev;
+  d.lev = lev;
+  d.nb = si.nb[lev];
+  d.nsplit = nsplit;
+  d.init = init;
+  d.region = region;
+  d.gsh = group_shift(lev);
+  d.wg0 = wg0;
+  d.nwg = region ? 64 * ((tpc + 7) / 8) * nsplit : 8 * tpc * nsplit;
+  return d;
+}
+
+// the k_m2l instance of a launch shape (N-tiles per wave, operator staging,
+// waves per workgroup)
+template <int P>
+void launch_m2l(const M2LLaunch& A, int nwg, int nt, int pa, int wpg, hipStream_t stream) {
+#define FMMX_M2L_LAUNCH(NT_, PA_, W_) k_m2l<P, NT_, PA_, W_><<<nwg, 64 * W_, 0, stream>>>(A)
+  if (pa == 2 && P > 11) pa = 0;  // two operator buffers fit the 160 KB of LDS up to p = 11
+  if (pa == 2) {
+    if constexpr (P <= 11) {
+      if (wpg == 16) FMMX_M2L_LAUNCH(1, 2, 16);
+      else FMMX_M2L_LAUNCH(1, 2, 8);
+    }
+  } else if (wpg == 8) {
+    if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 8);
+    else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 8);
+    else FMMX_M2L_LAUNCH(2, 0, 8);
+  } else {
+    if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 4);
+    else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 4);
+    else FMMX_M2L_LAUNCH(2, 0, 4);
+  }
+#undef FMMX_M2L_LAUNCH
+}
+
+// FP32 MFMA M2L of all levels l0..lmax in one launch, in the leaf level's
+// launch shape with each level's offset splits: levels with splits write
+// partial sums (reduced afterwards in fixed order), the others write their
+// locals; the L2L chain then adds the parents' locals level by level.
+int Engine::run_m2l_all(int l0) {
+  const M2LVariant leaf = m2l_variant(lmax);
+  M2LLaunch A{};
+  size_t need = 0;
+  for (int lev = l0; lev <= lmax; lev++)
+    if (si.nb[lev] > 0 && sj.nb[lev] > 0 && m2l_splits(lev) > 1) need += (size_t)m2l_splits(lev) * si.nb[lev] * P2;
+  int rc = m2l_part.ensure(std::max<size_t>(need, 1) * 4);
+  if (rc) return rc;
+  size_t off = 0;
+  int wg = 0;
+  for (int lev = l0; lev <= lmax; lev++) {
+    const int nb = si.nb[lev];
+    if (nb == 0) continue;
+    const long nl = (long)nb * P2;
+    if (sj.nb[lev] == 0) {
+      k_zero_t<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, lexp[lev].as<float>());
+      continue;
+    }
+    const int ns = m2l_splits(lev);
+    A.lv[A.nlv] = m2l_desc(lev, leaf.nt, leaf.wpg, ns, ns > 1 ? m2l_part.as<float>() + off : nullptr, 1, wg);
+    wg += A.lv[A.nlv].nwg;
+    A.nlv++;
+    if (ns > 1) off += (size_t)ns * nl;
+  }
+  if (A.nlv == 0) return FMMX_OK;
+  return dispatch_p(p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    if (per_phase) FMMX_HIP(hipEventRecord(ev[8], stream));
+    launch_m2l<P>(A, wg, leaf.nt, leaf.pa, leaf.wpg, stream);
+    if (per_phase) FMMX_HIP(hipEventRecord(ev[9], stream));
+    for (int k = 0; k < A.nlv; k++) {
+      const M2LLev& d = A.lv[k];
+      const long nl = (long)d.nb * P2;
+      if (d.nsplit > 1)
+        k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, d.nsplit, d.part, d.L, 1);
+    }
+    FMMX_HIP(hipGetLastError());
+    return FMMX_OK;
+  });
+}
+
 int Engine::run_m2l(int lev, bool init, int phase) {
   const int nb = si.nb[lev];
   if (nb == 0) return FMMX_OK;
@@ -2046,45 +2198,13 @@ int Engine::run_m2l(int lev, bool init, int phase) {
         k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
                                                                           lexp[lev].as<double>(), init ? 1 : 0);
     } else {
-      // variant: N-tiles per wave, operator staging, waves per workgroup
       const M2LVariant var = m2l_variant(lev);
-      int nt = var.nt, pa = var.pa, wpg = var.wpg;
-      // reflection-canonical operators (Laplace, EXACT) are signed while staged
-      const bool use_canon = canon && !yk;
-      if (use_canon && pa == 1) pa = 0;
-      // per class at most ceil(max class count / TN) tiles; grid = 8 classes x tiles x splits
-      int cmax = 0;
-      for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
-      const int tpc = std::max(1, (cmax + 16 * wpg * nt - 1) / (16 * wpg * nt));
-      const int ns2 = nsplit;
-      // a block of tiles per XCD once every XCD gets at least one tile
-      int region = tpc >= 8 ? 1 : 0;
-      if (const char* e = getenv("FMMX_M2L_REGION")) region = region && atoi(e) != 0;
-      const int nwg = region ? 64 * ((tpc + 7) / 8) * ns2 : 8 * tpc * ns2;
-#define FMMX_M2L_LAUNCH(NT_, PA_, W_)                                                                    \
-  k_m2l<P, NT_, PA_, W_><<<nwg, 64 * W_, 0, stream>>>(                                                  \
-      lev, nb, ns2, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),   \
-      sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), part_buf->as<float>(),     \
-      opm2l.as<float>() + opoff, (use_canon ? offtab_c : offtab).as<int4>(), init ? 1 : 0, region, gsh, \
-      pair_ctr.as<unsigned long long>() + 2 + lev)
+      M2LLaunch A{};
+      A.nlv = 1;
+      A.lv[0] = m2l_desc(lev, var.nt, var.wpg, nsplit, part_buf->as<float>(), init ? 1 : 0, 0);
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
-      if (pa == 2 && P > 11) pa = 0;  // two operator buffers fit the 160 KB of LDS up to p = 11
-      if (pa == 2) {
-        if constexpr (P <= 11) {
-          if (wpg == 16) FMMX_M2L_LAUNCH(1, 2, 16);
-          else FMMX_M2L_LAUNCH(1, 2, 8);
-        }
-      } else if (wpg == 8) {
-        if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 8);
-        else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 8);
-        else FMMX_M2L_LAUNCH(2, 0, 8);
-      } else {
-        if (nt == 1 && !pa) FMMX_M2L_LAUNCH(1, 0, 4);
-        else if (nt == 1) FMMX_M2L_LAUNCH(1, 1, 4);
-        else FMMX_M2L_LAUNCH(2, 0, 4);
-      }
-#undef FMMX_M2L_LAUNCH
+      launch_m2l<P>(A, A.lv[0].nwg, var.nt, var.pa, var.wpg, stream);
       if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
       if (nsplit > 1 && phase == 0)
         k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<float>(),
