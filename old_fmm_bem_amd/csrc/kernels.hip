@@ -13,6 +13,7 @@
 //        (operator [p^2 x p^2] x gathered source multipoles [p^2 x tile]).
 //   direct sums kernel/G_dir.cxx, Gni_dir.cxx, Gnj_dir.cxx.
 #include <cstdlib>
+#include <cstring>
 
 #include "engine.h"
 #include "tree_util.h"
@@ -1996,6 +1997,14 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
   // reflection-canonical operators are signed while staged: no register prefetch form
   if (v.pa == 1 && canon && !yk) v.pa = 0;
   if (const char* e = getenv("FMMX_M2L_SPLIT")) v.nsplit = atoi(e);
+  if (const char* e = getenv("FMMX_M2L_SPLITS")) {
+    // per level, comma-separated from level 2
+    for (int l = 2; e && *e; l++) {
+      if (l == lev) v.nsplit = atoi(e);
+      e = strchr(e, ',');
+      if (e) e++;
+    }
+  }
   if (v.nsplit <= 0) {
     // enough workgroups to cover the chip: split the 189 offsets when tiles are few
     if (fp64 || hh) {
