@@ -737,6 +737,18 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   if (hh && (rc = upload_helmholtz_operators())) return rc;
   // Yukawa: no far field where lambda * (leaf box) makes it vanish in FP64
   const bool far = !yk || lam * rb(lmax) <= YUK_LRB_MAX;
+  // outputs in the caller's order (device pointers, or staging for host ones)
+  const bool g = neq == FMMX_G || neq == FMMX_YUKAWA;
+  double* outs[4] = {vi, g ? gxi : nullptr, g ? gyi : nullptr, g ? gzi : nullptr};
+  double* dsts[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int k = 0; !hh && k < 4; k++) {
+    if (!outs[k] || ni == 0) continue;
+    dsts[k] = outs[k];
+    if (!dev) {
+      if ((rc = out[k].ensure(ni1 * 8))) return rc;
+      dsts[k] = out[k].as<double>();
+    }
+  }
   int e = 2;
   auto mark = [&]() -> int {
     if (per_phase) FMMX_HIP(hipEventRecord(ev[e++], stream));
@@ -810,9 +822,11 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     }
     if ((rc = mark())) return rc;  // ev6: after M2L+L2L
     if (overlap) FMMX_HIP(hipStreamWaitEvent(stream, ev[11], 0));
-    if (far && (rc = run_l2p(neq))) return rc;
+    // Laplace and Yukawa: L2P writes the outputs in the caller's order itself
+    if (far && (rc = run_l2p(neq, hh ? nullptr : dsts))) return rc;
     if ((rc = mark())) return rc;  // ev7: after L2P
   }
+  const bool scattered = far && ni > 0 && !hh;
   // outputs in caller order
   if (hh) {
     if (ni > 0) {
@@ -829,19 +843,8 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     last_neq = neq;
     return FMMX_OK;
   }
-  const bool g = neq == FMMX_G || neq == FMMX_YUKAWA;
-  double* outs[4] = {vi, g ? gxi : nullptr, g ? gyi : nullptr, g ? gzi : nullptr};
   double* srcs[4] = {vs.as<double>(), gxs.as<double>(), gys.as<double>(), gzs.as<double>()};
-  double* dsts[4] = {nullptr, nullptr, nullptr, nullptr};
-  for (int k = 0; k < 4; k++) {
-    if (!outs[k] || ni == 0) continue;
-    dsts[k] = outs[k];
-    if (!dev) {
-      if ((rc = out[k].ensure(ni1 * 8))) return rc;
-      dsts[k] = out[k].as<double>();
-    }
-  }
-  if (ni > 0)
+  if (ni > 0 && !scattered)
     k_scatter4<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), srcs[0], srcs[1], srcs[2], srcs[3], dsts[0],
                                              dsts[1], dsts[2], dsts[3]);
   for (int k = 0; k < 4; k++)

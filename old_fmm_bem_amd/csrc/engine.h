@@ -214,7 +214,9 @@ struct Engine {
   }
   M2LLev m2l_desc(int lev, int nt, int wpg, int nsplit, float* part, int init, int wg0);
   int run_l2l(int lev, bool add = false);
-  int run_l2p(int neq);
+  // dst: outputs in the caller's order (v, gx, gy, gz; NULL entries skipped),
+  // written by L2P itself instead of a separate scatter
+  int run_l2p(int neq, double* const* dst = nullptr);
   int run_direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
                  const float* nyi, const float* nzi, int nj, const float* xj, const float* yj,
                  const float* zj, const float* nxj, const float* nyj, const float* nzj, const float* qj,

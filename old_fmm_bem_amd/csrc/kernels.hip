@@ -416,6 +416,45 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
   }
 }
 
+// The far-field result of target i (sorted position) added to its near-field
+// sums: stored back in place, or, with out.perm set, written straight to the
+// caller's order (out.v[perm[i]], ...; the gradient outputs may be NULL),
+// which saves the separate scatter pass over the outputs.
+struct L2POut {
+  const uint32_t* perm;
+  double *v, *gx, *gy, *gz;
+};
+__device__ __forceinline__ void l2p_store(const L2POut& out, int i, int gn, int grad, double s1, double s2,
+                                          double phi, double gx, double gy, double gz,
+                                          const float* __restrict__ nx, const float* __restrict__ ny,
+                                          const float* __restrict__ nz, double* __restrict__ vs,
+                                          double* __restrict__ gxs, double* __restrict__ gys,
+                                          double* __restrict__ gzs) {
+  if (gn) {
+    const double v = vs[i] - s2 * ((double)nx[i] * gx + (double)ny[i] * gy + (double)nz[i] * gz);
+    if (!out.perm) vs[i] = v;
+    else if (out.v) out.v[out.perm[i]] = v;
+    return;
+  }
+  const double v = vs[i] + s1 * phi;
+  if (!out.perm) {
+    vs[i] = v;
+    if (grad) {
+      gxs[i] += s2 * gx;
+      gys[i] += s2 * gy;
+      gzs[i] += s2 * gz;
+    }
+    return;
+  }
+  const uint32_t j = out.perm[i];
+  if (out.v) out.v[j] = v;
+  if (grad) {
+    if (out.gx) out.gx[j] = gxs[i] + s2 * gx;
+    if (out.gy) out.gy[j] = gys[i] + s2 * gy;
+    if (out.gz) out.gz[j] = gzs[i] + s2 * gz;
+  }
+}
+
 // ---------------------------------------------------------------------
 // L2P: one wave per target leaf box; adds to the P2P results (sorted order)
 //   YK 1: Yukawa locals, phi = sum_n h_n(r) sum_m L~ S_n^m(u) with
@@ -433,7 +472,8 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
                                             int conv, int lev, double xmin, double ymin, double zmin, double rb,
                                             const double* __restrict__ corr, double c4, double* __restrict__ vs,
                                             double* __restrict__ gxs, double* __restrict__ gys,
-                                            double* __restrict__ gzs, double lrb, int lstride, int loff) {
+                                            double* __restrict__ gzs, double lrb, int lstride, int loff,
+                                            L2POut out) {
   // HALF: two leaves per wave (32 lanes each), for leaves of at most about
   // 32 targets where a whole wave per leaf would idle half its lanes
   constexpr int NR = P * (P + 1) / 2, P2 = P * P, NL = HALF ? 2 : 1, STEP = HALF ? 32 : 64;
@@ -555,17 +595,8 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
           }
         }
       }
-      const double s1 = c4 * rbinv, s2 = c4 * rbinv * rbinv;
-      if (gn) {
-        vs[i] -= s2 * ((double)nx[i] * (double)gx + (double)ny[i] * (double)gy + (double)nz[i] * (double)gz);
-      } else {
-        vs[i] += s1 * (double)phi;
-        if (grad) {
-          gxs[i] += s2 * (double)gx;
-          gys[i] += s2 * (double)gy;
-          gzs[i] += s2 * (double)gz;
-        }
-      }
+      l2p_store(out, i, gn, grad, c4 * rbinv, c4 * rbinv * rbinv, (double)phi, (double)gx, (double)gy, (double)gz, nx, ny,
+                nz, vs, gxs, gys, gzs);
     }
     return;
   }
@@ -638,17 +669,8 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
           }
         }
     }
-    const double s1 = c4 * rbinv, s2 = c4 * rbinv * rbinv;
-    if (gn) {
-      vs[i] -= s2 * ((double)nx[i] * (double)gx + (double)ny[i] * (double)gy + (double)nz[i] * (double)gz);
-    } else {
-      vs[i] += s1 * (double)phi;
-      if (grad) {
-        gxs[i] += s2 * (double)gx;
-        gys[i] += s2 * (double)gy;
-        gzs[i] += s2 * (double)gz;
-      }
-    }
+    l2p_store(out, i, gn, grad, c4 * rbinv, c4 * rbinv * rbinv, (double)phi, (double)gx, (double)gy, (double)gz, nx, ny,
+              nz, vs, gxs, gys, gzs);
   }
 }
 
@@ -1814,9 +1836,12 @@ int Engine::run_p2m(int neq) {
   });
 }
 
-int Engine::run_l2p(int neq) {
+int Engine::run_l2p(int neq, double* const* dst) {
   const int nb = si.nb[lmax];
   if (nb == 0) return FMMX_OK;
+  // with dst (v, gx, gy, gz in the caller's order) the results go there
+  L2POut lout{};
+  if (dst && !hh) lout = L2POut{si.perm.as<uint32_t>(), dst[0], dst[1], dst[2], dst[3]};
   const double* corr = corr_d.as<double>();
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
@@ -1830,7 +1855,7 @@ int Engine::run_l2p(int neq) {
       nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(), si.blast.as<int>(), si.x.as<float>(),        \
       si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),               \
       lexp[lmax].as<T_>(), neq == FMMX_GNI, grad, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, cl, V_, \
-      gxs.as<double>(), gys.as<double>(), gzs.as<double>(), lrb, STRIDE_, OFF_)
+      gxs.as<double>(), gys.as<double>(), gzs.as<double>(), lrb, STRIDE_, OFF_, lout)
     // Laplace: the convention as a template argument
 #define FMMX_L2P_H(T_, YK_, H_, V_, STRIDE_, OFF_)                                          \
   do {                                                                                     \
