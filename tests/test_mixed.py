@@ -1,0 +1,89 @@
+"""The mixed-precision engine against the FP64 engine on the same inputs, over
+the configurations its FP32 kernels specialise on: every expansion order
+(MFMA tiles of p^2 = 4 .. 144 rows, padded multipole layouts, the streaming
+P2M/L2P forms), both conventions (reflection-canonical or per-level M2L
+operators), sparse and deep trees (the one-launch M2L over 1 to 5 levels, empty
+source levels), leaves small enough for two P2P lanes per target, and the edge
+cases of the FP64 tests.  The FP64 engine runs separate kernels (vector-FMA
+M2L, non-MFMA translations, one lane per target in P2P), so agreement to FP32
+rounding checks the FP32 paths (measured 0.6-4.8e-7, bound 2e-6); the FP64
+engine itself is checked against the oracle in test_gpu.py."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2, uniform_cube
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import old_fmm_bem_amd as F  # noqa: E402
+
+
+def _both(p, lmax, conv, x, q, xj=None, neq=F.G, nrm=None, **kw):
+    out = []
+    for prec in ("mixed", "fp64"):
+        f = F.FMM(p=p, lmax=lmax, precision=prec, convention=conv, box=(0, 0, 0, 1), **kw)
+        f.set_points(x, xj, ni_normals=nrm, nj_normals=nrm)
+        out.append(f.evaluate(q, neq=neq))
+        f.close()
+    return out
+
+
+@pytest.mark.parametrize("conv", ["exact", "ref"])
+@pytest.mark.parametrize("p", [2, 3, 5, 7, 9, 11, 12])
+def test_mixed_matches_fp64_every_order(p, conv):
+    x, q = uniform_cube(20000, 100 + p)
+    nrm = np.random.default_rng(p).standard_normal((20000, 3)).astype(np.float32)
+    for neq in (F.G, F.GNJ):
+        print(f"p={p} conv={conv} neq={neq}", flush=True)
+        (vm, gm), (vd, gd) = _both(p, 4, conv, x, q, neq=neq, nrm=nrm)
+        ev = rel_l2(vm, vd)
+        print(f"  potential {ev:.2e}", flush=True)
+        assert ev < 2e-6
+        if neq == F.G:
+            eg = rel_l2(gm, gd)
+            print(f"  gradient {eg:.2e}", flush=True)
+            assert eg < 2e-6
+
+
+@pytest.mark.parametrize("lmax", [2, 3, 5, 6])
+def test_mixed_matches_fp64_sparse_deep_trees(lmax):
+    """Points on a sphere: most boxes empty, upper levels with few tiles,
+    levels where a class has no boxes."""
+    rng = np.random.default_rng(40 + lmax)
+    n = 60000
+    s = rng.standard_normal((n, 3))
+    x = (0.5 + 0.45 * s / np.linalg.norm(s, axis=1, keepdims=True)).astype(np.float32)
+    q = (rng.random(n) * 2 - 1).astype(np.float32)
+    print(f"lmax={lmax}", flush=True)
+    (vm, gm), (vd, gd) = _both(8, lmax, "exact", x, q)
+    ev, eg = rel_l2(vm, vd), rel_l2(gm, gd)
+    print(f"  potential {ev:.2e} gradient {eg:.2e}", flush=True)
+    assert ev < 2e-6 and eg < 2e-6
+
+
+def test_mixed_edge_cases():
+    rng = np.random.default_rng(11)
+    xj, q = uniform_cube(3000, 51)
+    # one target, disjoint target and source sets
+    xi = np.array([[0.3, 0.4, 0.5]], np.float32)
+    (vm, _), (vd, _) = _both(6, 3, "exact", xi, q, xj=xj)
+    assert rel_l2(vm, vd) < 2e-6
+    xi2, _ = uniform_cube(2000, 52)
+    (vm, gm), (vd, gd) = _both(10, 4, "exact", xi2, q, xj=xj)
+    assert rel_l2(vm, vd) < 2e-6 and rel_l2(gm, gd) < 2e-6
+    # every point in one leaf box: near field only
+    xc = (0.01 + 0.02 * rng.random((300, 3))).astype(np.float32)
+    qc = rng.random(300).astype(np.float32)
+    (vm, _), (vd, _) = _both(6, 3, "exact", xc, qc)
+    assert rel_l2(vm, vd) < 1e-6
+    # leaves of ~100 points (wide P2P workgroups) at lmax 2
+    xw, qw = uniform_cube(8000, 53)
+    (vm, gm), (vd, gd) = _both(10, 2, "exact", xw, qw)
+    assert rel_l2(vm, vd) < 2e-6 and rel_l2(gm, gd) < 2e-6
+    # empty source set
+    f = F.FMM(p=6, lmax=3, precision="mixed", convention="exact", box=(0, 0, 0, 1))
+    f.set_points(xi, np.zeros((0, 3), np.float32))
+    v, g = f.evaluate(np.zeros(0, np.float32))
+    assert v.shape == (1,) and v[0] == 0 and np.all(g == 0)
+    f.close()
