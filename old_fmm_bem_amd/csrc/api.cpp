@@ -755,6 +755,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
     return FMMX_OK;
   };
   if (!capturing) FMMX_HIP(hipEventRecord(ev[2], stream));
+  m2l_timed = false;
   e = 3;
   k_zero_f64<<<1, 64, 0, stream>>>(14, pair_ctr.as<double>() + 2);  // M2L pair counters per level
   if (nj > 0) {
@@ -1175,9 +1176,11 @@ int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st) {
   for (int l = 2; l <= e.lmax; l++) s.m2l_pairs += (int64_t)mp[2 + l];
   s.m2l_pairs_leaf = (int64_t)mp[2 + e.lmax];
   s.ms_m2l_leaf = 0;
-  if (e.per_phase && e.last_neq >= 0 && e.si.nb[e.lmax] > 0 && e.sj.nb[e.lmax] > 0 &&
-      hipEventElapsedTime(&ms, e.ev[8], e.ev[9]) == hipSuccess)
+  if (e.per_phase && e.m2l_timed && e.last_neq >= 0 && hipEventElapsedTime(&ms, e.ev[8], e.ev[9]) == hipSuccess)
     s.ms_m2l_leaf = ms;
+  // a query that failed above (events of a phase that did not run) must not
+  // leave the runtime's last error set for the caller's next launch
+  (void)hipGetLastError();
   *st = s;
   return FMMX_OK;
 }

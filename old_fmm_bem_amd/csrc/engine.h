@@ -207,6 +207,7 @@ struct Engine {
   int run_m2l(int lev, bool init, int phase = 0);
   int run_m2l_all(int l0);  // FP32 MFMA: every level in one launch
   bool last_m2l_all = false;  // the last evaluation took that form
+  bool m2l_timed = false;     // the last evaluation recorded ev[8], ev[9] around its M2L launch
   bool m2l_all_levels() const {
     if (fp64 || hh) return false;
     const char* e = getenv("FMMX_M2L_FUSED");

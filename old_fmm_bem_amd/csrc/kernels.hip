@@ -2140,7 +2140,10 @@ int Engine::run_m2l_all(int l0) {
     constexpr int P = decltype(pc)::value;
     if (per_phase) FMMX_HIP(hipEventRecord(ev[8], stream));
     launch_m2l<P>(A, wg, leaf.nt, leaf.pa, leaf.wpg, stream);
-    if (per_phase) FMMX_HIP(hipEventRecord(ev[9], stream));
+    if (per_phase) {
+      FMMX_HIP(hipEventRecord(ev[9], stream));
+      m2l_timed = true;
+    }
     for (int k = 0; k < A.nlv; k++) {
       const M2LLev& d = A.lv[k];
       const long nl = (long)d.nb * P2;
@@ -2227,7 +2230,10 @@ int Engine::run_m2l(int lev, bool init, int phase) {
           lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
           sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), part_buf->as<double>(),
           opm2l.as<double>() + opoff, offtab.as<int4>(), init ? 1 : 0, gsh, pair_ctr.as<unsigned long long>() + 2 + lev);
-      if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
+      if (tmark) {
+        FMMX_HIP(hipEventRecord(ev[9], stream));
+        m2l_timed = true;
+      }
       if (nsplit > 1 && phase == 0)
         k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
                                                                           lexp[lev].as<double>(), init ? 1 : 0);
@@ -2239,7 +2245,10 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
       launch_m2l<P>(A, A.lv[0].nwg, var.nt, var.pa, var.wpg, stream);
-      if (tmark) FMMX_HIP(hipEventRecord(ev[9], stream));
+      if (tmark) {
+        FMMX_HIP(hipEventRecord(ev[9], stream));
+        m2l_timed = true;
+      }
       if (nsplit > 1 && phase == 0)
         k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<float>(),
                                                                          lexp[lev].as<float>(), init ? 1 : 0);

@@ -267,3 +267,26 @@ def test_helmholtz_argument_errors():
     with pytest.raises(F.FMMError):
         g.evaluate(q, neq=F.HELMHOLTZ)  # odd p
     g.close()
+
+
+@pytest.mark.gpu
+def test_helmholtz_timing_stats_leave_no_runtime_error():
+    """Regression: with per-phase timing the Helmholtz evaluation has no timed
+    M2L launch; reading the stats must not leave the HIP runtime's last error
+    set, which PyTorch (same runtime state) would report at its next launch."""
+    torch = pytest.importorskip("torch")
+    x, _ = uniform_cube(20000, 61)
+    rng = np.random.default_rng(4)
+    xd = torch.from_numpy(x).cuda()
+    qd = torch.from_numpy((rng.random(20000) + 1j * rng.random(20000)).astype(np.complex64)).cuda()
+    for _ in range(2):
+        f = F.FMM(p=8, lmax=4, precision="mixed", convention="exact", k=5.0)
+        f.set_timing(True)
+        f.set_points(xd)
+        f.evaluate(qd, neq=F.HELMHOLTZ)
+        st = f.stats()
+        assert st["ms_m2l_leaf"] == 0 and st["ms_m2l"] > 0
+        f.close()
+        y = xd[:, 0].contiguous() * 2  # a torch launch after the handle's calls
+        torch.cuda.synchronize()
+        assert float(y.sum()) > 0
