@@ -1392,6 +1392,7 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
   for (int oi = o0; oi < o1; oi++) {
     const int4 off = offtab[cls * NEBM + oi];
     __syncthreads();
+    int has = 0;
     if (tid < M2L_TN) {
       int s = -1;
       if (tid < ntile) {
@@ -1402,8 +1403,12 @@ __global__ __launch_bounds__(256) void k_m2l_vec(int lev, int nb, int nsplit, co
           s = dense_j[morton_encode(jx, jy, jz, lev)];
       }
       src[tid] = s;
+      has = s >= 0;
       if (s >= 0) mypairs++;
     }
+    // no target of the tile has a source box at this offset (domain edge,
+    // sparse surfaces): neither stage the operator nor multiply
+    if (!__syncthreads_or(has)) continue;
     const T* A = opT + (size_t)off.w * D * D;
     for (int k0 = 0; k0 < D; k0 += KC) {
       const int kc = min(KC, D - k0);
