@@ -332,6 +332,9 @@ std::shared_ptr<OpSet> get_hh_ops(int p, double k, float rd, int lmax) {
   S->l2l.assign((size_t)nt_lev * 8 * dq, 0.0);
   S->nlev = lmax - 1;
   S->m2l_kmaj.resize((size_t)S->nlev * NOFF * dq);
+  // FP32 engine: MFMA fragment order of the D x D operators
+  const size_t frh = (size_t)(D / 4) * ((D + 15) / 16) * 64;
+  S->m2l_frag.resize((size_t)S->nlev * NOFF * frh);
   const int ntr = lmax > 2 ? (lmax - 2) * 8 : 0, njobs = ntr + S->nlev * NOFF;
   std::vector<std::thread> th;
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -352,6 +355,7 @@ std::shared_ptr<OpSet> get_hh_ops(int p, double k, float rd, int lmax) {
           offset_vector(oi, d);
           hh_op_m2l(p, k * (double)rd / (double)(1 << (li + 2)), d[0], d[1], d[2], A.data());
           transpose_into(A.data(), D, &S->m2l_kmaj[((size_t)li * NOFF + oi) * dq]);
+          fragment_into(A.data(), D, &S->m2l_frag[((size_t)li * NOFF + oi) * frh]);
         }
       }
     });
@@ -548,7 +552,7 @@ int Engine::upload_helmholtz_operators() {
       return rc;
   } else {
     if ((rc = upload<float>(hop_m2m, S->m2m, stream)) || (rc = upload<float>(hop_l2l, S->l2l, stream)) ||
-        (rc = upload<float>(hop_m2l, S->m2l_kmaj, stream)))
+        (rc = upload<float>(hop_m2l, S->m2l_frag, stream)))
       return rc;
   }
   hops_lmax = lmax;

@@ -1323,6 +1323,132 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(M2LLaunch A) {
   if (mypairs) atomicAdd(npairs, mypairs);
 }
 
+// Helmholtz M2L of the FP32 engine on MFMA (v_mfma_f32_16x16x4f32): split
+// complex expansions, D = 2 p^2 rows ([Re; Im] locals) and columns ([R; I]
+// multipoles), operators per level in fragment order.  The whole operator
+// (D^2 floats, 332 KB at p = 12) exceeds LDS, so it is staged in chunks of
+// HH_KC k-steps shared by the workgroup's waves; each wave keeps its 16
+// targets' source multipoles (D / 4 per lane) in registers and skips the
+// products of an offset for which none of its targets has a source box.  Tile,
+// class, split and XCD mapping as k_m2l.
+constexpr int HH_KC = 8;
+template <int P, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_m2l_hh(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
+                                                     const uint32_t* __restrict__ ord,
+                                                     const uint32_t* __restrict__ ikey,
+                                                     const int* __restrict__ dense_j, const float* __restrict__ M,
+                                                     float* __restrict__ L, float* __restrict__ part,
+                                                     const float* __restrict__ opF, const int4* __restrict__ offtab,
+                                                     int init, int region, int gsh,
+                                                     unsigned long long* __restrict__ npairs) {
+  using ML = MLayout<P>;
+  constexpr int P2 = P * P, D = 2 * P2, MT = (D + 15) / 16, KS = D / 4, FR = KS * MT * 64;
+  constexpr int NCH = (KS + HH_KC - 1) / HH_KC, CHF = HH_KC * MT * 64;  // chunks, floats per chunk
+  constexpr int NTH = 64 * WPG, TN = 16 * WPG;
+  static_assert(P % 2 == 0 && KS * 2 == P2, "each lane's k-range lies in one part");
+  using V4 = f32x4;
+  __shared__ __align__(16) float As[CHF];
+  __shared__ int s_class, s_start, s_n;
+  const int tid = threadIdx.x;
+  int cls0, split, tile;
+  if (region) {
+    const int xcd = blockIdx.x & 7, item = blockIdx.x >> 3, tpr = (int)(gridDim.x >> 6) / nsplit;
+    cls0 = item & 7;
+    split = (item >> 3) % nsplit;
+    tile = xcd * tpr + (item >> 3) / nsplit;
+  } else {
+    const int item = blockIdx.x >> 3, tpc = (int)(gridDim.x >> 3) / nsplit;
+    cls0 = blockIdx.x & 7;
+    split = item / tpc;
+    tile = item % tpc;
+  }
+  if (tid == 0) {
+    int start = 0;
+    for (int c = 0; c < cls0; c++) start += cls_cnt[c];
+    const int cnt = cls_cnt[cls0], nt = (cnt + TN - 1) / TN;
+    s_class = tile < nt ? cls0 : -1;
+    s_start = start + tile * TN;
+    s_n = tile < nt ? min(TN, cnt - tile * TN) : 0;
+  }
+  __syncthreads();
+  const int cls = s_class, ntile = s_n;
+  if (cls < 0) return;
+  const int w = tid >> 6, l = tid & 63, j = l & 15, kq = l >> 4;
+  const int g = 1 << lev;
+  const int t = w * 16 + j;
+  int tord = -1, tx = 0, ty = 0, tz = 0;
+  if (t < ntile) {
+    tord = (int)ord[s_start + t];
+    morton_decode(ikey[tord], lev, tx, ty, tz);
+  }
+  const bool wave_on = w * 16 < ntile;
+  const int per = (NEBM + nsplit - 1) / nsplit, o0 = split * per, o1 = min(NEBM, o0 + per);
+  V4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) acc[mt] = V4{0.f, 0.f, 0.f, 0.f};
+  unsigned long long mypairs = 0;
+  // lane kq holds k = kq KS + ks: part kq / 2, slots (kq % 2) KS + ks
+  const int bpart = kq >> 1, bslot = (kq & 1) * KS;
+  for (int o = o0; o < o1; o++) {
+    const int4 off = offtab[cls * NEBM + o];
+    int s_ = -1;
+    if (tord >= 0) {
+      const int jx = tx - off.x, jy = ty - off.y, jz = tz - off.z;
+      if (jx >= 0 && jx < g && jy >= 0 && jy < g && jz >= 0 && jz < g &&
+          (gsh < 0 || !((jx ^ tx) >> gsh | (jy ^ ty) >> gsh | (jz ^ tz) >> gsh)))
+        s_ = dense_j[morton_encode(jx, jy, jz, lev)];
+    }
+    if (s_ >= 0 && kq == 0) mypairs++;
+    const bool any = wave_on && __ballot(s_ >= 0) != 0;
+    float b[KS];
+    if (any) {
+      const float* ms = M + (size_t)(s_ >= 0 ? s_ : 0) * 2 * ML::VS + bpart * ML::VS;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) b[ks] = s_ >= 0 ? ms[ML::at(bslot + ks)] : 0.f;
+    }
+    const float* A = opF + (size_t)(off.w & 0xffff) * FR;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      const int kc = min(HH_KC, KS - c * HH_KC), nf = kc * MT * 64;
+      __syncthreads();  // every wave is done with the previous chunk
+      const float4* src = reinterpret_cast<const float4*>(A + (size_t)c * CHF);
+      for (int e = tid; e < nf / 4; e += NTH) reinterpret_cast<float4*>(As)[e] = src[e];
+      __syncthreads();
+      if (any) {
+#pragma unroll
+        for (int kk = 0; kk < HH_KC; kk++) {
+          const int ks = c * HH_KC + kk;
+          if (ks < KS) {
+            float af[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) af[mt] = As[(kk * MT + mt) * 64 + l];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) acc[mt] = Mfma<float>::fma(af[mt], b[ks], acc[mt]);
+          }
+        }
+      }
+    }
+  }
+  if (tord >= 0) {
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * mt + 4 * kq + r;
+        if (i < D) {
+          const float v = acc[mt][r];
+          if (nsplit == 1) {
+            float* dst = L + (size_t)tord * D + i;
+            *dst = init ? v : *dst + v;
+          } else {
+            part[((size_t)split * nb + tord) * D + i] = v;
+          }
+        }
+      }
+  }
+  if (mypairs) atomicAdd(npairs, mypairs);
+}
+
 // FP64 M2L on the vector FMA units (MI355X: FP64 vector peak = FP64 matrix
 // peak).  Same tiles and offset split as k_m2l; per offset the operator
 // (k-major) and the gathered source multipoles go through LDS in K-chunks
@@ -2037,7 +2163,7 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
   }
   if (v.nsplit <= 0) {
     // enough workgroups to cover the chip: split the 189 offsets when tiles are few
-    if (fp64 || hh) {
+    if (fp64) {
       v.nsplit = 1024 / ((si.nb[lev] + M2L_TN - 1) / M2L_TN + 8);
     } else {
       int cmax = 0;
@@ -2209,11 +2335,20 @@ int Engine::run_m2l(int lev, bool init, int phase) {
             lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
             sj.dense[lev].as<int>(), mexp[lev].as<double>(), lexp[lev].as<double>(), part_buf->as<double>(),
             hop_m2l.as<double>() + hoff, offtab.as<int4>(), init ? 1 : 0, gsh, pair_ctr.as<unsigned long long>() + 2 + lev);
-      else
-        k_m2l_vec<P, float, 2><<<tiles * nsplit, 256, 0, stream>>>(
+      else {
+        // MFMA, 8 waves of 16 targets per workgroup, operator chunks in LDS
+        constexpr int D = 2 * P * P, FRH = (D / 4) * ((D + 15) / 16) * 64;
+        int cmax = 0;
+        for (int c = 0; c < 8; c++) cmax = std::max(cmax, cls_cnt_h[lev][c]);
+        const int tpc = std::max(1, (cmax + 127) / 128);
+        const int region = tpc >= 8 ? 1 : 0;
+        const int nwg = region ? 64 * ((tpc + 7) / 8) * nsplit : 8 * tpc * nsplit;
+        k_m2l_hh<P, 8><<<nwg, 512, 0, stream>>>(
             lev, nb, nsplit, cls_cnt[lev].as<int>(), cls_ord[lev].as<uint32_t>(), si.bkey[lev].as<uint32_t>(),
             sj.dense[lev].as<int>(), mexp[lev].as<float>(), lexp[lev].as<float>(), part_buf->as<float>(),
-            hop_m2l.as<float>() + hoff, offtab.as<int4>(), init ? 1 : 0, gsh, pair_ctr.as<unsigned long long>() + 2 + lev);
+            hop_m2l.as<float>() + (size_t)(lev - 2) * NOFF * FRH, offtab.as<int4>(), init ? 1 : 0, region, gsh,
+            pair_ctr.as<unsigned long long>() + 2 + lev);
+      }
       if (nsplit > 1 && phase == 0) {
         if (fp64)
           k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
