@@ -1333,7 +1333,7 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l(M2LLaunch A) {
 // class, split and XCD mapping as k_m2l.
 constexpr int HH_KC = 8;
 template <int P, int WPG>
-__global__ __launch_bounds__(64 * WPG) void k_m2l_hh(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
+__global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_m2l_hh(int lev, int nb, int nsplit, const int* __restrict__ cls_cnt,
                                                      const uint32_t* __restrict__ ord,
                                                      const uint32_t* __restrict__ ikey,
                                                      const int* __restrict__ dense_j, const float* __restrict__ M,
@@ -1347,7 +1347,10 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hh(int lev, int nb, int nsplit
   constexpr int NTH = 64 * WPG, TN = 16 * WPG;
   static_assert(P % 2 == 0 && KS * 2 == P2, "each lane's k-range lies in one part");
   using V4 = f32x4;
-  __shared__ __align__(16) float As[CHF];
+  // two chunk buffers: the next chunk is loaded into registers while the
+  // current one feeds the MFMAs, then stored into the other buffer (one
+  // barrier per chunk)
+  __shared__ __align__(16) float As[2 * CHF];
   __shared__ int s_class, s_start, s_n;
   const int tid = threadIdx.x;
   int cls0, split, tile;
@@ -1389,6 +1392,33 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hh(int lev, int nb, int nsplit
   unsigned long long mypairs = 0;
   // lane kq holds k = kq KS + ks: part kq / 2, slots (kq % 2) KS + ks
   const int bpart = kq >> 1, bslot = (kq & 1) * KS;
+  constexpr int NV4 = (CHF / 4 + NTH - 1) / NTH;  // float4 per thread per chunk
+  V4 areg[NV4];
+#define FMMX_HH_OP(o_) (opF + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FR)
+#define FMMX_HH_LOAD(a_, c_)                                                                     \
+  {                                                                                              \
+    const int nf4_ = min(HH_KC, KS - (c_) * HH_KC) * MT * 16;                                     \
+    const V4* src_ = reinterpret_cast<const V4*>((a_) + (size_t)(c_) * CHF);             \
+    _Pragma("unroll") for (int v = 0; v < NV4; v++) {                                            \
+      const int e_ = tid + NTH * v;                                                              \
+      if (e_ < nf4_) areg[v] = src_[e_];                                                         \
+    }                                                                                            \
+  }
+#define FMMX_HH_STORE(buf_, c_)                                                                  \
+  {                                                                                              \
+    const int nf4_ = min(HH_KC, KS - (c_) * HH_KC) * MT * 16;                                     \
+    V4* dst_ = reinterpret_cast<V4*>(As + (buf_) * CHF);                                 \
+    _Pragma("unroll") for (int v = 0; v < NV4; v++) {                                            \
+      const int e_ = tid + NTH * v;                                                              \
+      if (e_ < nf4_) dst_[e_] = areg[v];                                                         \
+    }                                                                                            \
+  }
+  if (o0 < o1) {
+    FMMX_HH_LOAD(FMMX_HH_OP(o0), 0);
+    FMMX_HH_STORE(0, 0);
+  }
+  __syncthreads();
+  int buf = 0;
   for (int o = o0; o < o1; o++) {
     const int4 off = offtab[cls * NEBM + o];
     int s_ = -1;
@@ -1406,29 +1436,36 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hh(int lev, int nb, int nsplit
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) b[ks] = s_ >= 0 ? ms[ML::at(bslot + ks)] : 0.f;
     }
-    const float* A = opF + (size_t)(off.w & 0xffff) * FR;
+    const float* Acur = FMMX_HH_OP(o);
+    const float* Anext = o + 1 < o1 ? FMMX_HH_OP(o + 1) : nullptr;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
-      const int kc = min(HH_KC, KS - c * HH_KC), nf = kc * MT * 64;
-      __syncthreads();  // every wave is done with the previous chunk
-      const float4* src = reinterpret_cast<const float4*>(A + (size_t)c * CHF);
-      for (int e = tid; e < nf / 4; e += NTH) reinterpret_cast<float4*>(As)[e] = src[e];
-      __syncthreads();
+      // the next chunk: of this offset, or the first of the next offset
+      const bool has_next = c + 1 < NCH || Anext != nullptr;
+      if (c + 1 < NCH) FMMX_HH_LOAD(Acur, c + 1)
+      else if (Anext) FMMX_HH_LOAD(Anext, 0)
       if (any) {
+        const float* Ab = As + buf * CHF;
 #pragma unroll
         for (int kk = 0; kk < HH_KC; kk++) {
           const int ks = c * HH_KC + kk;
           if (ks < KS) {
             float af[MT];
 #pragma unroll
-            for (int mt = 0; mt < MT; mt++) af[mt] = As[(kk * MT + mt) * 64 + l];
+            for (int mt = 0; mt < MT; mt++) af[mt] = Ab[(kk * MT + mt) * 64 + l];
 #pragma unroll
             for (int mt = 0; mt < MT; mt++) acc[mt] = Mfma<float>::fma(af[mt], b[ks], acc[mt]);
           }
         }
       }
+      if (has_next) FMMX_HH_STORE(buf ^ 1, c + 1 < NCH ? c + 1 : 0)
+      __syncthreads();
+      buf ^= 1;
     }
   }
+#undef FMMX_HH_OP
+#undef FMMX_HH_LOAD
+#undef FMMX_HH_STORE
   if (tord >= 0) {
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
