@@ -1663,8 +1663,15 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
     // (sn.x carries the imaginary part of the charge, lam the wavenumber)
     const T r = r2 * inv;
     T sk, ck;
-    if constexpr (sizeof(T) == 4) sincosf(lam * r, &sk, &ck);
-    else sincos(lam * r, &sk, &ck);
+    if constexpr (sizeof(T) == 4) {
+      // hardware sine/cosine (argument in revolutions): near-field k r is
+      // O(1), where their absolute error is far below the FP32 path's
+      const float t = float(lam * r) * 0.15915494309189535f;
+      sk = __builtin_amdgcn_sinf(t);
+      ck = __builtin_amdgcn_cosf(t);
+    } else {
+      sincos(lam * r, &sk, &ck);
+    }
     const T qim = T(sn.x) * inv;
     p += qi * ck - qim * sk;
     fx += qi * sk + qim * ck;
