@@ -1,16 +1,19 @@
 // kernels.hip -- FMM phase kernels for CDNA4 (gfx950), wave64.
 //
 //   P2P  kernel/G_p2p.cxx, Gni_p2p.cxx, Gnj_p2p.cxx: one workgroup per target
-//        leaf box, source particles of the near boxes staged through LDS,
-//        FP32 (or FP64) pair math, FP64 per-target sums.
+//        leaf box (two lanes per target for small leaves), source particles
+//        of the near boxes staged through LDS, FP32 (or FP64) pair math, FP64
+//        per-target sums.
 //   P2M  kernel/G_p2m.cxx, Gn_p2m.cxx; L2P kernel/G_l2p.cxx, Gn_l2p.cxx: one
-//        wave per leaf box; solid harmonics from a Cartesian recurrence (no
-//        acos/atan per term), wave reduction through LDS.
+//        wave per leaf box (two for small leaves); solid harmonics from a
+//        Cartesian recurrence (no acos/atan per term); L2P writes the outputs
+//        in the caller's order.
 //   M2M/L2L kernel/m2m.cxx, l2l.cxx: dense p^2 x p^2 operators per child
-//        octant (level independent in the scaled basis).
+//        octant (level independent in the scaled basis), on MFMA in FP32.
 //   M2L  kernel/m2l.cxx: targets of one parity class share the same 189
 //        lattice offsets, so a tile of targets runs as 189 GEMMs
-//        (operator [p^2 x p^2] x gathered source multipoles [p^2 x tile]).
+//        (operator [p^2 x p^2] x gathered source multipoles [p^2 x tile]);
+//        FP32 on MFMA, every level in one launch.
 //   direct sums kernel/G_dir.cxx, Gni_dir.cxx, Gnj_dir.cxx.
 #include <cstdlib>
 #include <cstring>
@@ -932,7 +935,7 @@ __global__ __launch_bounds__(L2L ? 256 : 512) void k_translate_mfma(int nout, co
 }
 
 // ---------------------------------------------------------------------
-// M2L on MFMA.  A workgroup (4 waves) owns a tile of 64 targets of one
+// M2L on MFMA.  A workgroup (WPG waves) owns a tile of 16 WPG targets of one
 // parity class at a level; all of them see the same 189 lattice offsets
 // (fmm/ijbox.cxx:366-401).  Per offset the operator (p^2 x p^2, stored by the
 // host in MFMA fragment order) is staged once in LDS and each wave runs
