@@ -53,22 +53,47 @@ __global__ void k_gather3(int n, const uint32_t* __restrict__ perm, const float*
   b2[i] = a2[j];
 }
 
-__global__ void k_level_flags(int n, const uint32_t* __restrict__ keys, int shift, int* __restrict__ flag) {
+// Box starts of up to four levels in one pass: component k of f[i] flags
+// whether particle i (key order) opens a box at level l0 + k (the leaf keys
+// shifted by 3 (lmax - level) give that level's keys, fmm/boxdatai.cxx:10-34);
+// one exclusive scan of the int4 flags numbers the boxes of all four levels.
+__device__ __forceinline__ int comp4(const int4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+__global__ void k_level_flags4(int n, const uint32_t* __restrict__ keys, int lmax, int l0, int nl,
+                               int4* __restrict__ f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  flag[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1 : 0;
+  int v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int shift = 3 * (lmax - (l0 + k));
+    v[k] = k < nl && (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1 : 0;
+  }
+  f[i] = make_int4(v[0], v[1], v[2], v[3]);
 }
 
-__global__ void k_level_compact(int n, const uint32_t* __restrict__ keys, int shift, const int* __restrict__ flag,
-                                const int* __restrict__ pos, uint32_t* __restrict__ bkey,
-                                int* __restrict__ bfirst, int* __restrict__ cnt) {
+struct LevelOut4 {
+  uint32_t* bkey[4];
+  int *bfirst[4], *flag[4], *pos[4], *cnt[4];
+};
+
+__global__ void k_level_compact4(int n, const uint32_t* __restrict__ keys, int lmax, int l0, int nl,
+                                 const int4* __restrict__ f, const int4* __restrict__ p, LevelOut4 o) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (flag[i]) {
-    bkey[pos[i]] = keys[i] >> shift;
-    bfirst[pos[i]] = i;
+  const int4 fi = f[i], pi = p[i];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (k >= nl) break;
+    const int fl = comp4(fi, k), ps = comp4(pi, k);
+    o.flag[k][i] = fl;
+    o.pos[k][i] = ps;
+    if (fl) {
+      o.bkey[k][ps] = keys[i] >> (3 * (lmax - (l0 + k)));
+      o.bfirst[k][ps] = i;
+    }
+    if (i == n - 1) *o.cnt[k] = ps + fl;
   }
-  if (i == n - 1) *cnt = pos[i] + flag[i];
 }
 
 // parent ordinal of each box at level lev: the level lev-1 box holding its
@@ -424,14 +449,29 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
         (rc = s.bkey[lev].ensure(nn * 4)) || (rc = s.bfirst[lev].ensure(nn * 4)) ||
         (rc = s.parent[lev].ensure(nn * 4)) || (rc = s.cfirst[lev].ensure((nn + 1) * 4)))
       return rc;
-    int shift = 3 * (lmax - lev);
-    k_level_flags<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), shift, s.flag[lev].as<int>());
+  }
+  // box starts, numbers, keys and first particles of up to four levels per pass
+  if ((rc = lvl_f4.ensure(nn * 16)) || (rc = lvl_p4.ensure(nn * 16))) return rc;
+  size_t b4 = 0;
+  FMMX_HIP(rocprim::exclusive_scan(nullptr, b4, lvl_f4.as<int4>(), lvl_p4.as<int4>(), make_int4(0, 0, 0, 0),
+                                   (size_t)n, rocprim::plus<int4>(), stream));
+  if (b4 > scratch.bytes && (rc = scratch.ensure(b4 + 256))) return rc;
+  for (int l0 = 2; l0 <= lmax; l0 += 4) {
+    const int nl = std::min(4, lmax - l0 + 1);
+    LevelOut4 o{};
+    for (int k = 0; k < nl; k++) {
+      o.bkey[k] = s.bkey[l0 + k].as<uint32_t>();
+      o.bfirst[k] = s.bfirst[l0 + k].as<int>();
+      o.flag[k] = s.flag[l0 + k].as<int>();
+      o.pos[k] = s.pos[l0 + k].as<int>();
+      o.cnt[k] = cnt + l0 + k;
+    }
+    k_level_flags4<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, l0, nl, lvl_f4.as<int4>());
     size_t b2 = scratch.bytes;
-    FMMX_HIP(rocprim::exclusive_scan(scratch.p, b2, s.flag[lev].as<int>(), s.pos[lev].as<int>(), 0, (size_t)n,
-                                     rocprim::plus<int>(), stream));
-    k_level_compact<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), shift, s.flag[lev].as<int>(),
-                                                 s.pos[lev].as<int>(), s.bkey[lev].as<uint32_t>(),
-                                                 s.bfirst[lev].as<int>(), cnt + lev);
+    FMMX_HIP(rocprim::exclusive_scan(scratch.p, b2, lvl_f4.as<int4>(), lvl_p4.as<int4>(), make_int4(0, 0, 0, 0),
+                                     (size_t)n, rocprim::plus<int4>(), stream));
+    k_level_compact4<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, l0, nl, lvl_f4.as<int4>(),
+                                                  lvl_p4.as<int4>(), o);
   }
   // box-level kernels run over at most min(n, 8^lev) boxes (the counts stay on
   // the device)
