@@ -69,7 +69,7 @@ def run_engine(F, torch, x, q, p, lmax, precision, steps, warmup):
     # M2L launch shapes and offset splits per level, timed by whole
     # evaluations on this tree (untimed set-up)
     f.set_points(x)
-    f.tuning = f.autotune(q)
+    f.tuning = f.autotune(q, reps=6)
     # headline: one event pair per call, near field overlapped with far field
     f.set_timing(False)
     torch.cuda.synchronize()
