@@ -466,6 +466,9 @@ class BatchedInducedCharge:
         self.f.set_points(self._c, ni_normals=self._n)
         self.diag = (self.e1 + self.e2) / (2 * self.e1 * (self.e1 - self.e2))
         self.centroids = c
+        # per surface (centroids, areas, normals) in the surface's own
+        # coordinates, kept for the right-hand sides and energies
+        self.panels = parts
 
     def matvec(self, x):
         v, _ = self.f.evaluate((x * self.area / self.e1).float(), neq=GNI)
@@ -576,19 +579,17 @@ def batched_solvation(surfaces, charges, e1=4.0, e2=20.0, p=8, precision="mixed"
     # the oracle-free GPU direct sum of the engine
     f = FMM(p=p, precision="fp64", convention="exact")
     un = np.zeros(op.n)
-    from .formats import panels
-
     energies = []
-    for s, ((vert, face), (xq, q)) in enumerate(zip(surfaces, charges)):
-        c, area, nrm = panels(vert, face)
+    for s, (xq, q) in enumerate(charges):
+        c, area, nrm = op.panels[s]
         v, _ = f.direct(c, np.asarray(xq, np.float32), np.asarray(q, np.float32), neq=GNI, ni_normals=nrm)
         un[op.seg[s]:op.seg[s + 1]] = -4 * np.pi * v / e1
     b = torch.from_numpy(un).to(op.dev)
     x, res, it = gmres_batched(op.matvec, b, op.seg_id, op.nseg, precond=op.precondition, rtol=rtol,
                                restart=restart, maxiter=maxiter)
     si = x.cpu().numpy()
-    for s, ((vert, face), (xq, q)) in enumerate(zip(surfaces, charges)):
-        c, area, nrm = panels(vert, face)
+    for s, (xq, q) in enumerate(charges):
+        c, area, nrm = op.panels[s]
         sl = slice(op.seg[s], op.seg[s + 1])
         vr, _ = f.direct(np.asarray(xq, np.float32), c, (si[sl] * area / e1).astype(np.float32), neq=G)
         energies.append(float(0.5 * KCAL * np.sum(np.asarray(q, np.float64) * vr)) * scale[s])
