@@ -431,44 +431,48 @@ class BatchedInducedCharge:
     def __init__(self, surfaces, e1=4.0, e2=20.0, p=8, precision="mixed", lmax=0, extra_levels=4, device=None):
         import torch
 
-        from .formats import panels
-
         ns = len(surfaces)
         G = max(1, int(np.ceil(np.log2(max(2, ns)) / 3)))
         cells = 1 << G
-        parts, ext = [], 0.0
-        for vert, face in surfaces:
-            c, area, nrm = panels(vert, face)
-            parts.append((c, area, nrm))
-            ext = max(ext, float(np.abs(np.asarray(vert) - np.asarray(vert).mean(0)).max()))
-        h = 2.2 * ext  # cell size: every surface strictly inside its cell
-        cs, ar, nr, seg = [], [], [], [0]
-        for s, (c, area, nrm) in enumerate(parts):
-            i, j, k = s % cells, (s // cells) % cells, s // (cells * cells)
-            centre = (np.array([i, j, k]) + 0.5) * h
-            cs.append((c - c.mean(0) + centre).astype(np.float32))
-            ar.append(area)
-            nr.append(nrm)
-            seg.append(seg[-1] + len(c))
-        self.G, self.seg = G, np.array(seg)
-        self.e1, self.e2 = float(e1), float(e2)
         self.dev = torch.device(device or "cuda")  # the panel vectors live on the GPU
-        c = np.concatenate(cs)
-        self.n = len(c)
-        self.area = torch.from_numpy(np.concatenate(ar).astype(np.float64)).to(self.dev)
-        self.seg_id = torch.from_numpy(np.repeat(np.arange(ns), np.diff(self.seg))).to(self.dev)
+        # panel geometry of every surface at once on the device, the float32
+        # arithmetic of formats.panels (bio/main.cxx:257-281)
+        vs = [np.asarray(v, np.float32) for v, _ in surfaces]
+        voff = np.cumsum([0] + [len(v) for v in vs[:-1]])
+        V = torch.from_numpy(np.concatenate(vs)).to(self.dev)
+        Fi = torch.from_numpy(np.concatenate([np.asarray(f, np.int64) - 1 + o for (_, f), o in zip(surfaces, voff)]))
+        v = V[Fi.to(self.dev)]
+        c = v[:, 0] / 3.0 + v[:, 1] / 3.0 + v[:, 2] / 3.0
+        nv = torch.cross(v[:, 2] - v[:, 0], v[:, 1] - v[:, 0], dim=1)
+        area = (0.5 * torch.sqrt((nv.double() ** 2).sum(1))).float()
+        nrm = nv / area[:, None] / 2.0
+        seg = np.cumsum([0] + [len(f) for _, f in surfaces])
+        ext = max(float(np.abs(x - x.mean(0)).max()) for x in vs)
+        h = 2.2 * ext  # cell size: every surface strictly inside its cell
+        self.G, self.seg = G, seg
+        self.e1, self.e2 = float(e1), float(e2)
+        self.n = int(seg[-1])
+        seg_id = torch.from_numpy(np.repeat(np.arange(ns), np.diff(seg))).to(self.dev)
+        # surface s moves to the centre of cell s: c - mean(c_s) + centre_s
+        cnt = torch.from_numpy(np.diff(seg).astype(np.float64)).to(self.dev)
+        mean = torch.zeros((ns, 3), dtype=torch.float64, device=self.dev).index_add_(0, seg_id, c.double())
+        mean = (mean / cnt[:, None]).float()
+        s_ = np.arange(ns)
+        centre = torch.from_numpy(((np.stack([s_ % cells, (s_ // cells) % cells, s_ // (cells * cells)], 1) + 0.5)
+                                   * h).astype(np.float64)).to(self.dev)
+        self.area = area.double()
+        self.seg_id = seg_id
         self.nseg = ns
         self.f = FMM(p=p, lmax=lmax or G + extra_levels, precision=precision, convention="exact",
                      box=(0.0, 0.0, 0.0, float(h * cells)))
         self.f.set_groups(G)
-        self._c = torch.from_numpy(c).to(self.dev)
-        self._n = torch.from_numpy(np.concatenate(nr).astype(np.float32)).to(self.dev)
+        self._c = ((c - mean[seg_id]).double() + centre[seg_id]).float().contiguous()
+        self._n = nrm.contiguous()
         self.f.set_points(self._c, ni_normals=self._n)
         self.diag = (self.e1 + self.e2) / (2 * self.e1 * (self.e1 - self.e2))
-        self.centroids = c
-        # per surface (centroids, areas, normals) in the surface's own
-        # coordinates, kept for the right-hand sides and energies
-        self.panels = parts
+        # the panels in each surface's own coordinates (device), for the
+        # right-hand sides and energies
+        self.pc, self.pa, self.pn = c.contiguous(), area, nrm.contiguous()
 
     def matvec(self, x):
         v, _ = self.f.evaluate((x * self.area / self.e1).float(), neq=GNI)
@@ -578,21 +582,21 @@ def batched_solvation(surfaces, charges, e1=4.0, e2=20.0, p=8, precision="mixed"
     # right-hand side: -4 pi Gni(charges -> centroids) / e1 per surface, on
     # the oracle-free GPU direct sum of the engine
     f = FMM(p=p, precision="fp64", convention="exact")
-    un = np.zeros(op.n)
+    b = torch.zeros(op.n, dtype=torch.float64, device=op.dev)
     energies = []
-    for s, (xq, q) in enumerate(charges):
-        c, area, nrm = op.panels[s]
-        v, _ = f.direct(c, np.asarray(xq, np.float32), np.asarray(q, np.float32), neq=GNI, ni_normals=nrm)
-        un[op.seg[s]:op.seg[s + 1]] = -4 * np.pi * v / e1
-    b = torch.from_numpy(un).to(op.dev)
+    dq = [(torch.from_numpy(np.asarray(xq, np.float32)).to(op.dev), torch.from_numpy(np.asarray(q, np.float32)).to(op.dev))
+          for xq, q in charges]
+    for s, (xq, q) in enumerate(dq):
+        sl = slice(int(op.seg[s]), int(op.seg[s + 1]))
+        v, _ = f.direct(op.pc[sl], xq, q, neq=GNI, ni_normals=op.pn[sl])
+        b[sl] = -4 * np.pi * v / e1
     x, res, it = gmres_batched(op.matvec, b, op.seg_id, op.nseg, precond=op.precondition, rtol=rtol,
                                restart=restart, maxiter=maxiter)
     si = x.cpu().numpy()
-    for s, (xq, q) in enumerate(charges):
-        c, area, nrm = op.panels[s]
-        sl = slice(op.seg[s], op.seg[s + 1])
-        vr, _ = f.direct(np.asarray(xq, np.float32), c, (si[sl] * area / e1).astype(np.float32), neq=G)
-        energies.append(float(0.5 * KCAL * np.sum(np.asarray(q, np.float64) * vr)) * scale[s])
+    for s, (xq, q) in enumerate(dq):
+        sl = slice(int(op.seg[s]), int(op.seg[s + 1]))
+        vr, _ = f.direct(xq, op.pc[sl], (x[sl] * op.pa[sl] / e1).float(), neq=G)
+        energies.append(float(0.5 * KCAL * (q.double() * vr).sum()) * scale[s])
     f.close()
     op.close()
     return {"energies": np.array(energies), "si": si, "residuals": res.cpu().numpy(), "iterations": it,
