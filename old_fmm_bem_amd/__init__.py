@@ -206,13 +206,14 @@ class FMM:
     def set_m2l_variant(self, lev, nt=0, staging=-1, wpg=0, nsplit=0):
         _check(lib().fmmx_set_m2l_variant(self.h, lev, nt, staging, wpg, nsplit))
 
-    def autotune(self, q, neq=G, reps=3):
+    def autotune(self, q, neq=G, reps=3, margin=0.01):
         """Pick the M2L launch variant of every level by timing whole
         evaluations of q on the current points (so that the choice accounts
         for the other phases running beside the M2L): levels from the leaf
         up, each trying the MFMA staging/workgroup shapes and split counts
         around the default, the others held.  Returns {level: (n-tiles,
-        staging, waves per workgroup, splits)}."""
+        staging, waves per workgroup, splits)}.  A candidate replaces the
+        best so far only when faster by more than the fraction margin."""
         import torch
 
         def timed():
@@ -242,7 +243,9 @@ class FMM:
                         continue
                     seen.add(eff)
                     t = timed()
-                    if t < tbest:
+                    # the first (default) shape stays unless another is clearly
+                    # faster: differences within the timing noise do not switch
+                    if t < tbest * (1.0 - margin):
                         best, tbest = eff, t
             self.set_m2l_variant(lev, *best)
         return {lev: self.m2l_variant(lev) for lev in range(2, lmax + 1)}
