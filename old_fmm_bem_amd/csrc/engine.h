@@ -96,6 +96,7 @@ struct Engine {
   Side si, sj;
   DevBuf scratch;  // rocPRIM temporary storage
   DevBuf lvl_f4, lvl_p4;  // box-start flags and numbers of four levels (tree build)
+  DevBuf xyz4;            // packed coordinates in caller order (tree build)
   // near list at the leaf level (targets): counts and rows of NNEAR
   DevBuf nij, neij, pair_ctr;
   // far lists per level (only with FMMX_KEEP_LISTS): rows of NEBM

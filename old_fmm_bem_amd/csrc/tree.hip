@@ -27,9 +27,13 @@ inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
 // clamped to the nearest box here instead of producing invalid keys.
 __global__ void k_keys(int n, const float* __restrict__ x, const float* __restrict__ y,
                        const float* __restrict__ z, float xmin, float ymin, float zmin, double rbinv,
-                       int lmax, uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+                       int lmax, uint32_t* __restrict__ key, uint32_t* __restrict__ idx,
+                       float4* __restrict__ xyz) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // the coordinates packed as well (when xyz is set): the gather into key
+  // order then reads one 16-byte record per point instead of three words
+  if (xyz) xyz[i] = make_float4(x[i], y[i], z[i], 0.f);
   int kl = 1 << lmax;
   int f0 = (int)((double)(x[i] - xmin) * rbinv);
   int f1 = (int)((double)(y[i] - ymin) * rbinv);
@@ -39,6 +43,17 @@ __global__ void k_keys(int n, const float* __restrict__ x, const float* __restri
   f2 = min(max(f2, 0), kl - 1);
   key[i] = morton_encode(f0, f1, f2, lmax);
   idx[i] = (uint32_t)i;
+}
+
+// packed coordinates into key order, as three columns
+__global__ void k_gather_xyz(int n, const uint32_t* __restrict__ perm, const float4* __restrict__ xyz,
+                             float* __restrict__ b0, float* __restrict__ b1, float* __restrict__ b2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = xyz[perm[i]];
+  b0[i] = v.x;
+  b1[i] = v.y;
+  b2[i] = v.z;
 }
 
 // three coordinate (or normal) columns into key order in one pass
@@ -355,7 +370,7 @@ int Engine::sort_points(int n, const float* x, const float* y, const float* z, i
   const int lm = cfg.lmax;
   double rbinv = (double)((float)(1 << lm) / cfg.rd);
   k_keys<<<nblk(n), TPB, 0, stream>>>(n, d[0], d[1], d[2], cfg.xmin, cfg.ymin, cfg.zmin, rbinv, lm,
-                                      srt_keys_in.as<uint32_t>(), srt_idx_in.as<uint32_t>());
+                                      srt_keys_in.as<uint32_t>(), srt_idx_in.as<uint32_t>(), nullptr);
   size_t tb = 0;
   FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
                                      srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
@@ -423,9 +438,10 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
     return rc;
   s.nb.fill(0);
   if (n == 0) return FMMX_OK;
+  if ((rc = xyz4.ensure(nn * 16))) return rc;
   double rbinv = (double)((float)(1 << lmax) / rd);
   k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rbinv, lmax, s.keys_in.as<uint32_t>(),
-                                      s.idx_in.as<uint32_t>());
+                                      s.idx_in.as<uint32_t>(), xyz4.as<float4>());
   size_t tb = 0;
   FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
                                      s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
@@ -439,7 +455,8 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
                                      s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
                                      stream));
   const uint32_t* perm = s.perm.as<uint32_t>();
-  k_gather3<<<nblk(n), TPB, 0, stream>>>(n, perm, x, y, z, s.x.as<float>(), s.y.as<float>(), s.z.as<float>());
+  k_gather_xyz<<<nblk(n), TPB, 0, stream>>>(n, perm, xyz4.as<float4>(), s.x.as<float>(), s.y.as<float>(),
+                                            s.z.as<float>());
   if (s.has_normals)
     k_gather3<<<nblk(n), TPB, 0, stream>>>(n, perm, nx, ny, nz, s.nx.as<float>(), s.ny.as<float>(),
                                            s.nz.as<float>());
