@@ -1628,6 +1628,27 @@ __global__ void k_m2l_reduce(long n, int nsplit, const T* __restrict__ part, T* 
   L[e] = s;
 }
 
+// the reductions of several levels in one launch: elements [off[k], off[k+1])
+// belong to level k (same order of the split sums as k_m2l_reduce, init 1)
+struct ReduceLevels {
+  const float* part[MAXL + 1];
+  float* L[MAXL + 1];
+  long off[MAXL + 2];
+  int nsplit[MAXL + 1];
+  int nl;
+};
+__global__ void k_m2l_reduce_levels(ReduceLevels R) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R.off[R.nl]) return;
+  int k = 0;
+  while (k + 1 < R.nl && t >= R.off[k + 1]) k++;
+  const long n = R.off[k + 1] - R.off[k], e = t - R.off[k];
+  const float* part = R.part[k];
+  float s = 0.f;
+  for (int j = 0; j < R.nsplit[k]; j++) s += part[(size_t)j * n + e];
+  R.L[k][e] = s;
+}
+
 // ---------------------------------------------------------------------
 // reciprocal square root: v_rsq_f32 (1 ulp) in FP32, full precision in FP64
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -2322,12 +2343,20 @@ int Engine::run_m2l_all(int l0) {
       FMMX_HIP(hipEventRecord(ev[9], stream));
       m2l_timed = true;
     }
+    // the partial sums of every level with splits, one launch
+    ReduceLevels R{};
+    R.off[0] = 0;
     for (int k = 0; k < A.nlv; k++) {
       const M2LLev& d = A.lv[k];
-      const long nl = (long)d.nb * P2;
-      if (d.nsplit > 1)
-        k_m2l_reduce<float><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, d.nsplit, d.part, d.L, 1);
+      if (d.nsplit < 2) continue;
+      R.part[R.nl] = d.part;
+      R.L[R.nl] = d.L;
+      R.nsplit[R.nl] = d.nsplit;
+      R.off[R.nl + 1] = R.off[R.nl] + (long)d.nb * P2;
+      R.nl++;
     }
+    if (R.nl > 0)
+      k_m2l_reduce_levels<<<(int)((R.off[R.nl] + 255) / 256), 256, 0, stream>>>(R);
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
   });
