@@ -114,6 +114,38 @@ int fmmx_set_points(fmmx_handle* h, int ni, const float* xi, const float* yi, co
 int fmmx_evaluate(fmmx_handle* h, int neq, const float* qj, double* vi, double* gxi, double* gyi,
                   double* gzi, int flags);
 
+/* One phase of the matvec at a time, the reference's operator sequence
+ * (fmm/fmm.cxx:42-150) on the tree of the last fmmx_set_points; strengths,
+ * expansions and partial sums stay on the device between calls:
+ *   FMMX_PHASE_P2P  near field of qj (kernel/G_p2p.cxx:7, Gni_p2p.cxx:7,
+ *                   Gnj_p2p.cxx:7, Helmholtz_G_p2p.cxx) -> vi (and g)
+ *   FMMX_PHASE_P2M  leaf multipoles of qj (kernel/G_p2m.cxx:11; the dipole
+ *                   form Gn_p2m.cxx:11 for FMMX_GNJ)
+ *   FMMX_PHASE_M2M  multipoles of level lev (2..lmax-1) from lev+1
+ *                   (kernel/m2m.cxx:13)
+ *   FMMX_PHASE_M2L  locals of level lev from its interaction list
+ *                   (kernel/m2l.cxx:13): level 2 writes, deeper levels add to
+ *                   what L2L put there
+ *   FMMX_PHASE_L2L  locals of level lev (3..lmax) from lev-1, overwritten
+ *                   (kernel/l2l.cxx:13)
+ *   FMMX_PHASE_L2P  far field -> vi (and g) (kernel/G_l2p.cxx:11; the normal
+ *                   derivative Gn_l2p.cxx:11 for FMMX_GNI)
+ * neq is the kernel of the whole sequence (FMMX_G, FMMX_GNI, FMMX_GNJ or
+ * FMMX_YUKAWA); qj is read by P2P and P2M only, vi and g written by P2P and
+ * L2P only (caller order, overwritten; the P2P and L2P outputs add up to
+ * fmmx_evaluate's).  Expansions can be read after each phase
+ * (fmmx_get_multipoles_level, fmmx_get_locals).  No Helmholtz phases. */
+enum {
+  FMMX_PHASE_P2P = 0,
+  FMMX_PHASE_P2M = 1,
+  FMMX_PHASE_M2M = 2,
+  FMMX_PHASE_M2L = 3,
+  FMMX_PHASE_L2L = 4,
+  FMMX_PHASE_L2P = 5
+};
+int fmmx_evaluate_phase(fmmx_handle* h, int phase, int neq, int lev, const float* qj, double* vi, double* gxi,
+                        double* gyi, double* gzi, int flags);
+
 /* Direct sums (kernel/G_dir.cxx:6, Gni_dir.cxx:6, Gnj_dir.cxx:6) on the GPU
  * for ni targets against nj sources, FP64 accumulation, in the handle's
  * convention.  Stateless apart from the handle's device and stream (and its
@@ -206,6 +238,11 @@ int fmmx_set_timing(fmmx_handle* h, int per_phase);
  * (side 0 = targets i, 1 = sources j), 2: 1 if the last evaluate ran the
  * M2L of every level in one launch (FP32 Laplace and Yukawa), else 0 */
 int fmmx_get_count(fmmx_handle* h, int what, int side, int lev);
+/* root cube and leaf level of the tree in use (after set_points): the
+ * reference's globals xmin, ymin, zmin, rd and lmax (include/constants.h:
+ * 56-59), as fmmx_set_domain gave them or as computed from the points
+ * (rd <= 0, bio/main.cxx:355-368; lmax 0, fmm/nlevel.cxx:8) */
+int fmmx_get_domain(fmmx_handle* h, int* lmax, float* box4);
 /* sort permutation: sorted position -> caller index (sort/sorti.cxx:39-41) */
 int fmmx_get_perm(fmmx_handle* h, int side, int* perm);
 /* box keys at a level; particle ranges [first,last] (inclusive, as ndi/ndj
@@ -221,6 +258,8 @@ int fmmx_get_list(fmmx_handle* h, int lev, int kind, int* nij, int* neij);
  * for all source boxes stacked leaf level first (the reference's bx rows),
  * and local expansions at level lev after its M2L. */
 int fmmx_get_multipoles(fmmx_handle* h, double* bx_reim);
+/* the multipoles of one level lev (its rows of fmmx_get_multipoles) */
+int fmmx_get_multipoles_level(fmmx_handle* h, int lev, double* bx_reim);
 int fmmx_get_locals(fmmx_handle* h, int lev, double* ax_reim);
 
 /* Host only (no GPU needed): the dense real operator the engine applies,

@@ -27,7 +27,10 @@
  *    with plane waves, at any depth (the reference's path runs only at
  *    lmax 2, kernel/Helmholtz_m2l.cxx:111-122);
  *  - errors print a message and exit(1), as the reference's CUDA error path
- *    does (cutil-mp.h:776-787).
+ *    does (cutil-mp.h:776-787);
+ *  - alloc(1) allocates the box lists, interaction lists and expansions
+ *    fmm/fmm.cxx and the operator symbols use (the reference's sizes); the
+ *    reference's tables (fac, anm, sr, ynm, dnm) are not allocated.
  */
 #ifndef FMMX_REFSHIM_H
 #define FMMX_REFSHIM_H
@@ -65,6 +68,55 @@ void dealloc(int ialloc);
 /* fmm/fmm.cxx:29 -- one FMM matvec on the sorted arrays: vi (and gxi..gzi
  * for neq 0) from vj at xj (normals gxi.. for neq 1, gxj.. for neq 2) */
 void fmm(int mi, int mj, int neq);
+/* ---- The operator plugin: the symbols fmm/fmm.cxx:7-27 declares ----
+ * The reference picks its kernels at link time (biobem/Makefile:25-37 links
+ * the kernel/..._gpu.o objects instead of kernel/....o).  These symbols serve fmm/fmm.cxx
+ * compiled from the reference's sources: with the shim's set-up chain and
+ * box symbols, or with the reference's own tree objects (boxdatai.o,
+ * ijbox.o, ...) and only the kernels below taken from the shim.  Each kernel
+ * symbol runs one phase on the engine's device-resident state
+ * (fmmx_evaluate_phase), finding the engine's tree for the caller's current
+ * arrays itself; P2P and L2P add into vi (and gxi.. for G), as the
+ * reference's kernels do; multipoles and locals are mirrored into bx and ax
+ * after each phase (Laplace kernels; the unscaled layout of
+ * include/arrays.h:38).  fmm/fmm.cxx:104 asks ifbox for neib 3 at level 2,
+ * a list fmm/ijbox.cxx never builds: ifbox hands out the neib-1 list there
+ * and m2l at level 2 uses it. */
+/* kernel/precalc.cxx:10 -- nmp = mp(mp+1)/2 (mp must be the shim's p) */
+void precalc(int& nmp, int mp);
+/* fmm/boxdatai.cxx:10, boxdataj.cxx:10 -- leaf boxes of the sorted arrays:
+ * nfi/ndi (nfj/ndj/nej), lbi (lbj), rb = rd / 2^lmax */
+void boxdatai(int mi, int& lbi, double& rb);
+void boxdataj(int mj, int& lbj, double& rb);
+/* fmm/boxm2mi.cxx:6, boxm2mj.cxx:6 -- the parent level's boxes stacked in
+ * nfi (nfj, with nej) at nlbi[lev-1] (nlbj[lev-1]) */
+void boxm2mi(int& lbi, int lev);
+void boxm2mj(int& lbj, int lev);
+/* fmm/ifbox.cxx:7 -- lbjr = lbj; nij/neij/njb for level lev (neib 0 near,
+ * 1 and 2 far, 3 as 1) */
+void ifbox(int mi, int mj, int nmp, int lbi, int lbj, int& lbjr, int lev, int neib);
+/* kernel/G_p2p.cxx:7, Gni_p2p.cxx:7, Gnj_p2p.cxx:7, Helmholtz_G_p2p.cxx:7 */
+void G_p2p(int lbi, int lbj);
+void Gni_p2p(int lbi, int lbj);
+void Gnj_p2p(int lbi, int lbj);
+void Helmholtz_G_p2p(int lbi, int lbj);
+/* kernel/G_p2m.cxx:11, Gn_p2m.cxx:11, Helmholtz_G_p2m.cxx:11 */
+void G_p2m(int nmp, int mp, int lbj);
+void Gn_p2m(int nmp, int mp, int lbj);
+void Helmholtz_G_p2m(int nmp, int mp, int lbj);
+/* kernel/m2m.cxx:13, Helmholtz_m2m.cxx:13 */
+void m2m(int nmp, int mp, int lev, int lbj, int lbjo);
+void Helmholtz_m2m(int nmp, int mp, int lev, int lbj, int lbjo);
+/* kernel/m2l.cxx:13, Helmholtz_m2l.cxx:13 */
+void m2l(int nmp, int mp, int lbi, int lbj, int lev, int ini);
+void Helmholtz_m2l(int nmp, int mp, int lbi, int lbj, int lev, int ini);
+/* kernel/l2l.cxx:13 */
+void l2l(int nmp, int mp, int lev, int lbi);
+/* kernel/G_l2p.cxx:11, Gn_l2p.cxx:11, Helmholtz_G_l2p.cxx:11 */
+void G_l2p(int nmp, int mp, int lbi);
+void Gn_l2p(int nmp, int mp, int lbi);
+void Helmholtz_G_l2p(int nmp, int mp, int lbi);
+
 /* kernel/G_dir.cxx:6, Gni_dir.cxx:6, Gnj_dir.cxx:6 -- direct sums over
  * targets n0..n1 and sources n2..n3 (inclusive), overwriting the outputs */
 void G_dir(int n0, int n1, int n2, int n3);
@@ -73,11 +125,15 @@ void Gnj_dir(int n0, int n1, int n2, int n3);
 /* kernel/Helmholtz_G_dir.cxx:6 -- Yukawa direct sum, potential only */
 void Helmholtz_G_dir(int n0, int n1, int n2, int n3);
 
-/* The caller's globals the shim reads or writes (include/arrays.h:5-8, 30,
- * 31, 60, 64; include/constants.h:39-43, 57-61). */
+/* The caller's globals the shim reads or writes (include/arrays.h:5-8,
+ * 30-34, 38, 60, 64; include/constants.h:39-43, 57-61).  Also ax and bx
+ * (include/arrays.h:38, std::complex<double> (*)[mpsym] with the caller's
+ * mpsym), not declared here because their type depends on the caller's mp. */
 extern float *xi, *yi, *zi, *gxi, *gyi, *gzi, *vi;
 extern float *xj, *yj, *zj, *gxj, *gyj, *gzj, *vj;
 extern int *nbi, *nbj, *nek, *irank;
+extern int **ndi, *nfi, *nlbi, **ndj, *nej, *nfj, *nlbj, *nij, *njb;
+extern int (*neij)[189];
 extern double* tfmm;
 extern int nbmax, nbne, nbnet, lmax;
 extern float xmin, ymin, zmin, rd;

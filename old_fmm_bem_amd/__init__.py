@@ -76,6 +76,7 @@ _SYMBOLS = {
     "fmmx_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "fmmx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "fmmx_get_count": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "fmmx_get_domain": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fmmx_get_perm": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fmmx_get_boxes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fmmx_get_list": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -424,6 +425,13 @@ class FMM:
     @property
     def lmax(self):
         return lib().fmmx_get_count(self.h, 0, 0, 0)
+
+    def domain(self):
+        """(lmax, (xmin, ymin, zmin, rd)) of the tree in use."""
+        lm = C.c_int(0)
+        b = np.zeros(4, np.float32)
+        _check(lib().fmmx_get_domain(self.h, C.byref(lm), b.ctypes.data))
+        return lm.value, tuple(float(v) for v in b)
 
     def nboxes(self, side, lev):
         return lib().fmmx_get_count(self.h, 1, 0 if side == "i" else 1, lev)

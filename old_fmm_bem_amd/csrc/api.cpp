@@ -31,6 +31,7 @@ __global__ void k_gather_q(int n, const uint32_t* __restrict__ perm, const float
 struct CountSrc {
   const int* a[2 + MAXL + 1];  // box counts of both sides, class counts per level
   const unsigned long long* pairs;
+  const int* bad;  // coordinates outside the root cube (stage 0)
 };
 __global__ void k_pack_counts(CountSrc src, int lmax, int stage, int* __restrict__ out) {
   const int t = threadIdx.x;
@@ -39,6 +40,8 @@ __global__ void k_pack_counts(CountSrc src, int lmax, int stage, int* __restrict
     if (t < 2 * (MAXL + 1)) {
       const int* c = src.a[t / (MAXL + 1)];
       out[t] = c ? c[t % (MAXL + 1)] : 0;
+    } else if (t == 2 * (MAXL + 1)) {
+      out[t] = src.bad ? *src.bad : 0;
     }
   } else {
     // [0, 2): near pair count (u64), then 8 class counts per level
@@ -89,6 +92,11 @@ __global__ void k_scatter_c(int n, const uint32_t* __restrict__ perm, const doub
     d[2 * (size_t)j] = re[i];
     d[2 * (size_t)j + 1] = im[i];
   }
+}
+
+__global__ void k_zero_i32(int n, int* __restrict__ a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
 __global__ void k_zero_f64(long n, double* __restrict__ a) {
@@ -654,17 +662,24 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
       hi[k] = o2f(h[3 + k]);
     }
     if (ni + nj == 0) lo[0] = lo[1] = lo[2] = 0, hi[0] = hi[1] = hi[2] = 1;
-    // bio/main.cxx:373-380: largest extent times (1+1e-5), in float
+    // bio/main.cxx:364-368: the extent is a float difference, multiplied by
+    // the double 1+1e-5 and rounded once to the float rdx; rd is the largest
     float r = 0;
     for (int k = 0; k < 3; k++) {
       if (!std::isfinite(lo[k]) || !std::isfinite(hi[k])) return fail(FMMX_EINVAL, "non-finite coordinates");
-      r = std::max(r, (float)((hi[k] - lo[k]) * (float)(1 + 1e-5)));
+      const float ext = hi[k] - lo[k];
+      r = std::max(r, (float)((double)ext * (1 + 1e-5)));
     }
     if (!(r > 0)) r = 1.0f;
     xmin = lo[0]; ymin = lo[1]; zmin = lo[2]; rd = r;
   }
   lmax = cfg.lmax > 0 ? cfg.lmax : std::min(MAXL, nlevel(nj));
+  if (group_level > lmax)
+    return fail(FMMX_EINVAL, "group level " + std::to_string(group_level) + " is deeper than lmax " +
+                                 std::to_string(lmax) + " (fmmx_set_groups)");
   if ((rc = upload_operators())) return rc;
+  if ((rc = bad_flag.ensure(4))) return rc;
+  k_zero_i32<<<1, 32, 0, stream>>>(1, bad_flag.as<int>());
   // tree of both sides (keys, sort, gathers, per-level boxes)
   if ((rc = build_side(si, ni, d[0], d[1], d[2], d[3], d[4], d[5], false))) return rc;
   if ((rc = build_side(sj, nj, d[6], d[7], d[8], d[9], d[10], d[11], true, same ? &si : nullptr))) return rc;
@@ -675,9 +690,11 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   CountSrc cs{};
   cs.a[0] = ni > 0 ? si.cnt.as<int>() : nullptr;
   cs.a[1] = nj > 0 ? sj.cnt.as<int>() : nullptr;
+  cs.bad = bad_flag.as<int>();
   k_pack_counts<<<1, 64, 0, stream>>>(cs, lmax, 0, rb_dev.as<int>());
-  FMMX_HIP(hipMemcpyAsync(rb_host, rb_dev.p, 2 * (MAXL + 1) * 4, hipMemcpyDeviceToHost, stream));
+  FMMX_HIP(hipMemcpyAsync(rb_host, rb_dev.p, (2 * (MAXL + 1) + 1) * 4, hipMemcpyDeviceToHost, stream));
   FMMX_HIP(hipStreamSynchronize(stream));
+  if (rb_host[2 * (MAXL + 1)]) return fail(FMMX_EINVAL, "coordinates outside the root cube or not finite");
   for (int l = 2; l <= lmax; l++) {
     si.nb[l] = rb_host[l];
     sj.nb[l] = rb_host[MAXL + 1 + l];
@@ -703,7 +720,9 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   return FMMX_OK;
 }
 
-int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags) {
+// Arguments of an evaluation (whole or one phase) against the tree in use.
+int Engine::check_eval(int neq, const float* qj, const double* vi, const double* gxi, const double* gyi,
+                       const double* gzi) {
   if (!have_points) return fail(FMMX_ESTATE, "evaluate before set_points");
   if (neq != FMMX_G && neq != FMMX_GNI && neq != FMMX_GNJ && neq != FMMX_YUKAWA && neq != FMMX_HELMHOLTZ)
     return fail(FMMX_EINVAL, "neq must be 0, 1, 2, 10 or 20");
@@ -717,12 +736,17 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   if (neq == FMMX_GNI && si.n > 0 && !si.has_normals) return fail(FMMX_EINVAL, "Gni needs target normals");
   if (neq == FMMX_GNJ && sj.n > 0 && !sj.has_normals) return fail(FMMX_EINVAL, "Gnj needs source normals");
   if ((sj.n > 0 && !qj) || (si.n > 0 && !vi)) return fail(FMMX_EINVAL, "q or v pointer is NULL");
+  return FMMX_OK;
+}
+
+// Evaluation buffers and the kernel's operators (allocates only when a
+// buffer grows, so a repeated evaluation on one tree allocates nothing).
+int Engine::prepare(int neq) {
   FMMX_HIP(hipSetDevice(device));
-  const bool dev = flags & FMMX_DEVICE_PTRS;
   const int ni = si.n, nj = sj.n;
   const size_t es = fp64 ? 8 : 4;
   int rc = FMMX_OK;
-  size_t ni1 = (size_t)(ni > 0 ? ni : 1);
+  const size_t ni1 = (size_t)(ni > 0 ? ni : 1);
   yk = neq == FMMX_YUKAWA;
   hh = neq == FMMX_HELMHOLTZ;
   const int np = parts();
@@ -735,10 +759,19 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
         (rc = lexp[l].ensure((size_t)std::max(1, si.nb[l]) * P2 * np * es)))
       return rc;
   }
-  const float* q = stage(qin, qj, hh ? 2 * nj : nj, dev, stream, rc);
-  if (rc) return rc;
   if (yk && (rc = upload_yukawa_operators())) return rc;
   if (hh && (rc = upload_helmholtz_operators())) return rc;
+  return FMMX_OK;
+}
+
+int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags) {
+  int rc = check_eval(neq, qj, vi, gxi, gyi, gzi);
+  if (rc || (rc = prepare(neq))) return rc;
+  const bool dev = flags & FMMX_DEVICE_PTRS;
+  const int ni = si.n, nj = sj.n;
+  const size_t ni1 = (size_t)(ni > 0 ? ni : 1);
+  const float* q = stage(qin, qj, hh ? 2 * nj : nj, dev, stream, rc);
+  if (rc) return rc;
   // Yukawa: no far field where lambda * (leaf box) makes it vanish in FP64
   const bool far = !yk || lam * rb(lmax) <= YUK_LRB_MAX;
   // outputs in the caller's order (device pointers, or staging for host ones)
@@ -855,6 +888,91 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   for (int k = 0; k < 4; k++)
     if (!dev && dsts[k]) FMMX_HIP(hipMemcpyAsync(outs[k], dsts[k], (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
   if (!capturing) FMMX_HIP(hipEventRecord(ev[15], stream));
+  if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
+  last_neq = neq;
+  return FMMX_OK;
+}
+
+// One phase of the matvec (the reference's operator symbols one at a time,
+// fmm/fmm.cxx:42-150) on the device-resident state of the last set_points:
+// strengths, expansions and near-field sums persist between phases.  The
+// serialized order of evaluate() without the one-launch M2L.
+int Engine::phase(int ph, int neq, int lev, const float* qj, double* vi, double* gxi, double* gyi, double* gzi,
+                  int flags) {
+  if (neq == FMMX_HELMHOLTZ) return fail(FMMX_EUNSUPPORTED, "phases of the Helmholtz kernel are not provided");
+  const bool needs_q = ph == FMMX_PHASE_P2P || ph == FMMX_PHASE_P2M;
+  const bool writes = ph == FMMX_PHASE_P2P || ph == FMMX_PHASE_L2P;
+  if (ph < FMMX_PHASE_P2P || ph > FMMX_PHASE_L2P) return fail(FMMX_EINVAL, "phase out of range [0, 5]");
+  int rc = check_eval(neq, needs_q ? qj : (const float*)1, writes ? vi : (const double*)1, gxi, gyi, gzi);
+  if (rc) return rc;
+  const int l0 = first_m2l_level();
+  if ((ph == FMMX_PHASE_M2M && (lev < 2 || lev >= lmax)) || (ph == FMMX_PHASE_M2L && (lev < l0 || lev > lmax)) ||
+      (ph == FMMX_PHASE_L2L && (lev <= l0 || lev > lmax)))
+    return fail(FMMX_EINVAL, "level out of range for this phase");
+  if ((rc = prepare(neq))) return rc;
+  const bool dev = flags & FMMX_DEVICE_PTRS;
+  const int ni = si.n, nj = sj.n;
+  const size_t ni1 = (size_t)(ni > 0 ? ni : 1);
+  // Yukawa: no far field where lambda * (leaf box) makes it vanish in FP64
+  const bool far = !yk || lam * rb(lmax) <= YUK_LRB_MAX;
+  if (needs_q && nj > 0) {
+    const float* q = stage(qin, qj, nj, dev, stream, rc);
+    if (rc) return rc;
+    k_gather_q<<<nblk(nj), TPB, 0, stream>>>(nj, sj.perm.as<uint32_t>(), q, qs.as<float>());
+  }
+  const bool g = neq == FMMX_G || neq == FMMX_YUKAWA;
+  double* outs[4] = {vi, g ? gxi : nullptr, g ? gyi : nullptr, g ? gzi : nullptr};
+  double* dsts[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int k = 0; writes && k < 4; k++) {
+    if (!outs[k] || ni == 0) continue;
+    dsts[k] = outs[k];
+    if (!dev) {
+      if ((rc = out[k].ensure(ni1 * 8))) return rc;
+      dsts[k] = out[k].as<double>();
+    }
+  }
+  double* sums[4] = {vs.as<double>(), gxs.as<double>(), gys.as<double>(), gzs.as<double>()};
+  switch (ph) {
+    case FMMX_PHASE_P2P:
+      if (ni > 0) {
+        // the near field alone: P2P writes the per-target sums, which go out
+        // in caller order and are then cleared for the far field
+        for (int k = 0; k < 4; k++) k_zero_f64<<<nblk(ni), TPB, 0, stream>>>(ni, sums[k]);
+        if ((rc = run_p2p(neq))) return rc;
+        k_scatter4<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), sums[0], sums[1], sums[2], sums[3],
+                                                 dsts[0], dsts[1], dsts[2], dsts[3]);
+        for (int k = 0; k < 4; k++) k_zero_f64<<<nblk(ni), TPB, 0, stream>>>(ni, sums[k]);
+      }
+      break;
+    case FMMX_PHASE_P2M:
+      if (far && (rc = run_p2m(neq))) return rc;
+      break;
+    case FMMX_PHASE_M2M:
+      if (far && (rc = run_m2m(lev))) return rc;
+      break;
+    case FMMX_PHASE_M2L:
+      if (far && (rc = run_m2l(lev, lev == l0))) return rc;
+      break;
+    case FMMX_PHASE_L2L:
+      if (far && (rc = run_l2l(lev))) return rc;
+      break;
+    case FMMX_PHASE_L2P:
+      if (ni > 0) {
+        // L2P adds the far field to the per-target sums: cleared first, so
+        // that the outputs hold the far field alone
+        for (int k = 0; k < 4; k++) k_zero_f64<<<nblk(ni), TPB, 0, stream>>>(ni, sums[k]);
+        if (far) {
+          if ((rc = run_l2p(neq, dsts))) return rc;
+        } else {
+          k_scatter4<<<nblk(ni), TPB, 0, stream>>>(ni, si.perm.as<uint32_t>(), sums[0], sums[1], sums[2], sums[3],
+                                                   dsts[0], dsts[1], dsts[2], dsts[3]);
+        }
+      }
+      break;
+  }
+  FMMX_HIP(hipGetLastError());
+  for (int k = 0; writes && k < 4; k++)
+    if (!dev && dsts[k]) FMMX_HIP(hipMemcpyAsync(outs[k], dsts[k], (size_t)ni * 8, hipMemcpyDeviceToHost, stream));
   if (!dev) FMMX_HIP(hipStreamSynchronize(stream));
   last_neq = neq;
   return FMMX_OK;
@@ -1189,6 +1307,26 @@ int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st) {
   return FMMX_OK;
 }
 
+int fmmx_evaluate_phase(fmmx_handle* h, int phase, int neq, int lev, const float* qj, double* vi, double* gxi,
+                        double* gyi, double* gzi, int flags) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "NULL handle");
+  FMMX_GUARD({ return h->e.phase(phase, neq, lev, qj, vi, gxi, gyi, gzi, flags); })
+}
+
+int fmmx_get_domain(fmmx_handle* h, int* lmax, float* box4) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "NULL handle");
+  Engine& e = h->e;
+  if (!e.have_points) return fmmx::fail(FMMX_ESTATE, "no tree (fmmx_set_points)");
+  if (lmax) *lmax = e.lmax;
+  if (box4) {
+    box4[0] = e.xmin;
+    box4[1] = e.ymin;
+    box4[2] = e.zmin;
+    box4[3] = e.rd;
+  }
+  return FMMX_OK;
+}
+
 int fmmx_get_count(fmmx_handle* h, int what, int side, int lev) {
   if (!h) return -1;
   Engine& e = h->e;
@@ -1290,6 +1428,15 @@ int fmmx_get_multipoles(fmmx_handle* h, double* bx) {
     row += e.sj.nb[l];
   }
   return FMMX_OK;
+}
+
+int fmmx_get_multipoles_level(fmmx_handle* h, int lev, double* bx) {
+  if (!h || !bx) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  Engine& e = h->e;
+  if (e.last_neq < 0) return fmmx::fail(FMMX_ESTATE, "no evaluation yet");
+  if (lev < 2 || lev > e.lmax) return fmmx::fail(FMMX_EINVAL, "level out of range");
+  if (hipStreamSynchronize(e.stream) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "sync failed");
+  return copy_expansion(e, e.mexp[lev], e.sj.nb[lev], lev, false, bx);
 }
 
 int fmmx_get_locals(fmmx_handle* h, int lev, double* ax) {

@@ -160,7 +160,8 @@ struct Engine {
   int graph_capture(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi);
   int graph_launch();
   // stand-alone Morton sort (fmmx_sort): staging, keys, permutation
-  DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm;
+  DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm, srt_bad;
+  DevBuf bad_flag;  // set by the key kernel for coordinates outside the root cube
   int last_neq = -1;
   // timing
   bool per_phase = false;
@@ -174,6 +175,9 @@ struct Engine {
                  const float* nzi, int nj, const float* xj, const float* yj, const float* zj, const float* nxj,
                  const float* nyj, const float* nzj, int flags);
   int evaluate(int neq, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags);
+  int phase(int ph, int neq, int lev, const float* qj, double* vi, double* gxi, double* gyi, double* gzi, int flags);
+  int check_eval(int neq, const float* qj, const double* vi, const double* gxi, const double* gyi, const double* gzi);
+  int prepare(int neq);
   int direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,
              const float* nyi, const float* nzi, int nj, const float* xj, const float* yj, const float* zj,
              const float* nxj, const float* nyj, const float* nzj, const float* qj, double* vi, double* gxi,

@@ -13,13 +13,21 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
+// include/arrays.h:38 declares ax and bx as std::complex<double> (*)[mpsym]
+// with the caller's compile-time mpsym; the shim uses them as rows of
+// p(p+1)/2 complex values (the same objects: p must be the caller's mp)
+extern std::complex<double> (*ax)[1], (*bx)[1];
+
 namespace {
+
+int nmp_of(int p) { return p * (p + 1) / 2; }
 
 struct Shim {
   fmmx_handle* h = nullptr;
@@ -38,6 +46,12 @@ struct Shim {
   std::vector<float> tmp;
   std::vector<double> out[4];
   bool nek_owned = false;
+  // operator symbols: the kernel of the sequence in progress (set by the P2P
+  // and P2M symbols) and the list and expansion arrays alloc(1) made
+  int plug_neq = FMMX_G;
+  int plug_mi = -1, plug_mj = -1;  // particle counts of the sequence in progress
+  bool lists_owned = false;
+  std::vector<double> exp_buf;
 };
 
 Shim& S() {
@@ -190,6 +204,78 @@ void direct(const char* fn, int neq, int n0, int n1, int n2, int n3) {
     }
 }
 
+// The engine's tree for the caller's current (sorted) arrays, built once per
+// content: the operator symbols find it in place across a matvec and across
+// GMRES iterations.  neq selects the normals it carries (target normals for
+// Gni, source normals for Gnj); keep_lists also builds the far lists.
+fmmx_handle* plug_tree(const char* fn, int mi, int mj, int neq) {
+  const bool keep_lists = true;  // ifbox hands out the far lists
+  Shim& s = S();
+  fmmx_handle* h = handle(fn);
+  const float* a[12] = {xi, yi, zi, nullptr, nullptr, nullptr, xj, yj, zj, nullptr, nullptr, nullptr};
+  if (neq == FMMX_GNI) a[3] = gxi, a[4] = gyi, a[5] = gzi;
+  if (neq == FMMX_GNJ) a[9] = gxj, a[10] = gyj, a[11] = gzj;
+  int n[12];
+  for (int k = 0; k < 12; k++) n[k] = k < 6 ? mi : mj;
+  const int nmode = (neq == FMMX_YUKAWA ? FMMX_G : neq) | (keep_lists ? 0x100 : 0);
+  if (!tree_is_current(s.tree, mi, mj, nmode, a, n)) {
+    const double t0 = now();
+    check(fn, fmmx_set_timing(h, 1));
+    check(fn, fmmx_set_points(h, mi, a[0], a[1], a[2], a[3], a[4], a[5], mj, a[6], a[7], a[8], a[9], a[10], a[11],
+                              keep_lists ? FMMX_KEEP_LISTS : 0));
+    add_time(10, now() - t0);
+  }
+  return h;
+}
+
+// the tree of the sequence in progress, whatever normals it carries; built
+// with the kernel of the last P2P/P2M symbol otherwise
+fmmx_handle* plug_tree_any(const char* fn) {
+  Shim& s = S();
+  const int mi = s.plug_mi, mj = s.plug_mj;
+  if (s.tree.ni == mi && s.tree.nj == mj && (s.tree.nmode & 0x100)) return plug_tree(fn, mi, mj, s.tree.nmode & 0xff);
+  return plug_tree(fn, mi, mj, s.plug_neq);
+}
+
+// Particle counts of the sequence from the leaf ranges ndi/ndj (filled by
+// boxdatai/boxdataj, the shim's or the reference's: fmm/boxdatai.cxx:22-29)
+// for the symbols that receive leaf box counts; -1 keeps the last value.
+void plug_counts(const char* fn, int lbi, int lbj) {
+  Shim& s = S();
+  if (lbi >= 0 && ndi) s.plug_mi = lbi > 0 ? ndi[1][lbi - 1] + 1 : 0;
+  if (lbj >= 0 && ndj) s.plug_mj = lbj > 0 ? ndj[1][lbj - 1] + 1 : 0;
+  if (s.plug_mi < 0 || s.plug_mj < 0) die(fn, "no box data (boxdatai/boxdataj) for this matvec");
+}
+
+int n_targets() { return S().plug_mi; }
+
+// One engine phase for the operator symbols.  P2P and L2P add their outputs
+// into the caller's float arrays (vi, and gxi.. when grad), as the reference's
+// kernels accumulate into them.
+void plug_phase(const char* fn, int ph, int neq, int lev, bool grad, int slot) {
+  Shim& s = S();
+  const double t0 = now();
+  fmmx_handle* h = handle(fn);
+  if (neq == FMMX_YUKAWA) check(fn, fmmx_set_yukawa(h, lambda));
+  const int mi = n_targets();
+  const bool writes = ph == FMMX_PHASE_P2P || ph == FMMX_PHASE_L2P;
+  for (auto& v : s.out) v.resize(std::max(mi, 1));
+  const bool g = grad && (neq == FMMX_G || neq == FMMX_YUKAWA);
+  check(fn, fmmx_evaluate_phase(h, ph, neq, lev, vj, writes ? s.out[0].data() : nullptr,
+                                writes && g ? s.out[1].data() : nullptr, writes && g ? s.out[2].data() : nullptr,
+                                writes && g ? s.out[3].data() : nullptr, 0));
+  if (writes) {
+    for (int i = 0; i < mi; i++) vi[i] += (float)s.out[0][i];
+    if (g)
+      for (int i = 0; i < mi; i++) {
+        gxi[i] += (float)s.out[1][i];
+        gyi[i] += (float)s.out[2][i];
+        gzi[i] += (float)s.out[3][i];
+      }
+  }
+  add_time(slot, now() - t0);
+}
+
 }  // namespace
 
 extern "C" int fmmx_ref_configure(int p, int precision, int convention, int device) {
@@ -311,16 +397,44 @@ void boxallocate(int n0, int n1, int n2, int n3) {
   add_time(1, now() - t0);
 }
 
-// fmm/alloc.cxx:32-122: the lists, expansions and tables live in the
-// engine; alloc(1) makes sure the handle exists
+// fmm/alloc.cxx:32-122: the tables and the engine's own lists and
+// expansions live on the device; alloc(1) makes sure the handle exists and,
+// for fmm/fmm.cxx and the operator symbols, allocates the caller's box lists,
+// interaction lists and expansions with the reference's sizes (ndi, ndj, nij,
+// njb, neij and ax: nbne rows; nfi, nfj and bx: nbnet; nej: nbmax; nlbi,
+// nlbj: lmax)
+void dealloc(int ialloc);
 void alloc(int ialloc) {
   double t0 = now();
-  if (ialloc == 1) (void)handle("alloc");
+  Shim& s = S();
+  if (ialloc == 1) {
+    (void)handle("alloc");
+    if (s.lists_owned) dealloc(2);
+    const int ne = std::max(nbne, 1), nt = std::max(nbnet, 1), nmp = nmp_of(s.p);
+    ndi = new int*[2];
+    ndj = new int*[2];
+    for (int k = 0; k < 2; k++) {
+      ndi[k] = new int[ne];
+      ndj[k] = new int[ne];
+    }
+    nfi = new int[nt];
+    nfj = new int[nt];
+    nlbi = new int[std::max(lmax, 1)];
+    nlbj = new int[std::max(lmax, 1)];
+    nej = new int[std::max(nbmax, 1)];
+    nij = new int[ne];
+    njb = new int[ne];
+    neij = new int[ne][189];
+    ax = reinterpret_cast<std::complex<double>(*)[1]>(new std::complex<double>[(size_t)ne * nmp]);
+    bx = reinterpret_cast<std::complex<double>(*)[1]>(new std::complex<double>[(size_t)nt * nmp]);
+    s.lists_owned = true;
+  }
   add_time(2, now() - t0);
 }
 
-// fmm/alloc.cxx:124-214: frees what boxpart allocated; the engine keeps its
-// device buffers for the next call (as the reference's GPU staging does)
+// fmm/alloc.cxx:124-214: frees what boxpart and alloc(1) allocated (2: the
+// lists and expansions only); the engine keeps its device buffers for the
+// next call (as the reference's GPU staging does)
 void dealloc(int ialloc) {
   Shim& s = S();
   if (ialloc == 1 && s.nek_owned) {
@@ -329,6 +443,29 @@ void dealloc(int ialloc) {
     nek = nullptr;
     irank = nullptr;
     s.nek_owned = false;
+  }
+  if ((ialloc == 1 || ialloc == 2) && s.lists_owned) {
+    for (int k = 0; k < 2; k++) {
+      delete[] ndi[k];
+      delete[] ndj[k];
+    }
+    delete[] ndi;
+    delete[] ndj;
+    delete[] nfi;
+    delete[] nfj;
+    delete[] nlbi;
+    delete[] nlbj;
+    delete[] nej;
+    delete[] nij;
+    delete[] njb;
+    delete[] neij;
+    delete[] reinterpret_cast<std::complex<double>*>(ax);
+    delete[] reinterpret_cast<std::complex<double>*>(bx);
+    ndi = ndj = nullptr;
+    nfi = nfj = nlbi = nlbj = nej = nij = njb = nullptr;
+    neij = nullptr;
+    ax = bx = nullptr;
+    s.lists_owned = false;
   }
 }
 
@@ -382,3 +519,259 @@ void G_dir(int n0, int n1, int n2, int n3) { direct("G_dir", FMMX_G, n0, n1, n2,
 void Gni_dir(int n0, int n1, int n2, int n3) { direct("Gni_dir", FMMX_GNI, n0, n1, n2, n3); }
 void Gnj_dir(int n0, int n1, int n2, int n3) { direct("Gnj_dir", FMMX_GNJ, n0, n1, n2, n3); }
 void Helmholtz_G_dir(int n0, int n1, int n2, int n3) { direct("Helmholtz_G_dir", FMMX_YUKAWA, n0, n1, n2, n3); }
+
+// ---- the operator plugin: the symbols fmm/fmm.cxx:7-27 declares and calls ----
+//
+// Each symbol runs its phase of the matvec on the engine's device-resident
+// state (fmmx_evaluate_phase).  They work under the shim's own fmm()-level
+// symbols and equally under the reference's fmm.o with its own tree and list
+// objects (boxdatai.o, ijbox.o, ...): every operator finds the engine's tree
+// for the caller's current arrays itself, and reads only the leaf ranges
+// ndi/ndj, the level offsets nlbj and the box counts it is given.  The
+// reference's kernels read their interaction lists from nij/neij; the
+// engine's M2L walks the same interactions by box offsets, and its level-2
+// list is the one fmm/fmm.cxx:104 means (neib 1) whatever the caller built
+// there.  Expansions are mirrored into the caller's bx and ax after each
+// phase in the reference's layout (unscaled, complex, p(p+1)/2 per box;
+// Laplace kernels only): the leaf multipoles after P2M, level lev's rows
+// after M2M, the level's locals after L2L and after M2L.  The reference's
+// m2l also zeroes the bx rows it has used (kernel/m2l.cxx:84-89); the mirror
+// keeps them.
+
+namespace {
+
+// fmmx_get_multipoles_level / fmmx_get_locals rows into a caller array
+void mirror(const char* fn, bool local, int lev, std::complex<double>* dst, int rows) {
+  Shim& s = S();
+  if (!dst || rows <= 0 || s.plug_neq == FMMX_YUKAWA) return;
+  fmmx_handle* h = s.h;
+  const int nb = fmmx_get_count(h, 1, local ? 0 : 1, lev);
+  if (nb != rows) die(fn, "box count " + std::to_string(rows) + " differs from the engine's " + std::to_string(nb));
+  const int nmp = nmp_of(s.p);
+  s.exp_buf.resize((size_t)2 * rows * nmp);
+  check(fn, local ? fmmx_get_locals(h, lev, s.exp_buf.data()) : fmmx_get_multipoles_level(h, lev, s.exp_buf.data()));
+  std::memcpy((void*)dst, s.exp_buf.data(), sizeof(double) * s.exp_buf.size());
+}
+
+// row offset of level lev in the stacked multipoles (leaf first), as
+// fmm/boxm2mj.cxx:8 stacks them in nlbj
+int bx_row(int lev) {
+  int r = 0;
+  for (int l = lmax; l > lev; l--) r += fmmx_get_count(S().h, 1, 1, l);
+  return r;
+}
+
+void check_order(const char* fn, int mp) {
+  Shim& s = S();
+  env_config(s);
+  if (mp != s.p)
+    die(fn, "expansion order mp " + std::to_string(mp) + " differs from the shim's p " + std::to_string(s.p) +
+                " (fmmx_ref_configure or FMMX_REF_P)");
+}
+
+// fmm/boxdatai.cxx:10-34 and boxdataj.cxx:10-36: leaf keys of the caller's
+// (sorted) arrays in array order, then one box per run of equal keys
+int leaf_boxes(const char* fn, int n, const float* x, const float* y, const float* z, int* nf, int** nd,
+               int* ne) {
+  Shim& s = S();
+  sorted_keys(fn, n, x, y, z);
+  std::vector<int> key(std::max(n, 1));
+  for (int k = 0; k < n; k++) key[s.perm[k]] = s.keys[k];
+  if (ne) std::fill(ne, ne + nbmax, -1);
+  int lb = 0;
+  for (int i = 0, c = -1; i < n; i++) {
+    if (key[i] == c) continue;
+    c = key[i];
+    if (ne) ne[c] = lb;
+    nf[lb] = c;
+    nd[0][lb] = i;
+    if (lb > 0) nd[1][lb - 1] = i - 1;
+    lb++;
+  }
+  if (lb > 0) nd[1][lb - 1] = n - 1;
+  return lb;
+}
+
+// fmm/boxm2mi.cxx:6-24, boxm2mj.cxx:6-26: the parent level's keys (key / 8,
+// deduplicated) stacked after the child level's
+void parent_boxes(int& lb, int lev, int* nf, int* nl, int* ne) {
+  nl[lev - 1] = nl[lev] + lb;
+  const int nchild = lb;
+  if (ne) std::fill(ne, ne + nbmax, -1);
+  lb = 0;
+  for (int ii = 0, c = -1; ii < nchild; ii++) {
+    const int pk = nf[ii + nl[lev]] / 8;
+    if (pk == c) continue;
+    c = pk;
+    nf[lb + nl[lev - 1]] = pk;
+    if (ne) ne[pk] = lb;
+    lb++;
+  }
+}
+
+}  // namespace
+
+// kernel/precalc.cxx:10: nmp = mp(mp+1)/2; the engine's tables and
+// operators are built with its handle
+void precalc(int& nmp, int mp) {
+  const double t0 = now();
+  check_order("precalc", mp);
+  (void)handle("precalc");
+  nmp = nmp_of(mp);
+  add_time(3, now() - t0);
+}
+
+void boxdatai(int mi, int& lbi, double& rb) {
+  const double t0 = now();
+  if (!nfi || !ndi) die("boxdatai", "lists not allocated (alloc(1))");
+  rb = rd / (1 << lmax);
+  lbi = leaf_boxes("boxdatai", mi, xi, yi, zi, nfi, ndi, nullptr);
+  S().plug_mi = mi;
+  add_time(10, now() - t0);
+}
+
+void boxdataj(int mj, int& lbj, double& rb) {
+  const double t0 = now();
+  if (!nfj || !ndj || !nej) die("boxdataj", "lists not allocated (alloc(1))");
+  rb = rd / (1 << lmax);
+  lbj = leaf_boxes("boxdataj", mj, xj, yj, zj, nfj, ndj, nej);
+  S().plug_mj = mj;
+  add_time(11, now() - t0);
+}
+
+void boxm2mi(int& lbi, int lev) {
+  const double t0 = now();
+  parent_boxes(lbi, lev, nfi, nlbi, nullptr);
+  add_time(10, now() - t0);
+}
+
+void boxm2mj(int& lbj, int lev) {
+  const double t0 = now();
+  parent_boxes(lbj, lev, nfj, nlbj, nej);
+  add_time(11, now() - t0);
+}
+
+// fmm/ifbox.cxx:7-13 / ijbox.cxx:9-117: lbjr = lbj, the interaction list of
+// level lev in nij/neij (neib 0: near; 1, 2: far; 3, which the reference
+// never builds, gets the far list of neib 1) and njb
+void ifbox(int mi, int mj, int nmp, int lbi, int lbj, int& lbjr, int lev, int neib) {
+  (void)nmp;
+  const double t0 = now();
+  Shim& s = S();
+  s.plug_mi = mi;
+  s.plug_mj = mj;
+  lbjr = lbj;
+  fmmx_handle* h = plug_tree_any("ifbox");
+  if (nij && neij && lbi > 0) {
+    const int nb = fmmx_get_count(h, 1, 0, lev);
+    if (nb != lbi) die("ifbox", "target box count differs from the engine's");
+    check("ifbox", fmmx_get_list(h, lev, neib == 0 ? 0 : 1, nij, &neij[0][0]));
+  }
+  if (njb && nej && nfj && nlbj)
+    for (int jj = 0; jj < lbj; jj++) {
+      const int jb = jj + nlbj[lev - 1];
+      njb[jj] = nej[nfj[jb]] + nlbj[lev - 1];
+    }
+  add_time(13, now() - t0);
+}
+
+// kernel/G_p2p.cxx:7, Gni_p2p.cxx:7, Gnj_p2p.cxx:7, Helmholtz_G_p2p.cxx: the
+// near field added into vi (and gxi.. for G and Yukawa)
+static void p2p_symbol(const char* fn, int neq, int lbi, int lbj) {
+  Shim& s = S();
+  plug_counts(fn, lbi, lbj);
+  s.plug_neq = neq;
+  plug_tree(fn, s.plug_mi, s.plug_mj, neq);
+  plug_phase(fn, FMMX_PHASE_P2P, neq, 0, true, 17);
+}
+void G_p2p(int lbi, int lbj) { p2p_symbol("G_p2p", FMMX_G, lbi, lbj); }
+void Gni_p2p(int lbi, int lbj) { p2p_symbol("Gni_p2p", FMMX_GNI, lbi, lbj); }
+void Gnj_p2p(int lbi, int lbj) { p2p_symbol("Gnj_p2p", FMMX_GNJ, lbi, lbj); }
+void Helmholtz_G_p2p(int lbi, int lbj) { p2p_symbol("Helmholtz_G_p2p", FMMX_YUKAWA, lbi, lbj); }
+
+// kernel/G_p2m.cxx:11 (charges; the sequences of G and Gni), Gn_p2m.cxx:11
+// (dipoles; Gnj), Helmholtz_G_p2m.cxx: leaf multipoles, mirrored into bx
+static void p2m_symbol(const char* fn, bool dipoles, int mp, int lbj) {
+  Shim& s = S();
+  check_order(fn, mp);
+  plug_counts(fn, -1, lbj);
+  if (dipoles) s.plug_neq = FMMX_GNJ;
+  else if (s.plug_neq == FMMX_GNJ) s.plug_neq = FMMX_G;
+  plug_tree_any(fn);
+  const int neq = s.plug_neq;
+  if (dipoles) plug_tree(fn, s.plug_mi, s.plug_mj, FMMX_GNJ);
+  plug_phase(fn, FMMX_PHASE_P2M, neq, 0, false, 18);
+  mirror(fn, false, lmax, bx ? &bx[0][0] : nullptr, lbj);
+}
+void G_p2m(int nmp, int mp, int lbj) { (void)nmp; p2m_symbol("G_p2m", false, mp, lbj); }
+void Gn_p2m(int nmp, int mp, int lbj) { (void)nmp; p2m_symbol("Gn_p2m", true, mp, lbj); }
+void Helmholtz_G_p2m(int nmp, int mp, int lbj) {
+  (void)nmp;
+  S().plug_neq = FMMX_YUKAWA;
+  p2m_symbol("Helmholtz_G_p2m", false, mp, lbj);
+}
+
+// kernel/m2m.cxx:13, Helmholtz_m2m.cxx: multipoles of level lev from lev+1
+static void m2m_symbol(const char* fn, int mp, int lev, int lbj) {
+  check_order(fn, mp);
+  plug_tree_any(fn);
+  plug_phase(fn, FMMX_PHASE_M2M, S().plug_neq, lev, false, 19);
+  mirror(fn, false, lev, bx ? &bx[0][0] + (size_t)bx_row(lev) * nmp_of(mp) : nullptr, lbj);
+}
+void m2m(int nmp, int mp, int lev, int lbj, int lbjo) { (void)nmp; (void)lbjo; m2m_symbol("m2m", mp, lev, lbj); }
+void Helmholtz_m2m(int nmp, int mp, int lev, int lbj, int lbjo) {
+  (void)nmp;
+  (void)lbjo;
+  m2m_symbol("Helmholtz_m2m", mp, lev, lbj);
+}
+
+// kernel/m2l.cxx:13, Helmholtz_m2l.cxx: locals of level lev (level 2 written,
+// ini != 0 as kernel/m2l.cxx:23-28 zeroes them; deeper levels added to what
+// l2l wrote), mirrored into ax
+static void m2l_symbol(const char* fn, int mp, int lbi, int lev) {
+  check_order(fn, mp);
+  plug_tree_any(fn);
+  plug_phase(fn, FMMX_PHASE_M2L, S().plug_neq, lev, false, 20);
+  mirror(fn, true, lev, ax ? &ax[0][0] : nullptr, lbi);
+}
+void m2l(int nmp, int mp, int lbi, int lbj, int lev, int ini) {
+  (void)nmp;
+  (void)lbj;
+  (void)ini;
+  m2l_symbol("m2l", mp, lbi, lev);
+}
+void Helmholtz_m2l(int nmp, int mp, int lbi, int lbj, int lev, int ini) {
+  (void)nmp;
+  (void)lbj;
+  (void)ini;
+  m2l_symbol("Helmholtz_m2l", mp, lbi, lev);
+}
+
+// kernel/l2l.cxx:13: locals of level lev from its parents', mirrored into ax
+void l2l(int nmp, int mp, int lev, int lbi) {
+  (void)nmp;
+  check_order("l2l", mp);
+  plug_tree_any("l2l");
+  plug_phase("l2l", FMMX_PHASE_L2L, S().plug_neq, lev, false, 21);
+  mirror("l2l", true, lev, ax ? &ax[0][0] : nullptr, lbi);
+}
+
+// kernel/G_l2p.cxx:11 (G, and Gnj without the gradient: the shim leaves
+// gxi.. of a Gnj matvec alone), Gn_l2p.cxx:11 (Gni), Helmholtz_G_l2p.cxx:
+// the far field added into vi (and gxi..)
+static void l2p_symbol(const char* fn, int mp, int lbi, bool normal) {
+  Shim& s = S();
+  check_order(fn, mp);
+  plug_counts(fn, lbi, -1);
+  if (normal) s.plug_neq = FMMX_GNI;
+  plug_tree_any(fn);
+  if (normal) plug_tree(fn, s.plug_mi, s.plug_mj, FMMX_GNI);
+  plug_phase(fn, FMMX_PHASE_L2P, s.plug_neq, 0, true, 22);
+}
+void G_l2p(int nmp, int mp, int lbi) { (void)nmp; l2p_symbol("G_l2p", mp, lbi, false); }
+void Gn_l2p(int nmp, int mp, int lbi) { (void)nmp; l2p_symbol("Gn_l2p", mp, lbi, true); }
+void Helmholtz_G_l2p(int nmp, int mp, int lbi) {
+  (void)nmp;
+  S().plug_neq = FMMX_YUKAWA;
+  l2p_symbol("Helmholtz_G_l2p", mp, lbi, false);
+}

@@ -23,21 +23,27 @@ inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
 
 // fmm/boxn.cxx:12-20: (x - xmin) in float, times the scale kl/rd (a float
 // quotient widened to double, fmm/boxn.cxx:9), truncate.
-// The reference clamps only the upper face; coordinates outside the cube are
-// clamped to the nearest box here instead of producing invalid keys.
+// The reference clamps only the upper face.  A coordinate outside the closed
+// cube [min, min + rd] (or not finite) sets *bad, which the host turns into
+// FMMX_EINVAL; its key is clamped to the nearest box so that every index
+// stays in range.
 __global__ void k_keys(int n, const float* __restrict__ x, const float* __restrict__ y,
-                       const float* __restrict__ z, float xmin, float ymin, float zmin, double rbinv,
+                       const float* __restrict__ z, float xmin, float ymin, float zmin, float rd, double rbinv,
                        int lmax, uint32_t* __restrict__ key, uint32_t* __restrict__ idx,
-                       float4* __restrict__ xyz) {
+                       float4* __restrict__ xyz, int* __restrict__ bad) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float px = x[i], py = y[i], pz = z[i];
   // the coordinates packed as well (when xyz is set): the gather into key
   // order then reads one 16-byte record per point instead of three words
-  if (xyz) xyz[i] = make_float4(x[i], y[i], z[i], 0.f);
+  if (xyz) xyz[i] = make_float4(px, py, pz, 0.f);
+  const float d0 = px - xmin, d1 = py - ymin, d2 = pz - zmin;
+  // NaN fails every comparison
+  if (!(d0 >= 0.f && d0 <= rd && d1 >= 0.f && d1 <= rd && d2 >= 0.f && d2 <= rd)) atomicOr(bad, 1);
   int kl = 1 << lmax;
-  int f0 = (int)((double)(x[i] - xmin) * rbinv);
-  int f1 = (int)((double)(y[i] - ymin) * rbinv);
-  int f2 = (int)((double)(z[i] - zmin) * rbinv);
+  int f0 = (int)((double)d0 * rbinv);
+  int f1 = (int)((double)d1 * rbinv);
+  int f2 = (int)((double)d2 * rbinv);
   f0 = min(max(f0, 0), kl - 1);
   f1 = min(max(f1, 0), kl - 1);
   f2 = min(max(f2, 0), kl - 1);
@@ -369,8 +375,11 @@ int Engine::sort_points(int n, const float* x, const float* y, const float* z, i
   }
   const int lm = cfg.lmax;
   double rbinv = (double)((float)(1 << lm) / cfg.rd);
-  k_keys<<<nblk(n), TPB, 0, stream>>>(n, d[0], d[1], d[2], cfg.xmin, cfg.ymin, cfg.zmin, rbinv, lm,
-                                      srt_keys_in.as<uint32_t>(), srt_idx_in.as<uint32_t>(), nullptr);
+  if ((rc = srt_bad.ensure(4))) return rc;
+  k_fill_int<<<1, 32, 0, stream>>>(1, 0, srt_bad.as<int>());
+  k_keys<<<nblk(n), TPB, 0, stream>>>(n, d[0], d[1], d[2], cfg.xmin, cfg.ymin, cfg.zmin, cfg.rd, rbinv, lm,
+                                      srt_keys_in.as<uint32_t>(), srt_idx_in.as<uint32_t>(), nullptr,
+                                      srt_bad.as<int>());
   size_t tb = 0;
   FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
                                      srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
@@ -383,7 +392,10 @@ int Engine::sort_points(int n, const float* x, const float* y, const float* z, i
   const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   FMMX_HIP(hipMemcpyAsync(perm_out, srt_perm.p, nb, kind, stream));
   if (keys_out) FMMX_HIP(hipMemcpyAsync(keys_out, srt_keys.p, nb, kind, stream));
+  int bad = 0;
+  FMMX_HIP(hipMemcpyAsync(&bad, srt_bad.p, 4, hipMemcpyDeviceToHost, stream));
   FMMX_HIP(hipStreamSynchronize(stream));
+  if (bad) return fail(FMMX_EINVAL, "coordinates outside the root cube or not finite");
   return FMMX_OK;
 }
 
@@ -440,8 +452,8 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   if (n == 0) return FMMX_OK;
   if ((rc = xyz4.ensure(nn * 16))) return rc;
   double rbinv = (double)((float)(1 << lmax) / rd);
-  k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rbinv, lmax, s.keys_in.as<uint32_t>(),
-                                      s.idx_in.as<uint32_t>(), xyz4.as<float4>());
+  k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rd, rbinv, lmax, s.keys_in.as<uint32_t>(),
+                                      s.idx_in.as<uint32_t>(), xyz4.as<float4>(), bad_flag.as<int>());
   size_t tb = 0;
   FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
                                      s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,

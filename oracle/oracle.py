@@ -181,12 +181,16 @@ class OracleFMM:
 
 
 def bounding_cube(x):
-    """Root cube as the BEM callers build it (bio/main.cxx:364-380): min corner
-    and the largest extent times (1+1e-5), all in float."""
+    """Root cube as the BEM caller builds it (bio/main.cxx:355-368): float
+    min corner; per axis the float extent xmax-xmin times the double literal
+    1+1e-5, the double product rounded once into the float rdx; rd the
+    largest of rdx, rdy, rdz."""
     x = np.asarray(x, dtype=np.float32)
     lo = x.min(axis=0).astype(np.float32)
     hi = x.max(axis=0).astype(np.float32)
-    rd = np.float32(max(np.float32((hi[k] - lo[k]) * np.float32(1 + 1e-5)) for k in range(3)))
+    ext = (hi - lo).astype(np.float32)  # float - float
+    rdk = (ext.astype(np.float64) * (1 + 1e-5)).astype(np.float32)  # double product -> float
+    rd = np.float32(max(rdk[0], rdk[1], rdk[2]))
     return float(lo[0]), float(lo[1]), float(lo[2]), float(rd)
 
 

@@ -26,7 +26,9 @@
 #include "arrays.h"
 #undef MAIN
 
+#include <algorithm>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 extern void memoryuse();
@@ -173,8 +175,59 @@ static void read_floats(FILE* f, float* d, int n) {
   if (n > 0 && fread(d, 4, (size_t)n, f) != (size_t)n) { fprintf(stderr, "short read\n"); exit(2); }
 }
 
+// The root cube of the BEM caller, bio/main.cxx:355-368 (that file needs
+// PETSc and is not built here): the same statements on the same types --
+// the reference's float globals xmin, ymin, zmin, rd, float locals xmax..,
+// rdx.., the double literal 1+1e-5 -- compiled with the reference's flags.
+// Input: int ncloud, then per cloud int n, float x[n], y[n], z[n].
+// Output: one record box_<k> = {xmin, ymin, zmin, rd} per cloud.
+static int box_mode(const char* fin_path, const char* fout_path) {
+  FILE* fi = fopen(fin_path, "rb");
+  if (!fi) return 2;
+  out = fopen(fout_path, "wb");
+  if (!out) return 2;
+  int ncloud = 0;
+  if (fread(&ncloud, 4, 1, fi) != 1) return 2;
+  for (int k = 0; k < ncloud; k++) {
+    int n = 0;
+    if (fread(&n, 4, 1, fi) != 1) return 2;
+    std::vector<float> px(n), py(n), pz(n);
+    read_floats(fi, px.data(), n);
+    read_floats(fi, py.data(), n);
+    read_floats(fi, pz.data(), n);
+    float xmax, ymax, zmax, rdx, rdy, rdz;
+    xmin = 1e6;
+    xmax = -1e6;
+    ymin = 1e6;
+    ymax = -1e6;
+    zmin = 1e6;
+    zmax = -1e6;
+    for (int l = 0; l < n; ++l) {
+      xmin = std::min(px[l], xmin);
+      xmax = std::max(px[l], xmax);
+      ymin = std::min(py[l], ymin);
+      ymax = std::max(py[l], ymax);
+      zmin = std::min(pz[l], zmin);
+      zmax = std::max(pz[l], zmax);
+    }
+    rdx = (xmax - xmin) * (1 + 1e-5);
+    rdy = (ymax - ymin) * (1 + 1e-5);
+    rdz = (zmax - zmin) * (1 + 1e-5);
+    rd = std::max(rdx, rdy);
+    rd = std::max(rd, rdz);
+    const float b[4] = {xmin, ymin, zmin, rd};
+    char nm[32];
+    snprintf(nm, sizeof nm, "box_%d", k);
+    recf(nm, 4, b);
+  }
+  fclose(fi);
+  fclose(out);
+  return 0;
+}
+
 int main(int argc, char** argv) {
-  if (argc < 3) { fprintf(stderr, "usage: ref_driver in.bin out.bin\n"); return 2; }
+  if (argc == 4 && std::string(argv[1]) == "box") return box_mode(argv[2], argv[3]);
+  if (argc < 3) { fprintf(stderr, "usage: ref_driver in.bin out.bin | ref_driver box in.bin out.bin\n"); return 2; }
   FILE* fi = fopen(argv[1], "rb");
   if (!fi) return 2;
   int hdr[5];

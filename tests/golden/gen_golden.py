@@ -9,7 +9,9 @@ sort permutations, box keys/ranges, near and far interaction lists, precalc
 tables, multipoles after P2M and after the upward pass, local expansions per
 level, P2P-only and final potentials/gradients (sorted and caller order), the
 direct sum.  Yukawa cases (neq 10, the y* names) hold the shipped fmm()'s
-potentials and the direct sum kernel/Helmholtz_G_dir.cxx.  (The shipped fmm() itself is not run: with its level-2 neib=3
+potentials and the direct sum kernel/Helmholtz_G_dir.cxx.  ref_box.npz holds point clouds and
+the root cube bio/main.cxx:355-368 computes for each (the driver's box
+mode).  (The shipped fmm() itself is not run: with its level-2 neib=3
 list it reads past the end of nfj at kernel/m2l.cxx:43, a heap overflow
 under AddressSanitizer.)
 
@@ -120,6 +122,61 @@ def read_dump(path):
     return out
 
 
+def box_clouds():
+    """Point clouds for the root cube of bio/main.cxx:355-368: two-point
+    clouds whose x extent is a float where rounding the double product
+    extent*(1+1e-5) once to float differs from a float product (found by
+    search, plus 0.40973523), and seeded random clouds of several shapes."""
+    rng = np.random.default_rng(40)
+    clouds = []
+    ext = [np.float32(0.40973523)]
+    while len(ext) < 16:
+        e = np.float32(rng.uniform(0.01, 50.0))
+        if np.float32(np.float64(e) * (1 + 1e-5)) != np.float32(e * np.float32(1 + 1e-5)):
+            ext.append(e)
+    for e in ext:
+        lo = np.float32(rng.uniform(-3, 3))
+        x = np.array([[lo, 0.1, 0.2], [np.float32(lo + e), 0.15, 0.25]], np.float32)
+        if np.float32(x[1, 0] - x[0, 0]) != e:  # keep the float extent exact
+            x[1, 0] = np.float32(x[0, 0] + e)
+        clouds.append(x)
+    for k in range(12):
+        n = int(rng.integers(50, 2000))
+        if k % 3 == 0:
+            x = rng.random((n, 3), dtype=np.float32) * np.float32(rng.uniform(0.1, 10)) - np.float32(rng.uniform(0, 5))
+        elif k % 3 == 1:
+            v = rng.standard_normal((n, 3))
+            x = (v / np.linalg.norm(v, axis=1)[:, None] * rng.uniform(0.5, 3)).astype(np.float32)
+        else:
+            x = (rng.standard_normal((n, 3)) * rng.uniform(0.5, 2, 3)).astype(np.float32)
+        clouds.append(x.astype(np.float32))
+    return clouds
+
+
+def write_box_golden(out_dir):
+    clouds = box_clouds()
+    tmp = os.path.join(ORACLE, "_ref", "tmp")
+    os.makedirs(tmp, exist_ok=True)
+    fin, fout = os.path.join(tmp, "box.in"), os.path.join(tmp, "box.out")
+    with open(fin, "wb") as f:
+        f.write(struct.pack("<i", len(clouds)))
+        for x in clouds:
+            f.write(struct.pack("<i", len(x)))
+            for k in range(3):
+                f.write(np.ascontiguousarray(x[:, k], np.float32).tobytes())
+    subprocess.run([os.path.join(ORACLE, "_ref", "ref_driver"), "box", fin, fout], check=True)
+    d = read_dump(fout)
+    keep = {}
+    for k, x in enumerate(clouds):
+        keep[f"cloud_{k}"] = x
+        keep[f"box_{k}"] = d[f"box_{k}"]
+    dst = os.path.join(out_dir, "ref_box.npz")
+    np.savez_compressed(dst, **keep)
+    print(f"box: {len(clouds)} clouds -> {os.path.relpath(dst, ROOT)}")
+    os.remove(fin)
+    os.remove(fout)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -134,6 +191,10 @@ def main():
     # therefore run a build with AddressSanitizer in recover mode, where the
     # extra element lands in a redzone and nothing else is overwritten (same
     # strict floating-point flags, so the arithmetic is unchanged).
+    if not a.only or a.only == "box":
+        write_box_golden(a.out)
+    if a.only == "box":
+        return 0
     asan_flags = "-O1 -fno-fast-math -ffp-contract=off -w -fsanitize=address -fsanitize-recover=address"
     if any(CASES[c][3] == 10 for c in CASES if not a.only or c == a.only):
         subprocess.run(["make", "-s", "-C", ORACLE, "-f", "Makefile.ref", f"REF={a.ref}", "OUT=_ref/asan",

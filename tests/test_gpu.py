@@ -243,6 +243,55 @@ def test_auto_box_and_level():
     f.close()
 
 
+def test_auto_box_matches_reference():
+    """rd <= 0: the engine's root cube equals the one bio/main.cxx:355-368
+    computes (tests/golden/ref_box.npz, from the statements compiled with the
+    reference's flags), bit for bit, including extents where a float product
+    rounds differently."""
+    import os
+
+    from conftest import GOLDEN
+
+    d = np.load(os.path.join(GOLDEN, "ref_box.npz"))
+    f = F.FMM(p=4, lmax=3, precision="fp64", convention="ref")
+    for k in range(len([c for c in d if c.startswith("cloud_")])):
+        x = d[f"cloud_{k}"]
+        f.set_points(x)
+        lm, box = f.domain()
+        assert lm == 3
+        assert np.array_equal(np.array(box, np.float32), d[f"box_{k}"]), (k, box, d[f"box_{k}"])
+    f.close()
+
+
+def test_invalid_coordinates_and_groups():
+    """FMMX_EINVAL for NaN coordinates and for points outside an explicit
+    root cube (include/fmmx.h), and for a group level deeper than lmax."""
+    x, q = uniform_cube(2000, 71)
+    f = F.FMM(p=4, lmax=3, precision="fp64", convention="exact", box=(0, 0, 0, 1))
+    for bad in (np.nan, 1.5, -0.25, np.inf):
+        xb = x.copy()
+        xb[17, 1] = bad
+        with pytest.raises(F.FMMError, match="outside the root cube"):
+            f.set_points(xb)
+        with pytest.raises(F.FMMError):
+            f.evaluate(q)  # no tree after a failed set_points
+    f.set_points(x)  # recovers
+    v, _ = f.evaluate(q)
+    assert np.isfinite(v).all()
+    g = F.FMM(p=4, lmax=0, precision="fp64", convention="exact")
+    xn = x.copy()
+    xn[5, 2] = np.nan
+    with pytest.raises(F.FMMError, match="non-finite"):
+        g.set_points(xn)
+    g.close()
+    f.set_groups(4)  # deeper than lmax 3
+    with pytest.raises(F.FMMError, match="group level"):
+        f.set_points(x)
+    f.set_groups(2)
+    f.set_points(x)
+    f.close()
+
+
 def test_device_pointers_match_host():
     x, q = uniform_cube(30000, 70)
     f = F.FMM(p=10, lmax=4, precision="mixed", convention="exact", box=(0, 0, 0, 1))

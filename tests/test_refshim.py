@@ -34,15 +34,17 @@ def _read_output(path):
         name = data[p:p + 32].split(b"\0", 1)[0].decode()
         typ, n = struct.unpack_from("<2i", data, p + 32)
         p += 40
-        out[name] = np.frombuffer(data[p:p + 4 * n], dtype=np.int32 if typ == 0 else np.float32).copy()
-        p += 4 * n
+        dt = {0: np.int32, 1: np.float32, 2: np.float64}[typ]
+        es = np.dtype(dt).itemsize
+        out[name] = np.frombuffer(data[p:p + es * n], dtype=dt).copy()
+        p += es * n
     return out
 
 
-def _run(d, tmp_path, env=None):
+def _run(d, tmp_path, env=None, exe=CALLER, args=()):
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     _write_input(fin, d)
-    r = subprocess.run([CALLER, fin, fout], capture_output=True, text=True, timeout=120,
+    r = subprocess.run([exe, fin, fout, *args], capture_output=True, text=True, timeout=120,
                        env=dict(os.environ, **(env or {})))
     return r, (_read_output(fout) if r.returncode == 0 else None)
 
@@ -76,6 +78,78 @@ def test_reference_caller_yukawa(case, tmp_path):
     e_ref = rel_l2(d["fmm_vi"], d["dir_vi"])
     assert rel_l2(o["fmm_vi"], d["dir_vi"]) < 0.1 * e_ref
     assert rel_l2(o["fmm2_vi_sorted"], 2 * o["fmm_vi_sorted"]) < 1e-6
+    assert rel_l2(o["dir_vi"], d["dir_vi"]) < 1e-7
+
+
+def _check_outputs(o, d, neq):
+    """The matvec outputs of a caller run against what the reference wrote."""
+    assert rel_l2(o["fmm_vi_sorted"], d["fmm_vi_sorted"]) < 2e-6
+    assert rel_l2(o["fmm_vi"], d["fmm_vi"]) < 2e-6
+    if neq == 0:
+        g = np.stack([o["fmm_gxi"], o["fmm_gyi"], o["fmm_gzi"]], 1)
+        assert rel_l2(g, np.stack([d["fmm_gxi"], d["fmm_gyi"], d["fmm_gzi"]], 1)) < 2e-6
+    # second matvec on the cached tree, strengths doubled
+    assert rel_l2(o["fmm2_vi_sorted"], 2 * o["fmm_vi_sorted"]) < 1e-6
+    assert np.array_equal(o["nbi"], d["nbi"]) and np.array_equal(o["nbj"], d["nbj"])
+
+
+def _cplx(a):
+    return a.view(np.complex128)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_operator_symbols_match_reference(case, tmp_path):
+    """The operator plugin (the symbols fmm/fmm.cxx:7-27 calls) sequenced as
+    fmm/fmm.cxx:38-150 does, phase by phase, against the reference's own
+    dumps: stacked box keys bit for bit, far lists bit for bit (level 2: the
+    neib-1 list), multipoles after P2M and after the upward pass, locals of
+    every level after its M2L (FP64 engine, REF convention: 1e-10), near
+    field and outputs (float, 2e-6)."""
+    _need_caller()
+    d = golden(case)
+    ni, nj, lmax, neq = (int(v) for v in d["in_cfg"][:4])
+    print(f"case={case} neq={neq} lmax={lmax}", flush=True)
+    r, o = _run(d, tmp_path, args=("plugin",))
+    assert r.returncode == 0, r.stderr
+    for k in ("nlbi", "nlbj", "all_nfi", "all_nfj"):
+        assert np.array_equal(o[k], d[k]), k
+    for lev in range(2, lmax + 1):
+        assert np.array_equal(o[f"far_nij_{lev}"], d[f"far_nij_{lev}"]), lev
+        assert np.array_equal(o[f"far_neij_{lev}"], d[f"far_neij_{lev}"]), lev
+    assert rel_l2(o["p2p_vi"], d["p2p_vi"]) < 2e-6
+    if "p2m_bx" in d:
+        assert rel_l2(_cplx(o["p2m_bx"]), _cplx(d["p2m_bx"])) < 1e-10
+        assert rel_l2(_cplx(o["up_bx"]), _cplx(d["up_bx"])) < 1e-10
+        for lev in range(2, lmax + 1):
+            e = rel_l2(_cplx(o[f"ax_{lev}"]), _cplx(d[f"ax_{lev}"]))
+            assert e < 1e-10, (lev, e)
+    _check_outputs(o, d, neq)
+
+
+REF_FMM = os.path.join(ROOT, "oracle", "_ref", "ref_caller_fmm")
+REF_KERNELS = os.path.join(ROOT, "oracle", "_ref", "ref_caller_kernels")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exe", [REF_FMM, REF_KERNELS], ids=["fmm.o", "fmm.o+tree"])
+@pytest.mark.parametrize("case", ["g3", "gni3", "gnj3", "g2", "gedge4"])
+def test_reference_fmm_object_with_shim_kernels(exe, case, tmp_path):
+    """The reference's fmm/fmm.cxx compiled from its sources (oracle/Makefile.ref
+    "callers", built where the reference is present) linked against the
+    shim: with the shim's box symbols (fmm.o), and with all of the reference's
+    tree, list and set-up objects so that only the kernels come from the shim
+    (fmm.o+tree, biobem/Makefile:25-37).  Its own level-2 call asks for the
+    list neib 3, which fmm/ijbox.cxx never builds; the shim's m2l uses the
+    full level-2 list, so the outputs are those of the golden driver."""
+    if not os.path.exists(exe):
+        pytest.skip(f"{os.path.relpath(exe, ROOT)} not built (needs the reference sources)")
+    d = golden(case)
+    ni, nj, lmax, neq = (int(v) for v in d["in_cfg"][:4])
+    print(f"exe={os.path.basename(exe)} case={case} neq={neq} lmax={lmax}", flush=True)
+    r, o = _run(d, tmp_path, exe=exe)
+    assert r.returncode == 0, r.stderr
+    _check_outputs(o, d, neq)
     assert rel_l2(o["dir_vi"], d["dir_vi"]) < 1e-7
 
 
