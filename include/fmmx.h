@@ -146,6 +146,34 @@ enum {
 int fmmx_evaluate_phase(fmmx_handle* h, int phase, int neq, int lev, const float* qj, double* vi, double* gxi,
                         double* gyi, double* gzi, int flags);
 
+/* ---- Adjoint of the matvec v = K q (fmmx_evaluate with kernel neq) ----
+ * No reference counterpart (the reference has no derivatives); built from
+ * its kernels.  Every entry recomputes the expansions it needs.
+ *
+ * fmmx_evaluate_transpose: u = K^T w for strengths wi on the targets (caller
+ * order), one value per source (caller order).  G, Yukawa and Helmholtz are
+ * symmetric kernels; the transpose of Gni (kernel/Gni_p2p.cxx:7-32, target
+ * normals) is minus Gnj (kernel/Gnj_p2p.cxx:7-32) carrying the target
+ * normals, and the other way round.  For G and Yukawa g*j (all three or
+ * none) receive the gradient of the transposed field at the sources.  The
+ * transposed problem runs on a second tree of the same root cube and depth
+ * whose targets are the sources, built at the first call after set_points.
+ * Helmholtz: wi and uj complex pairs as in fmmx_evaluate. */
+int fmmx_evaluate_transpose(fmmx_handle* h, int neq, const float* wi, double* uj, double* gxj, double* gyj,
+                            double* gzj, int flags);
+/* Gradients of L = sum_i w_i (K q)_i for G or Yukawa: dq[j] = dL/dq_j =
+ * (K^T w)_j; dxi = dL/dx_i = w_i grad phi(x_i) (the field gradient of
+ * kernel/G_l2p.cxx:68-70); dxj = dL/dx_j = q_j grad psi(x_j) with
+ * psi = K^T w.  dxi, dxj: x, y and z components as three consecutive blocks
+ * of ni (nj) doubles.  Any output may be NULL. */
+int fmmx_gradients(fmmx_handle* h, int neq, const float* qj, const float* wi, double* dq, double* dxi,
+                   double* dxj, int flags);
+/* The energy E = 1/2 q . K q of one point set (set_points with targets =
+ * sources; the self pair excluded in FMMX_CONV_EXACT) for G or Yukawa, and
+ * its gradients dEdq = K q and dEdx = q_k grad phi(x_k) (three blocks of n).
+ * E is summed on the device in a fixed order.  Any output may be NULL. */
+int fmmx_energy(fmmx_handle* h, int neq, const float* q, double* E, double* dEdq, double* dEdx, int flags);
+
 /* Direct sums (kernel/G_dir.cxx:6, Gni_dir.cxx:6, Gnj_dir.cxx:6) on the GPU
  * for ni targets against nj sources, FP64 accumulation, in the handle's
  * convention.  Stateless apart from the handle's device and stream (and its

@@ -77,6 +77,11 @@ _SYMBOLS = {
     "fmmx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "fmmx_get_count": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "fmmx_get_domain": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fmmx_evaluate_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
+    "fmmx_get_multipoles_level": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "fmmx_evaluate_transpose": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
+    "fmmx_gradients": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
+    "fmmx_energy": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 4 + [C.c_int]),
     "fmmx_get_perm": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fmmx_get_boxes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fmmx_get_list": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -345,6 +350,77 @@ class FMM:
         if want_g:
             return v, g.T
         return v, None
+
+    # -- adjoint ----------------------------------------------------------
+    def _out(self, shape, like_dev):
+        if like_dev:
+            import torch
+            return torch.empty(shape, dtype=torch.float64, device="cuda"), (lambda t: t.data_ptr())
+        return np.empty(shape), (lambda t: t.ctypes.data)
+
+    def transpose(self, w, neq=G, gradient=False):
+        """u = K^T w (one value per source, caller order) for strengths w on
+        the targets; for G and YUKAWA with gradient also the gradient of the
+        transposed field at the sources (nj, 3).  HELMHOLTZ: complex w, u."""
+        dev = self._device
+        if neq == HELMHOLTZ:
+            wp = self._pairs(w, dev)
+            u, ptr = self._out((self.nj, 2), dev)
+            _check(lib().fmmx_evaluate_transpose(self.h, neq, wp.data_ptr() if dev else wp.ctypes.data, ptr(u),
+                                                 None, None, None, 1 if dev else 0))
+            if dev:
+                import torch
+                return torch.view_as_complex(u), None
+            return u[:, 0] + 1j * u[:, 1], None
+        wa = _Arr(w, dev)
+        want_g = gradient and neq in (G, YUKAWA)
+        u, ptr = self._out(self.nj, dev)
+        g, _ = self._out((3, self.nj), dev) if want_g else (None, None)
+        gp = [ptr(g[k]) for k in range(3)] if want_g else [None] * 3
+        _check(lib().fmmx_evaluate_transpose(self.h, neq, wa.ptr, ptr(u), *gp, 1 if dev else 0))
+        return u, (g.T if want_g else None)
+
+    def gradients(self, q, w, neq=G):
+        """Gradients of L = w . (K q) for G (or YUKAWA): dL/dq (nj,),
+        dL/dx_i (ni, 3) and dL/dx_j (nj, 3), caller order."""
+        dev = self._device
+        qa, wa = _Arr(q, dev), _Arr(w, dev)
+        dq, ptr = self._out(self.nj, dev)
+        dxi, _ = self._out((3, self.ni), dev)
+        dxj, _ = self._out((3, self.nj), dev)
+        _check(lib().fmmx_gradients(self.h, neq, qa.ptr, wa.ptr, ptr(dq), ptr(dxi), ptr(dxj), 1 if dev else 0))
+        return dq, dxi.T, dxj.T
+
+    def energy(self, q, neq=G):
+        """E = 1/2 q . K q on one point set (targets = sources) and its
+        gradients dE/dq (n,) and dE/dx (n, 3)."""
+        dev = self._device
+        qa = _Arr(q, dev)
+        e = np.zeros(1)
+        dq, ptr = self._out(self.ni, dev)
+        dx, _ = self._out((3, self.ni), dev)
+        if dev:
+            import torch
+            et = torch.zeros(1, dtype=torch.float64, device="cuda")
+            _check(lib().fmmx_energy(self.h, neq, qa.ptr, et.data_ptr(), ptr(dq), ptr(dx), 1))
+            return float(et.item()), dq, dx.T
+        _check(lib().fmmx_energy(self.h, neq, qa.ptr, e.ctypes.data, ptr(dq), ptr(dx), 0))
+        return float(e[0]), dq, dx.T
+
+    def evaluate_phase(self, phase, q=None, neq=G, lev=0, gradient=True):
+        """One phase of the matvec (P2P=0, P2M=1, M2M=2, M2L=3, L2L=4,
+        L2P=5; fmmx_evaluate_phase).  P2P and L2P return (v, g); the others
+        None."""
+        dev = self._device
+        qa = _Arr(q, dev) if q is not None else None
+        writes = phase in (0, 5)
+        want_g = writes and gradient and neq in (G, YUKAWA)
+        v, ptr = self._out(self.ni, dev) if writes else (None, None)
+        g, _ = self._out((3, self.ni), dev) if want_g else (None, None)
+        gp = [ptr(g[k]) for k in range(3)] if want_g else [None] * 3
+        _check(lib().fmmx_evaluate_phase(self.h, phase, neq, lev, qa.ptr if qa else None, ptr(v) if writes else None,
+                                         *gp, 1 if dev else 0))
+        return (v, g.T if want_g else None) if writes else None
 
     @staticmethod
     def _pairs(q, dev):

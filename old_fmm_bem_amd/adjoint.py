@@ -1,73 +1,62 @@
-"""Adjoints of the FMM matvec and a torch.autograd wrapper.
+"""Adjoints of the FMM matvec and a torch.autograd wrapper, over the C ABI
+(fmmx_evaluate_transpose, fmmx_gradients, fmmx_energy in include/fmmx.h).
 
-The matvec phi = K q of kernel/G_p2p.cxx (K_ij = G(x_i - x_j), G(r) =
-1/(4 pi |r|)) is symmetric when targets and sources are the same points, and
-its derivative with respect to the points is the field gradient the forward
-pass already returns (kernel/G_l2p.cxx:68-70 writes it as gxi, gyi, gzi).
-For L = sum_i w_i phi_i:
+For v = K q (kernel/G_p2p.cxx: K_ij = G(x_i - x_j), G(r) = 1/(4 pi |r|))
+and L = sum_i w_i v_i:
 
-    dL/dq_k = (K^T w)_k = (K w)_k
-    dL/dx_k = w_k grad phi_q(x_k) + q_k grad phi_w(x_k)
+    dL/dq   = K^T w                       (the transposed matvec)
+    dL/dx_i = w_i grad phi_q(x_i)          (kernel/G_l2p.cxx:68-70)
+    dL/dx_j = q_j grad psi_w(x_j),  psi_w = K^T w
 
-(G is even, so the gradient of G(x_i - x_k) with respect to x_k is minus the
-gradient with respect to x_i).  The backward pass therefore runs one more
-matvec with strengths w on the tree of the forward pass; expansions are
-recomputed, not stored.  For the double-layer kernels on one point set with
-one set of normals, K_Gni^T = -K_Gnj (kernel/Gni_p2p.cxx vs Gnj_p2p.cxx:
-(x_i - x_j).n_i = -(x_j - x_i).n_i).
+The engine computes all three on the device: one forward evaluation with
+the field gradient and one transposed evaluation on a second tree whose
+targets are the sources (expansions recomputed, not stored).  The transpose
+of the target-normal kernel Gni (kernel/Gni_p2p.cxx) is minus the
+source-normal kernel Gnj (kernel/Gnj_p2p.cxx) carrying the target normals.
 """
 from __future__ import annotations
 
-import numpy as np
-
-from . import FMM, G, GNI, GNJ
+from . import FMM, G
 
 
 def adjoint(f: FMM, w, neq=G):
-    """K^T w for a handle whose targets and sources are the same points
-    (set_points(x) or set_points(x, x, ...)); for Gni/Gnj the handle must
-    hold the same normals on both sides."""
-    if neq == G:
-        v, _ = f.evaluate(w, neq=G, gradient=False)
-        return v
-    v, _ = f.evaluate(w, neq=GNJ if neq == GNI else GNI)
-    return -v
+    """K^T w: one value per source of the handle's tree (caller order)."""
+    return f.transpose(w, neq=neq)[0]
 
 
-def energy_and_gradients(f: FMM, q):
-    """E = 1/2 q.K q, dE/dq = K q and dE/dx_k = q_k grad phi(x_k) for the
-    single-layer kernel on the handle's point set (targets = sources)."""
-    v, g = f.evaluate(q, neq=G)
-    if isinstance(v, np.ndarray):
-        qd = np.asarray(q, np.float64)
-        return 0.5 * float(qd @ v), v, qd[:, None] * g
-    qd = q.to(v.dtype)
-    return 0.5 * float((qd * v).sum()), v, qd[:, None] * g
+def energy_and_gradients(f: FMM, q, neq=G):
+    """E = 1/2 q.K q, dE/dq = K q and dE/dx_k = q_k grad phi(x_k) on the
+    handle's point set (targets = sources)."""
+    return f.energy(q, neq=neq)
 
 
-def potential(x, q, p=10, lmax=0, precision="mixed", convention="exact", box=None, device=0):
-    """Differentiable phi = K q (float64) on the points x (targets = sources),
-    torch CUDA tensors x (n, 3) and q (n,).  Gradients flow to q and x."""
+def potential(x, q, xj=None, p=10, lmax=0, precision="mixed", convention="exact", box=None, device=0):
+    """Differentiable phi = K q (float64) at the targets x from charges q at
+    the sources xj (default: the targets themselves), torch CUDA tensors.
+    Gradients flow to q, x and xj through fmmx_gradients."""
     import torch
+
+    same = xj is None
 
     class _Potential(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, xx, qq):
+        def forward(ctx, xx, qq, yy):
             f = FMM(p=p, lmax=lmax, precision=precision, convention=convention, box=box, device=device)
-            f.set_points(xx.detach())
-            v, g = f.evaluate(qq.detach().to(torch.float32))
+            f.set_points(xx.detach(), None if same else yy.detach())
+            v, _ = f.evaluate(qq.detach().to(torch.float32), gradient=False)
             ctx.f = f
-            ctx.save_for_backward(qq, g)
+            ctx.save_for_backward(qq)
             return v
 
         @staticmethod
         def backward(ctx, w):
-            qq, g_q = ctx.saved_tensors
+            (qq,) = ctx.saved_tensors
             f = ctx.f
-            v_w, g_w = f.evaluate(w.detach().to(torch.float32).contiguous())
+            dq, dxi, dxj = f.gradients(qq.detach().to(torch.float32), w.detach().to(torch.float32).contiguous())
             f.close()
             ctx.f = None
-            gx = w[:, None] * g_q + qq.to(torch.float64)[:, None] * g_w
-            return gx.to(x.dtype), v_w.to(qq.dtype)
+            if same:
+                return (dxi + dxj).to(x.dtype), dq.to(qq.dtype), None
+            return dxi.to(x.dtype), dq.to(qq.dtype), dxj.to(xj.dtype)
 
-    return _Potential.apply(x, q)
+    return _Potential.apply(x, q, x if same else xj)

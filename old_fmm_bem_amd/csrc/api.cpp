@@ -527,6 +527,7 @@ int Engine::set_domain(int lm, float x0, float y0, float z0, float r) {
   cfg.rd = r;
   user_box = r > 0;
   have_points = false;  // the tree of the previous domain no longer applies
+  tr_valid = false;
   return FMMX_OK;
 }
 
@@ -541,6 +542,7 @@ int Engine::set_groups(int level) {
   drop_graph();
   group_level = level;
   have_points = false;  // the near lists depend on it
+  tr_valid = false;
   return FMMX_OK;
 }
 
@@ -642,6 +644,7 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
     if (rc) return rc;
   }
   have_points = false;
+  tr_valid = false;
   FMMX_HIP(hipEventRecord(ev[0], stream));
   // root cube
   if (user_box) {
@@ -1311,6 +1314,23 @@ int fmmx_evaluate_phase(fmmx_handle* h, int phase, int neq, int lev, const float
                         double* gyi, double* gzi, int flags) {
   if (!h) return fmmx::fail(FMMX_EINVAL, "NULL handle");
   FMMX_GUARD({ return h->e.phase(phase, neq, lev, qj, vi, gxi, gyi, gzi, flags); })
+}
+
+int fmmx_evaluate_transpose(fmmx_handle* h, int neq, const float* wi, double* uj, double* gxj, double* gyj,
+                            double* gzj, int flags) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "NULL handle");
+  FMMX_GUARD({ return h->e.evaluate_transpose(neq, wi, uj, gxj, gyj, gzj, flags); })
+}
+
+int fmmx_gradients(fmmx_handle* h, int neq, const float* qj, const float* wi, double* dq, double* dxi, double* dxj,
+                   int flags) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "NULL handle");
+  FMMX_GUARD({ return h->e.gradients(neq, qj, wi, dq, dxi, dxj, flags); })
+}
+
+int fmmx_energy(fmmx_handle* h, int neq, const float* q, double* E, double* dEdq, double* dEdx, int flags) {
+  if (!h) return fmmx::fail(FMMX_EINVAL, "NULL handle");
+  FMMX_GUARD({ return h->e.energy(neq, q, E, dEdq, dEdx, flags); })
 }
 
 int fmmx_get_domain(fmmx_handle* h, int* lmax, float* box4) {

@@ -163,6 +163,16 @@ struct Engine {
   DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm, srt_bad;
   DevBuf bad_flag;  // set by the key kernel for coordinates outside the root cube
   int last_neq = -1;
+  // The transposed problem (targets: this handle's sources, sources: its
+  // targets) in a second engine on the same root cube and depth, built at
+  // the first transposed evaluation after set_points (adjoint.cpp)
+  std::unique_ptr<Engine> tr;
+  bool tr_valid = false;
+  DevBuf adj_w, adj_wk, adj_q, adj_w2, adj_t[19];  // adjoint staging and temporaries
+  int transposed(Engine*& t);
+  int evaluate_transpose(int neq, const float* wi, double* uj, double* gxj, double* gyj, double* gzj, int flags);
+  int gradients(int neq, const float* qj, const float* wi, double* dq, double* dxi, double* dxj, int flags);
+  int energy(int neq, const float* qj, double* E, double* dEdq, double* dEdx, int flags);
   // timing
   bool per_phase = false;
   hipEvent_t ev[16] = {};

@@ -147,3 +147,108 @@ def test_transpose_identity(neq):
     # normwise: a wrong sign or a missing transpose is off by ~|u||Kv|/sqrt(n)
     assert abs(lhs - rhs) <= 1e-6 * np.linalg.norm(u) * np.linalg.norm(Kv)
     f.close()
+
+
+def _two_sets(seed, ni=9000, nj=11000):
+    rng = np.random.default_rng(seed)
+    xi = rng.random((ni, 3)).astype(np.float32)
+    xj = (0.05 + 0.9 * rng.random((nj, 3))).astype(np.float32)
+    ni_n = rng.standard_normal((ni, 3)).astype(np.float32)
+    nj_n = rng.standard_normal((nj, 3)).astype(np.float32)
+    q = (2 * rng.random(nj) - 1).astype(np.float32)
+    w = (2 * rng.random(ni) - 1).astype(np.float32)
+    return xi, xj, ni_n, nj_n, q, w
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 3e-6)])
+@pytest.mark.parametrize("neq", [0, 1, 2])
+def test_transpose_distinct_sets(neq, precision, tol):
+    """fmmx_evaluate_transpose with different target and source sets: K^T w
+    against the oracle's direct sum with the roles exchanged (G symmetric;
+    Gni^T = -Gnj carrying the target normals, Gnj^T = -Gni carrying the
+    source normals), the identity <w, K q> = <K^T w, q>, and for G the
+    gradient of the transposed field at the sources."""
+    import old_fmm_bem_amd as F
+
+    xi, xj, ni_n, nj_n, q, w = _two_sets(50 + neq)
+    print(f"neq={neq} precision={precision}", flush=True)
+    f = F.FMM(p=12, lmax=4, precision=precision, convention="exact", box=(0, 0, 0, 1))
+    f.set_points(xi, xj, ni_normals=ni_n, nj_normals=nj_n)
+    u, gu = f.transpose(w, neq=neq, gradient=(neq == 0))
+    if neq == 0:
+        ou, og = O.direct(xj, xi, w, conv=O.EXACT)
+    elif neq == 1:
+        ou, _ = O.direct(xj, xi, w, neq=2, conv=O.EXACT, nj_normals=ni_n)
+        ou = -ou
+    else:
+        ou, _ = O.direct(xj, xi, w, neq=1, conv=O.EXACT, ni_normals=nj_n)
+        ou = -ou
+    e = rel_l2(u, ou)
+    print(f"K^T w vs oracle {e:.2e}", flush=True)
+    assert e < 10 * tol
+    if neq == 0:
+        assert rel_l2(gu, og) < 30 * tol
+    Kq, _ = f.evaluate(q, neq=neq, gradient=False)
+    lhs, rhs = float(w.astype(np.float64) @ Kq), float(u @ q.astype(np.float64))
+    assert abs(lhs - rhs) <= tol * np.linalg.norm(w) * np.linalg.norm(Kq)
+    f.close()
+
+
+@pytest.mark.gpu
+def test_gradients_distinct_sets_vs_direct_and_differences():
+    """fmmx_gradients for L = w . K q with different target and source sets
+    (FP64, p = 12): dL/dq, dL/dx_i and dL/dx_j against the oracle's direct
+    sums, and dL/dx_j against central differences of L computed by the FMM
+    itself (h = 1e-4, one Richardson step) to 1e-6."""
+    import old_fmm_bem_amd as F
+
+    xi, xj, _, _, q, w = _two_sets(60, ni=6000, nj=7000)
+    f = F.FMM(p=12, lmax=3, precision="fp64", convention="exact", box=(0, 0, 0, 1))
+    f.set_points(xi, xj)
+    dq, dxi, dxj = f.gradients(q, w)
+    ov_w, og_w = O.direct(xj, xi, w, conv=O.EXACT)
+    _, og_q = O.direct(xi, xj, q, conv=O.EXACT)
+    w64, q64 = w.astype(np.float64)[:, None], q.astype(np.float64)[:, None]
+    e = (rel_l2(dq, ov_w), rel_l2(dxi, w64 * og_q), rel_l2(dxj, q64 * og_w))
+    print("dq %.2e dxi %.2e dxj %.2e" % e, flush=True)
+    assert max(e) < 1e-6
+
+    def loss(xx):
+        f.set_points(xi, xx)
+        v, _ = f.evaluate(q, gradient=False)
+        return float(w.astype(np.float64) @ v)
+
+    rng = np.random.default_rng(61)
+    ok = np.flatnonzero(_away_from_faces(xj, 3, 3e-4))
+    picks, axes = rng.choice(ok, 24, replace=False), rng.integers(0, 3, 24)
+    fd = np.array([central_difference(loss, xj, k, a) for k, a in zip(picks, axes)])
+    an = np.array([dxj[k, a] for k, a in zip(picks, axes)])
+    print("max |fd - an| / max |an| =", np.abs(fd - an).max() / np.abs(an).max(), flush=True)
+    assert np.abs(fd - an).max() <= 1e-6 * np.abs(an).max()
+    f.close()
+
+
+@pytest.mark.gpu
+def test_autograd_two_point_sets():
+    """torch.autograd through potential(x, q, xj): gradients of L = w . phi
+    reach q, the targets and the sources, and equal fmmx_gradients."""
+    import torch
+
+    import old_fmm_bem_amd as F
+    from old_fmm_bem_amd.adjoint import potential
+
+    xi, xj, _, _, q, w = _two_sets(62, ni=5000, nj=6000)
+    xt = torch.from_numpy(xi).cuda().requires_grad_(True)
+    yt = torch.from_numpy(xj).cuda().requires_grad_(True)
+    qt = torch.from_numpy(q).cuda().requires_grad_(True)
+    wt = torch.from_numpy(w).cuda().to(torch.float64)
+    phi = potential(xt, qt, yt, p=10, lmax=3, precision="fp64", box=(0, 0, 0, 1))
+    (wt * phi).sum().backward()
+    f = F.FMM(p=10, lmax=3, precision="fp64", convention="exact", box=(0, 0, 0, 1))
+    f.set_points(xi, xj)
+    dq, dxi, dxj = f.gradients(q, w)
+    assert rel_l2(qt.grad.cpu().numpy(), dq) < 1e-6
+    assert rel_l2(xt.grad.cpu().numpy(), dxi) < 1e-6
+    assert rel_l2(yt.grad.cpu().numpy(), dxj) < 1e-6
+    f.close()
