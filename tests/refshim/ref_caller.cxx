@@ -155,6 +155,22 @@ int main(int argc, char** argv) {
   nbi = new int[np];
   nbj = new int[np];
   tfmm = new double[100]();
+#ifdef REF_GLOBALS
+  // the work arrays of the reference's own box and sort objects, as
+  // test/test.cxx:42-65 allocates them (nc holds one count per leaf box)
+  {
+    const int nw = std::max(np, 1 << (3 * lev));
+    nfn = new int[nw];
+    na = new int[nw];
+    nb = new int[nw];
+    nc = new int[nw];
+    nd = new int[nw];
+    sortd = new float[nw];
+    nprocs = 1;
+    myrank = 0;
+    umem = 0;
+  }
+#endif
   float* rd_i[6] = {xi, yi, zi, gxi, gyi, gzi};
   for (auto a : rd_i)
     if (std::fread(a, 4, ni, in) != (size_t)ni) return 2;

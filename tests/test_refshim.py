@@ -93,10 +93,6 @@ def _check_outputs(o, d, neq):
     assert np.array_equal(o["nbi"], d["nbi"]) and np.array_equal(o["nbj"], d["nbj"])
 
 
-def _cplx(a):
-    return a.view(np.complex128)
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", GOLDEN_CASES)
 def test_operator_symbols_match_reference(case, tmp_path):
@@ -112,18 +108,22 @@ def test_operator_symbols_match_reference(case, tmp_path):
     print(f"case={case} neq={neq} lmax={lmax}", flush=True)
     r, o = _run(d, tmp_path, args=("plugin",))
     assert r.returncode == 0, r.stderr
-    for k in ("nlbi", "nlbj", "all_nfi", "all_nfj"):
+    # level offsets of the stacked lists: entries 1..lmax-1 (entry 0, level 1,
+    # is never written by the reference, fmm/boxm2mi.cxx:8)
+    for k in ("nlbi", "nlbj"):
+        assert np.array_equal(o[k][1:lmax], d[k][1:lmax]), k
+    for k in ("all_nfi", "all_nfj"):
         assert np.array_equal(o[k], d[k]), k
     for lev in range(2, lmax + 1):
         assert np.array_equal(o[f"far_nij_{lev}"], d[f"far_nij_{lev}"]), lev
         assert np.array_equal(o[f"far_neij_{lev}"], d[f"far_neij_{lev}"]), lev
     assert rel_l2(o["p2p_vi"], d["p2p_vi"]) < 2e-6
     if "p2m_bx" in d:
-        assert rel_l2(_cplx(o["p2m_bx"]), _cplx(d["p2m_bx"])) < 1e-10
-        assert rel_l2(_cplx(o["up_bx"]), _cplx(d["up_bx"])) < 1e-10
+        for k in ("p2m_bx", "up_bx"):
+            assert np.abs(o[k] - d[k]).max() <= 1e-11 * np.abs(d[k]).max(), k
         for lev in range(2, lmax + 1):
-            e = rel_l2(_cplx(o[f"ax_{lev}"]), _cplx(d[f"ax_{lev}"]))
-            assert e < 1e-10, (lev, e)
+            ref = d[f"ax_{lev}"]
+            assert np.abs(o[f"ax_{lev}"] - ref).max() <= 1e-10 * np.abs(ref).max(), lev
     _check_outputs(o, d, neq)
 
 
