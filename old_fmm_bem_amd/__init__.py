@@ -235,6 +235,10 @@ class FMM:
         lmax = self.lmax
         shapes = [(1, 0, 8)] if self.precision == FP64 else \
             [(1, 0, 8), (1, 0, 4), (1, 1, 8)] + ([(1, 2, 16), (1, 2, 8)] if self.p <= 11 else [])
+        if self.precision != FP64 and neq in (G, GNI, GNJ):
+            # the error-compensated FP16 kernel (falls back to the FP32 shape
+            # where it does not apply: Yukawa, REF convention, p outside 6..12)
+            shapes += [(1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
         for lev in range(lmax, 1, -1):
             if self.nboxes("i", lev) == 0:
                 continue

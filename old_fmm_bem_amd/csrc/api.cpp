@@ -10,6 +10,7 @@
 
 #include "engine.h"
 #include "helmholtz.h"
+#include "m2l.h"
 #include "yukawa.h"
 #include "tree_util.h"
 
@@ -151,6 +152,10 @@ struct OpSet {
   std::vector<double> m2l_frag;  // MFMA fragment order (FP32 engine)
   std::vector<double> m2l_canon; // fragment order, the 56 offsets with dx, dy, dz >= 0 (EXACT)
   std::vector<double> m2l_kmaj;  // transposed [op][k][row] (FP64 engine)
+  // FP16 split M2L (EXACT, p 6..12): hi/lo fragments of the 56 canonical
+  // operators scaled by 2^split_e
+  std::vector<uint4> m2l_split;
+  int split_e = 0;
   int nlev = 1;
 };
 std::mutex g_cache_mu;
@@ -246,6 +251,17 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
       canon_vector(ci, d);
       const int oi = offset_index(d[0], d[1], d[2]);
       std::copy(&S->m2l_frag[(size_t)oi * fr], &S->m2l_frag[(size_t)oi * fr] + fr, &S->m2l_canon[(size_t)ci * fr]);
+    }
+    if (m2l_split_supported(T.p)) {
+      std::vector<double> ck((size_t)NCANON * sq);
+      for (int ci = 0; ci < NCANON; ci++) {
+        int d[3];
+        canon_vector(ci, d);
+        const int oi = offset_index(d[0], d[1], d[2]);
+        std::copy(&S->m2l_kmaj[(size_t)oi * sq], &S->m2l_kmaj[(size_t)oi * sq] + sq, &ck[(size_t)ci * sq]);
+      }
+      S->split_e = split_scale_exp(ck.data(), ck.size());
+      split_fragments(ck.data(), NCANON, T.p, S->split_e, S->m2l_split);
     }
   }
   std::lock_guard<std::mutex> lk(g_cache_mu);
@@ -512,6 +528,20 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   }
   if ((rc = offtab_c.ensure(ot.size() * 4))) return rc;
   FMMX_HIP(hipMemcpy(offtab_c.p, ot.data(), ot.size() * 4, hipMemcpyHostToDevice));
+  // the same entries of each class in order of reflection (stable): the FP16
+  // kernel with DMA-staged operators re-signs its accumulator once per group
+  for (int cls = 0; cls < 8; cls++) {
+    int* row = &ot[(size_t)cls * NEBM * 4];
+    std::vector<std::array<int, 4>> v(NEBM);
+    for (int o = 0; o < NEBM; o++) v[o] = {row[4 * o], row[4 * o + 1], row[4 * o + 2], row[4 * o + 3]};
+    std::stable_sort(v.begin(), v.end(), [](const std::array<int, 4>& a, const std::array<int, 4>& b) {
+      return (a[3] >> 16) < (b[3] >> 16);
+    });
+    for (int o = 0; o < NEBM; o++)
+      for (int k = 0; k < 4; k++) row[4 * o + k] = v[o][k];
+  }
+  if ((rc = offtab_s.ensure(ot.size() * 4))) return rc;
+  FMMX_HIP(hipMemcpy(offtab_s.p, ot.data(), ot.size() * 4, hipMemcpyHostToDevice));
   return FMMX_OK;
 }
 
@@ -603,6 +633,15 @@ int Engine::upload_operators() {
     if ((rc = upload<float>(op_m2m, S->m2m_frag, stream)) || (rc = upload<float>(op_l2l, S->l2l_frag, stream)) ||
         (rc = upload<float>(op_m2l, canon ? S->m2l_canon : S->m2l_frag, stream)))
       return rc;
+    split_ok = false;
+    if (canon && !S->m2l_split.empty()) {
+      if ((rc = op_m2l_h.ensure(S->m2l_split.size() * 16))) return rc;
+      FMMX_HIP(hipMemcpyAsync(op_m2l_h.p, S->m2l_split.data(), S->m2l_split.size() * 16, hipMemcpyHostToDevice,
+                              stream));
+      FMMX_HIP(hipStreamSynchronize(stream));
+      sa_e = S->split_e;
+      split_ok = true;
+    }
   }
   ops_lmax = lmax;
   ops_rd = rd;
@@ -760,6 +799,8 @@ int Engine::prepare(int neq) {
   for (int l = 2; l <= lmax; l++) {
     if ((rc = mexp[l].ensure((size_t)std::max(1, sj.nb[l]) * VS * np * es)) ||
         (rc = lexp[l].ensure((size_t)std::max(1, si.nb[l]) * P2 * np * es)))
+      return rc;
+    if (split_ok && !hh && !fp64 && (rc = mexp_h[l].ensure((size_t)std::max(1, sj.nb[l]) * split_box_halfs(p) * 2)))
       return rc;
   }
   if (yk && (rc = upload_yukawa_operators())) return rc;
@@ -1198,7 +1239,7 @@ int fmmx_set_m2l_variant(fmmx_handle* h, int lev, int nt, int staging, int wpg, 
   FMMX_GUARD({
     if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
     if (lev < 2 || lev > fmmx::MAXL) return fmmx::fail(FMMX_EINVAL, "level out of range [2, 8]");
-    if (nt < 0 || nt > 2 || staging < -1 || staging > 2 || (wpg != 0 && wpg != 4 && wpg != 8 && wpg != 16) ||
+    if (nt < 0 || nt > 2 || staging < -1 || staging > 4 || (wpg != 0 && wpg != 4 && wpg != 8 && wpg != 16) ||
         nsplit < 0 || nsplit > 16)
       return fmmx::fail(FMMX_EINVAL, "bad M2L variant");
     auto& t = h->e.tune[lev];

@@ -19,6 +19,7 @@
 #include <cstring>
 
 #include "engine.h"
+#include "m2l.h"
 #include "tree_util.h"
 
 namespace fmmx {
@@ -98,13 +99,6 @@ struct HC<float> {
 };
 
 
-// storage of a multipole slot c in the padded layout read by the M2L B
-// operand: four segments of KSP = roundup4(ceil(P2/4)) floats
-template <int P>
-struct MLayout {
-  static constexpr int P2 = P * P, KS = (P2 + 3) / 4, KSP = (KS + 3) / 4 * 4, VS = 4 * KSP;
-  static __host__ __device__ constexpr int at(int c) { return (c / KS) * KSP + c % KS; }
-};
 
 template <int P, typename T>
 __device__ __forceinline__ void solid_harmonics(T x, T y, T z, T* Sr, T* Si) {
@@ -952,56 +946,6 @@ constexpr int M2L_TN = 64;  // targets per workgroup of the FP64 vector kernel
 // the matrix cores; the next offset's source multipoles always are
 // (software pipelining).  The operator goes through one LDS buffer shared by
 // the 4 waves.
-// Reflection of an M2L offset through the coordinate planes: with reflection
-// bits r = (x, y, z), M2L(reflected offset) = T_r M2L(offset) T_r for the
-// diagonal +-1 matrix T_r on the real slots: x flips Re by (-1)^m and Im by
-// (-1)^(m+1); y flips Im; z flips both by (-1)^(n+m).  Bit c of mask[r] is set
-// where T_r has -1.
-template <int P>
-struct ReflMask {
-  static constexpr int W = (P * P + 31) / 32;
-  uint32_t mask[8][W];
-  constexpr ReflMask() : mask{} {
-    for (int r = 0; r < 8; r++) {
-      int c = 0;
-      for (int im = 0; im < 2; im++)
-        for (int n = 0; n < P; n++)
-          for (int m = im; m <= n; m++, c++) {
-            int neg = 0;
-            if (r & 1) neg ^= (m & 1) ^ im;
-            if (r & 2) neg ^= im;
-            if (r & 4) neg ^= (n + m) & 1;
-            if (neg) mask[r][c >> 5] |= 1u << (c & 31);
-          }
-    }
-  }
-  __host__ __device__ constexpr bool neg(int r, int c) const { return (mask[r][c >> 5] >> (c & 31)) & 1u; }
-};
-
-}  // namespace
-
-// One M2L launch can cover several levels: workgroups [wg0, wg0 + nwg) of
-// the grid run level `lev` with that level's tiles, splits and buffers.
-struct M2LLev {
-  const int* cls_cnt;
-  const uint32_t* ord;
-  const uint32_t* ikey;
-  const int* dense_j;
-  const float* M;
-  float* L;
-  float* part;
-  const float* opF;
-  const int4* offtab;
-  unsigned long long* npairs;
-  int lev, nb, nsplit, init, region, gsh, wg0, nwg;
-};
-struct M2LLaunch {
-  M2LLev lv[MAXL + 1];  // ascending wg0
-  int nlv;
-};
-
-namespace {
-
 template <int P, int NT, int PA, int WPG>
 __global__ __launch_bounds__(64 * WPG) void k_m2l(M2LLaunch A) {
   // this workgroup's level (uniform selects: no dynamic indexing of the
@@ -2211,15 +2155,27 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
     v.wpg = lf.wpg;
   }
   if (v.nt <= 0) v.nt = 1;
-  if (v.pa < 0) v.pa = 0;
+  // default: the FP16 split kernel where it applies (FP32 Laplace engine,
+  // EXACT, p 6..12), else LDS staging
+  if (v.pa < 0) v.pa = (split_ok && !yk && !hh && !fp64) ? 4 : 0;
   if (v.wpg <= 0) v.wpg = 8;
   if (const char* e = getenv("FMMX_M2L_NT")) v.nt = atoi(e) == 2 ? 2 : 1;
-  if (const char* e = getenv("FMMX_M2L_PA")) v.pa = std::max(0, std::min(2, atoi(e)));
+  if (const char* e = getenv("FMMX_M2L_PA")) v.pa = std::max(0, std::min(4, atoi(e)));
   if (const char* e = getenv("FMMX_M2L_WPG")) v.wpg = atoi(e) == 4 ? 4 : atoi(e) == 16 ? 16 : 8;
+  if (v.pa >= 3) {
+    // FP16 split kernel: Laplace FP32 engine, p 6..12; two operator buffers
+    // with one N-tile and 8 waves where they fit LDS
+    if (!split_ok || yk || hh || fp64) v.pa = 0;
+    else if (v.pa == 4 && !split_db_fits(p)) v.pa = 3;
+    if (v.pa == 4) v.nt = 1, v.wpg = 8;
+    if (v.pa == 3 && v.nt == 2) v.wpg = 4;
+    if (v.wpg == 16) v.wpg = 8;
+  }
   if (v.pa == 2 && (p > 11 || v.nt != 1)) v.pa = 0;  // two operator buffers fit LDS up to p = 11
   if (v.wpg == 16 && v.pa != 2) v.wpg = 8;
   // reflection-canonical operators are signed while staged: no register prefetch form
   if (v.pa == 1 && canon && !yk) v.pa = 0;
+  if (v.pa < 3 && v.wpg == 16 && v.pa != 2) v.wpg = 8;
   if (const char* e = getenv("FMMX_M2L_SPLIT")) v.nsplit = atoi(e);
   if (const char* e = getenv("FMMX_M2L_SPLITS")) {
     // per level, comma-separated from level 2
@@ -2270,6 +2226,12 @@ M2LLev Engine::m2l_desc(int lev, int nt, int wpg, int nsplit, float* part, int i
   d.L = lexp[lev].as<float>();
   d.part = part;
   d.opF = (yk ? yop_m2l : op_m2l).as<float>() + opoff;
+  // FP16 split form: per-level operator sets in REF, one set otherwise
+  const size_t frh = split_ok ? split_frag_count(p) : 0;
+  d.Mh = mexp_h[lev].as<uint4>();
+  d.opH = op_m2l_h.as<uint4>() + (m2l_per_level ? (size_t)(lev - 2) * NOFF * frh : 0);
+  d.mmax = mmax_buf.as<unsigned>() + lev;
+  d.sa_e = sa_e;
   d.offtab = (use_canon ? offtab_c : offtab).as<int4>();
   d.npairs = pair_ctr.as<unsigned long long>() + 2 + lev;
   d.lev = lev;
@@ -2310,14 +2272,44 @@ void launch_m2l(const M2LLaunch& A, int nwg, int nt, int pa, int wpg, hipStream_
 // launch shape with each level's offset splits: levels with splits write
 // partial sums (reduced afterwards in fixed order), the others write their
 // locals; the L2L chain then adds the parents' locals level by level.
+bool Engine::m2l_split(int lev) const { return !fp64 && !hh && !yk && m2l_variant(lev).pa >= 3; }
+
+// The FP16 split kernel's inputs for levels [l0, l1]: the multipoles of every
+// level split into hi/lo pairs scaled by 2^(14 - exponent of the level's
+// largest magnitude), one pair of launches for all levels
+int Engine::run_msplit(int l0, int l1) {
+  MSplitLevels S{};
+  S.off[0] = 0;
+  int rc = mmax_buf.ensure((MAXL + 1) * 4);
+  if (rc) return rc;
+  for (int lev = l0; lev <= l1; lev++) {
+    const int nb = sj.nb[lev];
+    if (nb == 0 || si.nb[lev] == 0) continue;
+    S.M[S.nl] = mexp[lev].as<float>();
+    S.Mh[S.nl] = mexp_h[lev].as<uint4>();
+    S.mmax[S.nl] = mmax_buf.as<unsigned>() + lev;
+    S.off[S.nl + 1] = S.off[S.nl] + nb;
+    S.nl++;
+  }
+  if (S.nl == 0) return FMMX_OK;
+  return launch_msplit(p, S, stream);
+}
+
+// FP32 M2L of all levels l0..lmax in one launch, in the leaf level's launch
+// shape with each level's offset splits: levels with splits write partial
+// sums (reduced afterwards in fixed order), the others write their locals;
+// the L2L chain then adds the parents' locals level by level.  With the FP16
+// split shape (staging 3, 4) the multipoles are split first.
 int Engine::run_m2l_all(int l0) {
   const M2LVariant leaf = m2l_variant(lmax);
+  const bool split = leaf.pa >= 3;
   M2LLaunch A{};
   size_t need = 0;
   for (int lev = l0; lev <= lmax; lev++)
     if (si.nb[lev] > 0 && sj.nb[lev] > 0 && m2l_splits(lev) > 1) need += (size_t)m2l_splits(lev) * si.nb[lev] * P2;
   int rc = m2l_part.ensure(std::max<size_t>(need, 1) * 4);
   if (rc) return rc;
+  if (split && (rc = run_msplit(l0, lmax))) return rc;
   size_t off = 0;
   int wg = 0;
   for (int lev = l0; lev <= lmax; lev++) {
@@ -2330,6 +2322,7 @@ int Engine::run_m2l_all(int l0) {
     }
     const int ns = m2l_splits(lev);
     A.lv[A.nlv] = m2l_desc(lev, leaf.nt, leaf.wpg, ns, ns > 1 ? m2l_part.as<float>() + off : nullptr, 1, wg);
+    if (leaf.pa == 4) A.lv[A.nlv].offtab = offtab_s.as<int4>();
     wg += A.lv[A.nlv].nwg;
     A.nlv++;
     if (ns > 1) off += (size_t)ns * nl;
@@ -2338,7 +2331,12 @@ int Engine::run_m2l_all(int l0) {
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if (per_phase) FMMX_HIP(hipEventRecord(ev[8], stream));
-    launch_m2l<P>(A, wg, leaf.nt, leaf.pa, leaf.wpg, stream);
+    if (split) {
+      int r2 = launch_m2l_split(p, A, wg, leaf.nt, leaf.wpg, leaf.pa == 4, stream);
+      if (r2) return r2;
+    } else {
+      launch_m2l<P>(A, wg, leaf.nt, leaf.pa, leaf.wpg, stream);
+    }
     if (per_phase) {
       FMMX_HIP(hipEventRecord(ev[9], stream));
       m2l_timed = true;
@@ -2458,9 +2456,19 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       M2LLaunch A{};
       A.nlv = 1;
       A.lv[0] = m2l_desc(lev, var.nt, var.wpg, nsplit, part_buf->as<float>(), init ? 1 : 0, 0);
+      if (var.pa == 4) A.lv[0].offtab = offtab_s.as<int4>();
+      if (var.pa >= 3) {
+        int r2 = run_msplit(lev, lev);
+        if (r2) return r2;
+      }
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
-      launch_m2l<P>(A, A.lv[0].nwg, var.nt, var.pa, var.wpg, stream);
+      if (var.pa >= 3) {
+        int r2 = launch_m2l_split(p, A, A.lv[0].nwg, var.nt, var.wpg, var.pa == 4, stream);
+        if (r2) return r2;
+      } else {
+        launch_m2l<P>(A, A.lv[0].nwg, var.nt, var.pa, var.wpg, stream);
+      }
       if (tmark) {
         FMMX_HIP(hipEventRecord(ev[9], stream));
         m2l_timed = true;
