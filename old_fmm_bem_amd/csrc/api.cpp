@@ -800,7 +800,7 @@ int Engine::prepare(int neq) {
     if ((rc = mexp[l].ensure((size_t)std::max(1, sj.nb[l]) * VS * np * es)) ||
         (rc = lexp[l].ensure((size_t)std::max(1, si.nb[l]) * P2 * np * es)))
       return rc;
-    if (split_ok && !hh && !fp64 && (rc = mexp_h[l].ensure((size_t)std::max(1, sj.nb[l]) * split_box_halfs(p) * 2)))
+    if (split_ok && !hh && !fp64 && (rc = mexp_h[l].ensure((size_t)(sj.nb[l] + 1) * split_box_halfs(p) * 2)))
       return rc;
   }
   if (yk && (rc = upload_yukawa_operators())) return rc;

@@ -94,7 +94,8 @@ struct Engine {
   Tables tab;
 
   Side si, sj;
-  DevBuf scratch;  // rocPRIM temporary storage
+  DevBuf scratch;    // rocPRIM temporary storage (level scans)
+  DevBuf rs_counts;  // digit counts per tile of the radix sort
   DevBuf lvl_f4, lvl_p4;  // box-start flags and numbers of four levels (tree build)
   DevBuf xyz4;            // packed coordinates in caller order (tree build)
   // near list at the leaf level (targets): counts and rows of NNEAR

@@ -2236,6 +2236,7 @@ M2LLev Engine::m2l_desc(int lev, int nt, int wpg, int nsplit, float* part, int i
   d.npairs = pair_ctr.as<unsigned long long>() + 2 + lev;
   d.lev = lev;
   d.nb = si.nb[lev];
+  d.nbj = sj.nb[lev];
   d.nsplit = nsplit;
   d.init = init;
   d.region = region;
@@ -2288,7 +2289,7 @@ int Engine::run_msplit(int l0, int l1) {
     S.M[S.nl] = mexp[lev].as<float>();
     S.Mh[S.nl] = mexp_h[lev].as<uint4>();
     S.mmax[S.nl] = mmax_buf.as<unsigned>() + lev;
-    S.off[S.nl + 1] = S.off[S.nl] + nb;
+    S.off[S.nl + 1] = S.off[S.nl] + nb + 1;  // and the zero box
     S.nl++;
   }
   if (S.nl == 0) return FMMX_OK;

@@ -95,7 +95,7 @@ struct M2LLev {
   const uint4* Mh;
   const uint4* opH;
   const unsigned* mmax;
-  int lev, nb, nsplit, init, region, gsh, wg0, nwg, sa_e;
+  int lev, nb, nbj, nsplit, init, region, gsh, wg0, nwg, sa_e;
 };
 struct M2LLaunch {
   M2LLev lv[MAXL + 1];  // ascending wg0
@@ -103,7 +103,8 @@ struct M2LLaunch {
 };
 
 // levels of the multipole split (largest magnitude, then the FP16 pairs):
-// boxes [off[k], off[k+1]) of the flattened range belong to level k
+// boxes [off[k], off[k+1]) of the flattened range belong to level k, the
+// last of them a zero box (the split multipoles hold nb + 1 boxes)
 struct MSplitLevels {
   const float* M[MAXL + 1];
   uint4* Mh[MAXL + 1];

@@ -80,9 +80,9 @@ __global__ __launch_bounds__(MS_T) void k_msplit_max(MSplitLevels S) {
   if (threadIdx.x <= MAXL) smax[threadIdx.x] = 0u;
   __syncthreads();
   const long b = t >> 2;
-  if (b < S.off[S.nl]) {
-    int k = 0;
-    while (k + 1 < S.nl && b >= S.off[k + 1]) k++;
+  int k = 0;
+  while (k + 1 < S.nl && b >= S.off[k + 1]) k++;
+  if (b < S.off[S.nl] && b != S.off[k + 1] - 1) {  // not the zero box
     const int g = (int)(t & 3);
     const float* m = S.M[k] + (size_t)(b - S.off[k]) * ML::VS + g * ML::KSP;
     float mx = 0.f;
@@ -109,10 +109,13 @@ __global__ __launch_bounds__(MS_T) void k_msplit(MSplitLevels S) {
   while (k + 1 < S.nl && b >= S.off[k + 1]) k++;
   const int g = (int)(t & 3);
   const float sc = ldexpf(1.f, split_exp(S.mmax[k]));
-  const float* m = S.M[k] + (size_t)(b - S.off[k]) * ML::VS + g * ML::KSP;
+  // the last box of a level's range is the zero box (read where a target
+  // has no source box)
+  const bool zero = b == S.off[k + 1] - 1;
+  const float* m = S.M[k] + (size_t)(zero ? 0 : b - S.off[k]) * ML::VS + g * ML::KSP;
   float x[H::SEG];
 #pragma unroll
-  for (int u = 0; u < H::SEG; u++) x[u] = (u < ML::KS && g * ML::KS + u < ML::P2) ? m[u] * sc : 0.f;
+  for (int u = 0; u < H::SEG; u++) x[u] = (!zero && u < ML::KS && g * ML::KS + u < ML::P2) ? m[u] * sc : 0.f;
   uint32_t w[H::GH / 2];
 #pragma unroll
   for (int j = 0; j < H::GH / 2; j++) w[j] = 0u;
@@ -226,9 +229,9 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_h(M2LLaunch A) {
       }                                                                                       \
       if (s_ >= 0 && kq == 0) mypairs++;                                                      \
       any_[n] = __ballot(s_ >= 0) != 0;                                                       \
-      const uint4* ms_ = Mh + ((size_t)(s_ >= 0 ? s_ : 0) * H::VSH + kq * H::GH) / 8;         \
+      const uint4* ms_ = Mh + ((size_t)(s_ >= 0 ? s_ : Q.nbj) * H::VSH + kq * H::GH) / 8;     \
       _Pragma("unroll") for (int v_ = 0; v_ < NB4; v_++) {                                    \
-        const uint4 m4_ = s_ >= 0 ? ms_[v_] : make_uint4(0u, 0u, 0u, 0u);                     \
+        const uint4 m4_ = ms_[v_];                                                            \
         dst_[n][4 * v_] = m4_.x;                                                              \
         dst_[n][4 * v_ + 1] = m4_.y;                                                          \
         dst_[n][4 * v_ + 2] = m4_.z;                                                          \
@@ -396,7 +399,7 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
 #pragma unroll
   for (int k = 1; k < MAXL + 1; k++)
     if (k < A.nlv && (int)blockIdx.x >= A.lv[k].wg0) Q = A.lv[k];
-  const int lev = Q.lev, nb = Q.nb, nsplit = Q.nsplit, init = Q.init, region = Q.region, gsh = Q.gsh;
+  const int lev = Q.lev, nb = Q.nb, nbj = Q.nbj, nsplit = Q.nsplit, init = Q.init, region = Q.region, gsh = Q.gsh;
   const int* __restrict__ cls_cnt = Q.cls_cnt;
   const uint32_t* __restrict__ ord = Q.ord;
   const uint32_t* __restrict__ ikey = Q.ikey;
@@ -437,7 +440,9 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
   __syncthreads();
   const int cls = s_class, ntile = s_n;
   if (cls < 0) return;
-  const int w = tid >> 6, l = tid & 63, j = l & 15, kq = l >> 4;
+  // the wave index, known to the compiler as uniform (scalar addressing of
+  // the operator copies)
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, j = l & 15, kq = l >> 4;
   const int g = 1 << lev;
   int tord[NT], tx[NT], ty[NT], tz[NT];
   uint32_t tk[NT];  // target keys (Morton: x bits 3b+1, y 3b, z 3b+2)
@@ -484,9 +489,10 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
       }                                                                                       \
       if (s_ >= 0 && kq == 0) mypairs++;                                                      \
       any_[n] = __ballot(s_ >= 0) != 0;                                                       \
-      const uint4* ms_ = Mh + ((size_t)(s_ >= 0 ? s_ : 0) * H::VSH + kq * H::GH) / 8;         \
+      /* no source box: the level's zero box after the last one */                            \
+      const uint4* ms_ = Mh + ((size_t)(s_ >= 0 ? s_ : nbj) * H::VSH + kq * H::GH) / 8;       \
       _Pragma("unroll") for (int v_ = 0; v_ < NB4; v_++) {                                    \
-        const uint4 m4_ = s_ >= 0 ? ms_[v_] : make_uint4(0u, 0u, 0u, 0u);                     \
+        const uint4 m4_ = ms_[v_];                                                            \
         dst_[n][4 * v_] = m4_.x;                                                              \
         dst_[n][4 * v_ + 1] = m4_.y;                                                          \
         dst_[n][4 * v_ + 2] = m4_.z;                                                          \
@@ -498,9 +504,9 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
   // 1 KB chunks w, w + WPG, ... (lane l: 16 bytes at chunk offset 16 l)
 #define FMMX_DMA_A(o_, buf_)                                                                  \
   {                                                                                           \
-    const uint4* src_ = opH + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FRH;           \
+    const uint4* src_ = opH + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FRH + l;       \
     for (int c_ = w; c_ < NCH; c_ += WPG)                                                     \
-      __builtin_amdgcn_global_load_lds((const void*)(src_ + c_ * 64 + l),                     \
+      __builtin_amdgcn_global_load_lds((const void*)(src_ + c_ * 64),                         \
                                        (__attribute__((address_space(3))) void*)(As + (buf_) * FRH + c_ * 64), \
                                        16, 0, 0);                                             \
   }

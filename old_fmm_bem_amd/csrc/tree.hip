@@ -3,8 +3,10 @@
 // radix sort), SoA gathers (sort/sortvar.cxx), per-level box lists
 // (fmm/boxdatai.cxx, boxdataj.cxx, boxm2mi.cxx, boxm2mj.cxx), and the
 // interaction lists of fmm/ijbox.cxx in the reference's loop order.
-#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+
+#include <cmath>
+#include <mutex>
 
 #include "engine.h"
 #include "tree_util.h"
@@ -12,11 +14,6 @@
 namespace fmmx {
 
 namespace {
-
-// radix sorts always by onesweep passes (rocPRIM's default switches to a merge
-// sort up to 2^20 items, ten passes where the 3*lmax-bit keys need two)
-using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                           rocprim::default_config, 0>;
 
 constexpr int TPB = 256;
 inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
@@ -48,7 +45,135 @@ __global__ void k_keys(int n, const float* __restrict__ x, const float* __restri
   f1 = min(max(f1, 0), kl - 1);
   f2 = min(max(f2, 0), kl - 1);
   key[i] = morton_encode(f0, f1, f2, lmax);
-  idx[i] = (uint32_t)i;
+  if (idx) idx[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------
+// Stable LSD radix sort of (key, index) pairs, 8-bit digits: the GPU form of
+// the reference's counting sort (sort/csort.cxx:5-17: counts per key, prefix
+// sums, placement in input order).  Per pass: digit counts per tile of
+// RS_TILE keys (digit-major, tile-minor), one exclusive scan of them, then a
+// stable scatter: within a tile each wave ranks its keys chunk by chunk (64
+// keys, same-digit lanes found by 8 ballots), the waves' counts are
+// prefix-summed in wave order, and tile, wave and lane order give the
+// positions, so equal keys keep their input order.
+constexpr int RS_T = 256, RS_CH = 16, RS_TILE = RS_T * RS_CH;  // keys per tile
+constexpr int RS_SCAN_T = 1024;
+
+inline int rs_tiles(long n) { return (int)((n + RS_TILE - 1) / RS_TILE); }
+inline int rs_passes(int bits) { return bits <= 0 ? 1 : (bits + 7) / 8; }
+
+// digit counts of one tile: counts[d * ntile + tile]
+__global__ __launch_bounds__(RS_T) void k_rs_hist(int n, const uint32_t* __restrict__ keys, int shift, int ntile,
+                                                  uint32_t* __restrict__ counts) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const long base = (long)blockIdx.x * RS_TILE;
+  for (int e = threadIdx.x; e < RS_TILE; e += RS_T) {
+    const long i = base + e;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  counts[(size_t)threadIdx.x * ntile + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of total counters in place (one workgroup, rounds of
+// 4 RS_SCAN_T counters)
+__global__ __launch_bounds__(RS_SCAN_T) void k_rs_scan(long total, uint32_t* __restrict__ c) {
+  __shared__ uint32_t wsum[RS_SCAN_T / 64];
+  __shared__ uint32_t carry_s;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) carry_s = 0u;
+  __syncthreads();
+  for (long r0 = 0; r0 < total; r0 += 4L * RS_SCAN_T) {
+    uint32_t v[4], s = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const long i = r0 + 4L * t + k;
+      v[k] = i < total ? c[i] : 0u;
+      s += v[k];
+    }
+    uint32_t inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += o;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    uint32_t before = carry_s, all = 0u;
+    for (int w = 0; w < RS_SCAN_T / 64; w++) {
+      if (w < wv) before += wsum[w];
+      all += wsum[w];
+    }
+    uint32_t run = before + inc - s;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const long i = r0 + 4L * t + k;
+      if (i < total) c[i] = run;
+      run += v[k];
+    }
+    __syncthreads();  // every thread has read carry_s and wsum
+    if (t == 0) carry_s += all;
+    __syncthreads();
+  }
+}
+
+// stable scatter of one pass (vin NULL: the values are the input positions)
+__global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const uint32_t* __restrict__ kin,
+                                                     const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+                                                     uint32_t* __restrict__ vout, int shift, int ntile,
+                                                     const uint32_t* __restrict__ offs) {
+  __shared__ uint32_t wc[RS_T / 64][256];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  for (int e = t; e < (RS_T / 64) * 256; e += RS_T) (&wc[0][0])[e] = 0u;
+  __syncthreads();
+  const long base = (long)blockIdx.x * RS_TILE + (long)w * (RS_CH * 64);
+  const uint64_t below = (1ull << l) - 1ull;
+  uint32_t k[RS_CH], v[RS_CH], rk[RS_CH];
+#pragma unroll
+  for (int c = 0; c < RS_CH; c++) {
+    const long i = base + c * 64 + l;
+    const bool ok = i < n;
+    k[c] = ok ? kin[i] : 0u;
+    v[c] = ok ? (vin ? vin[i] : (uint32_t)i) : 0u;
+    const uint32_t d = (k[c] >> shift) & 255u;
+    // lanes of this chunk holding the same digit
+    uint64_t m = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const uint64_t bb = __ballot((d >> b) & 1u);
+      m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t prev = wc[w][d];
+    rk[c] = prev + (uint32_t)__popcll(m & below);
+    // the highest such lane advances the wave's count of the digit
+    if (ok && (m >> l) == 1ull) wc[w][d] = prev + (uint32_t)__popcll(m);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // wave offsets within the tile plus the tile's base, per digit
+  {
+    const int d = t;  // RS_T == 256 digits
+    uint32_t run = offs[(size_t)d * ntile + blockIdx.x];
+#pragma unroll
+    for (int ww = 0; ww < RS_T / 64; ww++) {
+      const uint32_t c = wc[ww][d];
+      wc[ww][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < RS_CH; c++) {
+    const long i = base + c * 64 + l;
+    if (i < n) {
+      const uint32_t pos = wc[w][(k[c] >> shift) & 255u] + rk[c];
+      kout[pos] = k[c];
+      vout[pos] = v[c];
+    }
+  }
 }
 
 // packed coordinates into key order, as three columns
@@ -349,6 +474,27 @@ __global__ __launch_bounds__(CL_T) void k_class_place(ClassLevels L) {
 
 }  // namespace
 
+// Stable radix sort of the keys in kbuf[0] by their low `bits` bits, the
+// input positions as values, on stream s.  kbuf[0], kbuf[1] (keys) and
+// vbuf[0], vbuf[1] (values) ping-pong, n words each; counts holds
+// 256 * rs_tiles(n) words.  The sorted keys and values end in
+// kbuf[rs_out_slot(bits)] and vbuf[rs_out_slot(bits)].
+int rs_out_slot(int bits) { return rs_passes(bits) & 1; }
+
+void radix_sort(int n, int bits, uint32_t* const kbuf[2], uint32_t* const vbuf[2], uint32_t* counts,
+                hipStream_t s) {
+  if (n <= 0) return;
+  const int nt = rs_tiles(n), passes = rs_passes(bits);
+  const long total = 256L * nt;
+  for (int ps = 0; ps < passes; ps++) {
+    const int src = ps & 1, dst = src ^ 1;
+    k_rs_hist<<<nt, RS_T, 0, s>>>(n, kbuf[src], 8 * ps, nt, counts);
+    k_rs_scan<<<1, RS_SCAN_T, 0, s>>>(total, counts);
+    k_rs_scatter<<<nt, RS_T, 0, s>>>(n, kbuf[src], ps == 0 ? nullptr : vbuf[src], kbuf[dst], vbuf[dst], 8 * ps, nt,
+                                     counts);
+  }
+}
+
 // Keys of one point set at the leaf level and their stable sort, as
 // sort/sorti.cxx:12-26 (boxn, then the counting sort of sort/csort.cxx) leaves
 // them in nfn/na (sorted keys) and nbi (sorted position -> caller index).
@@ -375,20 +521,19 @@ int Engine::sort_points(int n, const float* x, const float* y, const float* z, i
   }
   const int lm = cfg.lmax;
   double rbinv = (double)((float)(1 << lm) / cfg.rd);
-  if ((rc = srt_bad.ensure(4))) return rc;
+  if ((rc = srt_bad.ensure(4)) || (rc = rs_counts.ensure((size_t)256 * rs_tiles(n) * 4))) return rc;
+  const int slot = rs_out_slot(3 * lm);
+  uint32_t* kb[2];
+  uint32_t* vb[2];
+  kb[slot] = srt_keys.as<uint32_t>();
+  kb[slot ^ 1] = srt_keys_in.as<uint32_t>();
+  vb[slot] = srt_perm.as<uint32_t>();
+  vb[slot ^ 1] = srt_idx_in.as<uint32_t>();
   k_fill_int<<<1, 32, 0, stream>>>(1, 0, srt_bad.as<int>());
-  k_keys<<<nblk(n), TPB, 0, stream>>>(n, d[0], d[1], d[2], cfg.xmin, cfg.ymin, cfg.zmin, cfg.rd, rbinv, lm,
-                                      srt_keys_in.as<uint32_t>(), srt_idx_in.as<uint32_t>(), nullptr,
-                                      srt_bad.as<int>());
-  size_t tb = 0;
-  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
-                                     srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
-                                     stream));
-  if ((rc = scratch.ensure(tb + 256))) return rc;
-  tb = scratch.bytes;
-  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(scratch.p, tb, srt_keys_in.as<uint32_t>(), srt_keys.as<uint32_t>(),
-                                     srt_idx_in.as<uint32_t>(), srt_perm.as<uint32_t>(), (size_t)n, 0, 3 * lm,
-                                     stream));
+  k_keys<<<nblk(n), TPB, 0, stream>>>(n, d[0], d[1], d[2], cfg.xmin, cfg.ymin, cfg.zmin, cfg.rd, rbinv, lm, kb[0],
+                                      nullptr, nullptr, srt_bad.as<int>());
+  radix_sort(n, 3 * lm, kb, vb, rs_counts.as<uint32_t>(), stream);
+  FMMX_HIP(hipGetLastError());
   const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   FMMX_HIP(hipMemcpyAsync(perm_out, srt_perm.p, nb, kind, stream));
   if (keys_out) FMMX_HIP(hipMemcpyAsync(keys_out, srt_keys.p, nb, kind, stream));
@@ -452,20 +597,21 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   if (n == 0) return FMMX_OK;
   if ((rc = xyz4.ensure(nn * 16))) return rc;
   double rbinv = (double)((float)(1 << lmax) / rd);
-  k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rd, rbinv, lmax, s.keys_in.as<uint32_t>(),
-                                      s.idx_in.as<uint32_t>(), xyz4.as<float4>(), bad_flag.as<int>());
-  size_t tb = 0;
-  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(nullptr, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
-                                     s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
-                                     stream));
+  if ((rc = rs_counts.ensure((size_t)256 * rs_tiles(n) * 4))) return rc;
+  const int slot = rs_out_slot(3 * lmax);
+  uint32_t* kb[2];
+  uint32_t* vb[2];
+  kb[slot] = s.keys.as<uint32_t>();
+  kb[slot ^ 1] = s.keys_in.as<uint32_t>();
+  vb[slot] = s.perm.as<uint32_t>();
+  vb[slot ^ 1] = s.idx_in.as<uint32_t>();
+  k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rd, rbinv, lmax, kb[0], nullptr,
+                                      xyz4.as<float4>(), bad_flag.as<int>());
+  radix_sort(n, 3 * lmax, kb, vb, rs_counts.as<uint32_t>(), stream);
   size_t sb = 0;
   FMMX_HIP(rocprim::exclusive_scan(nullptr, sb, (int*)nullptr, (int*)nullptr, 0, (size_t)n, rocprim::plus<int>(),
                                    stream));
-  if ((rc = scratch.ensure(std::max(tb, sb) + 256))) return rc;
-  tb = scratch.bytes;
-  FMMX_HIP(rocprim::radix_sort_pairs<SortCfg>(scratch.p, tb, s.keys_in.as<uint32_t>(), s.keys.as<uint32_t>(),
-                                     s.idx_in.as<uint32_t>(), s.perm.as<uint32_t>(), (size_t)n, 0, 3 * lmax,
-                                     stream));
+  if ((rc = scratch.ensure(sb + 256))) return rc;
   const uint32_t* perm = s.perm.as<uint32_t>();
   k_gather_xyz<<<nblk(n), TPB, 0, stream>>>(n, perm, xyz4.as<float4>(), s.x.as<float>(), s.y.as<float>(),
                                             s.z.as<float>());
