@@ -2167,7 +2167,7 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
     // with one N-tile and 8 waves where they fit LDS
     if (!split_ok || yk || hh || fp64) v.pa = 0;
     else if (v.pa == 4 && !split_db_fits(p)) v.pa = 3;
-    if (v.pa == 4) v.nt = 1, v.wpg = 8;
+    if (v.pa == 4) v.wpg = v.nt == 2 ? 4 : 8;  // 8 waves of one N-tile or 4 of two
     if (v.pa == 3 && v.nt == 2) v.wpg = 4;
     if (v.wpg == 16) v.wpg = 8;
   }

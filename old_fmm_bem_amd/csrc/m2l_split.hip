@@ -393,7 +393,7 @@ __constant__ HSign2<P> c_hsign2 = HSign2<P>();
 // by T_r (one mask per reflection group), and the accumulator holds
 // T_r (partial sum) for the current group r, re-signed by T_r T_r' where the
 // group changes and by T_r after the last offset.
-template <int P, int WPG>
+template <int P, int NT, int WPG>
 __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
   M2LLev Q = A.lv[0];
 #pragma unroll
@@ -411,7 +411,6 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
   const int4* __restrict__ offtab = Q.offtab;
   const int bid = (int)blockIdx.x - Q.wg0, nwg = Q.nwg;
   using H = HLayout<P>;
-  constexpr int NT = 1;
   constexpr int P2 = H::P2, MT = H::MT, KH = H::KH, FRH = H::FRH, NB4 = H::NB4, NCH = KH * MT;
   constexpr int TN = 16 * NT * WPG;
   __shared__ uint4 As[2 * FRH];
@@ -462,9 +461,11 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
   const bool wave_on = w * NT * 16 < ntile;
   const int per = (NEBM + nsplit - 1) / nsplit, o0 = split * per, o1 = min(NEBM, o0 + per);
   const float oscale = ldexpf(1.f, -(split_exp(Q.mmax) + Q.sa_e));
-  f32x4 acc[MT];
+  f32x4 acc[NT][MT];
 #pragma unroll
-  for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int n = 0; n < NT; n++)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) acc[n][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
   unsigned long long mypairs = 0;
   const HSign2<P>& HS = c_hsign2<P>;
   uint32_t bcur[NT][4 * NB4];
@@ -539,7 +540,8 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
           const int i = 16 * mt + 4 * kq + q;                                                 \
           if (i < P2) {                                                                       \
             const uint32_t f = ((HS.rneg[rcur][i >> 5] ^ HS.rneg[r][i >> 5]) >> (i & 31)) & 1u; \
-            acc[mt][q] = __uint_as_float(__float_as_uint(acc[mt][q]) ^ (f << 31));            \
+            _Pragma("unroll") for (int n = 0; n < NT; n++)                                    \
+              acc[n][mt][q] = __uint_as_float(__float_as_uint(acc[n][mt][q]) ^ (f << 31));    \
           }                                                                                   \
         }                                                                                     \
       }                                                                                       \
@@ -550,8 +552,11 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
         bm[q] = u0 < 0 ? 0u : (uint32_t)((bn >> u0) & 1u) << 15 | (uint32_t)((bn >> (u0 + 1)) & 1u) << 31; \
       }                                                                                       \
     }                                                                                         \
-    if (wave_on && AC_[0]) {                                                                  \
-      _Pragma("unroll") for (int q = 0; q < 4 * NB4; q++) BC_[0][q] ^= bm[q];                 \
+    bool run_ = false;                                                                        \
+    _Pragma("unroll") for (int n = 0; n < NT; n++) run_ = run_ || AC_[n];                     \
+    if (wave_on && run_) {                                                                    \
+      _Pragma("unroll") for (int n = 0; n < NT; n++)                                          \
+      _Pragma("unroll") for (int q = 0; q < 4 * NB4; q++) BC_[n][q] ^= bm[q];                 \
       const uint4* Ab = As + buf * FRH;                                                       \
       h8 af[2][MT];                                                                           \
       _Pragma("unroll") for (int mt = 0; mt < MT; mt++) af[0][mt] = __builtin_bit_cast(h8, Ab[mt * 64 + l]); \
@@ -562,7 +567,8 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
         }                                                                                     \
         __builtin_amdgcn_sched_barrier(0);                                                    \
         _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                     \
-          acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[ks & 1][mt], bfrag<H>(BC_[0], ks), acc[mt], 0, 0, 0); \
+        _Pragma("unroll") for (int n = 0; n < NT; n++)                                        \
+          acc[n][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[ks & 1][mt], bfrag<H>(BC_[n], ks), acc[n][mt], 0, 0, 0); \
         __builtin_amdgcn_sched_barrier(0);                                                    \
       }                                                                                       \
     }                                                                                         \
@@ -576,20 +582,22 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
 #undef FMMX_STEP
 #undef FMMX_LOAD_BH
 #undef FMMX_DMA_A
-  if (tord[0] >= 0) {
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    if (tord[n] < 0) continue;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int i = 16 * mt + 4 * kq + q;
         if (i < P2) {
-          float v = acc[mt][q] * oscale;
+          float v = acc[n][mt][q] * oscale;
           if (rcur >= 0 && ((HS.rneg[rcur][i >> 5] >> (i & 31)) & 1u)) v = -v;
           if (nsplit == 1) {
-            float* dst = L + (size_t)tord[0] * P2 + i;
+            float* dst = L + (size_t)tord[n] * P2 + i;
             *dst = init ? v : *dst + v;
           } else {
-            part[((size_t)split * nb + tord[0]) * P2 + i] = v;
+            part[((size_t)split * nb + tord[n]) * P2 + i] = v;
           }
         }
       }
@@ -691,7 +699,10 @@ int launch_m2l_split(int p, const M2LLaunch& A, int nwg, int nt, int wpg, int db
     constexpr int P = decltype(pc)::value;
     if (db && split_db_fits(P)) {
       // operators by DMA into two LDS buffers, reflections on the operands
-      if constexpr ((size_t)2 * HLayout<P>::FRH * 16 <= 160 * 1024 - 64) k_m2l_hd<P, 8><<<nwg, 512, 0, stream>>>(A);
+      if constexpr ((size_t)2 * HLayout<P>::FRH * 16 <= 160 * 1024 - 64) {
+        if (nt == 2) k_m2l_hd<P, 2, 4><<<nwg, 256, 0, stream>>>(A);
+        else k_m2l_hd<P, 1, 8><<<nwg, 512, 0, stream>>>(A);
+      }
     } else if (nt == 2) {
       k_m2l_h<P, 2, 4, 0><<<nwg, 256, 0, stream>>>(A);
     } else if (wpg == 4) {

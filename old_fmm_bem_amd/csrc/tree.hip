@@ -52,16 +52,19 @@ __global__ void k_keys(int n, const float* __restrict__ x, const float* __restri
 // Stable LSD radix sort of (key, index) pairs, 8-bit digits: the GPU form of
 // the reference's counting sort (sort/csort.cxx:5-17: counts per key, prefix
 // sums, placement in input order).  Per pass: digit counts per tile of
-// RS_TILE keys (digit-major, tile-minor), one exclusive scan of them, then a
+// RS_TILE keys (digit-major, tile-minor), the prefix over tiles of every
+// digit (one workgroup per digit, also writing the digit's total), then a
 // stable scatter: within a tile each wave ranks its keys chunk by chunk (64
 // keys, same-digit lanes found by 8 ballots), the waves' counts are
-// prefix-summed in wave order, and tile, wave and lane order give the
-// positions, so equal keys keep their input order.
-constexpr int RS_T = 256, RS_CH = 16, RS_TILE = RS_T * RS_CH;  // keys per tile
+// prefix-summed in wave order, the digit totals give each digit's start,
+// and tile, wave and lane order give the positions, so equal keys keep
+// their input order.
+constexpr int RS_T = 256, RS_CH = 4, RS_TILE = RS_T * RS_CH;  // keys per tile
 constexpr int RS_SCAN_T = 1024;
 
 inline int rs_tiles(long n) { return (int)((n + RS_TILE - 1) / RS_TILE); }
 inline int rs_passes(int bits) { return bits <= 0 ? 1 : (bits + 7) / 8; }
+inline long rs_count_words(long n) { return 256L * (rs_tiles(n) + 1); }
 
 // digit counts of one tile: counts[d * ntile + tile]
 __global__ __launch_bounds__(RS_T) void k_rs_hist(int n, const uint32_t* __restrict__ keys, int shift, int ntile,
@@ -70,65 +73,67 @@ __global__ __launch_bounds__(RS_T) void k_rs_hist(int n, const uint32_t* __restr
   h[threadIdx.x] = 0u;
   __syncthreads();
   const long base = (long)blockIdx.x * RS_TILE;
-  for (int e = threadIdx.x; e < RS_TILE; e += RS_T) {
-    const long i = base + e;
+#pragma unroll
+  for (int c = 0; c < RS_CH; c++) {
+    const long i = base + c * RS_T + threadIdx.x;
     if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
   }
   __syncthreads();
   counts[(size_t)threadIdx.x * ntile + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of total counters in place (one workgroup, rounds of
-// 4 RS_SCAN_T counters)
-__global__ __launch_bounds__(RS_SCAN_T) void k_rs_scan(long total, uint32_t* __restrict__ c) {
-  __shared__ uint32_t wsum[RS_SCAN_T / 64];
-  __shared__ uint32_t carry_s;
+// block-wide exclusive scan of one value per thread (RS_SCAN_T threads);
+// *all receives the block total
+__device__ __forceinline__ uint32_t rs_block_scan(uint32_t v, uint32_t* wsum, uint32_t* all) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (t == 0) carry_s = 0u;
-  __syncthreads();
-  for (long r0 = 0; r0 < total; r0 += 4L * RS_SCAN_T) {
-    uint32_t v[4], s = 0u;
+  uint32_t inc = v;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const long i = r0 + 4L * t + k;
-      v[k] = i < total ? c[i] : 0u;
-      s += v[k];
-    }
-    uint32_t inc = s;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t o = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += o;
-    }
-    if (lane == 63) wsum[wv] = inc;
-    __syncthreads();
-    uint32_t before = carry_s, all = 0u;
-    for (int w = 0; w < RS_SCAN_T / 64; w++) {
-      if (w < wv) before += wsum[w];
-      all += wsum[w];
-    }
-    uint32_t run = before + inc - s;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const long i = r0 + 4L * t + k;
-      if (i < total) c[i] = run;
-      run += v[k];
-    }
-    __syncthreads();  // every thread has read carry_s and wsum
-    if (t == 0) carry_s += all;
-    __syncthreads();
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
   }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t before = 0u, tot = 0u;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+    if (w < wv) before += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *all = tot;
+  return before + inc - v;
+}
+
+// prefix over tiles of digit blockIdx.x, in place; total[d] = the digit's count
+__global__ __launch_bounds__(RS_SCAN_T) void k_rs_rowscan(int ntile, uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ total) {
+  __shared__ uint32_t wsum[RS_SCAN_T / 64];
+  uint32_t* row = counts + (size_t)blockIdx.x * ntile;
+  uint32_t carry = 0u;
+  for (int r0 = 0; r0 < ntile; r0 += RS_SCAN_T) {
+    const int i = r0 + threadIdx.x;
+    const uint32_t v = i < ntile ? row[i] : 0u;
+    uint32_t all;
+    const uint32_t ex = rs_block_scan(v, wsum, &all);
+    if (i < ntile) row[i] = carry + ex;
+    carry += all;
+  }
+  if (threadIdx.x == 0) total[blockIdx.x] = carry;
 }
 
 // stable scatter of one pass (vin NULL: the values are the input positions)
 __global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const uint32_t* __restrict__ kin,
                                                      const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
                                                      uint32_t* __restrict__ vout, int shift, int ntile,
-                                                     const uint32_t* __restrict__ offs) {
+                                                     const uint32_t* __restrict__ offs,
+                                                     const uint32_t* __restrict__ total) {
   __shared__ uint32_t wc[RS_T / 64][256];
+  __shared__ uint32_t wsum[RS_T / 64];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   for (int e = t; e < (RS_T / 64) * 256; e += RS_T) (&wc[0][0])[e] = 0u;
-  __syncthreads();
+  // start of each digit: exclusive scan of the digit totals (RS_T == 256)
+  uint32_t dall;
+  const uint32_t dstart = rs_block_scan(total[t], wsum, &dall);
   const long base = (long)blockIdx.x * RS_TILE + (long)w * (RS_CH * 64);
   const uint64_t below = (1ull << l) - 1ull;
   uint32_t k[RS_CH], v[RS_CH], rk[RS_CH];
@@ -153,10 +158,10 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const uint32_t* __re
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  // wave offsets within the tile plus the tile's base, per digit
+  // wave offsets within the tile plus the tile's and the digit's starts
   {
-    const int d = t;  // RS_T == 256 digits
-    uint32_t run = offs[(size_t)d * ntile + blockIdx.x];
+    const int d = t;
+    uint32_t run = dstart + offs[(size_t)d * ntile + blockIdx.x];
 #pragma unroll
     for (int ww = 0; ww < RS_T / 64; ww++) {
       const uint32_t c = wc[ww][d];
@@ -476,22 +481,23 @@ __global__ __launch_bounds__(CL_T) void k_class_place(ClassLevels L) {
 
 // Stable radix sort of the keys in kbuf[0] by their low `bits` bits, the
 // input positions as values, on stream s.  kbuf[0], kbuf[1] (keys) and
-// vbuf[0], vbuf[1] (values) ping-pong, n words each; counts holds
-// 256 * rs_tiles(n) words.  The sorted keys and values end in
-// kbuf[rs_out_slot(bits)] and vbuf[rs_out_slot(bits)].
+// vbuf[0], vbuf[1] (values) ping-pong, n words each.  The sorted keys and
+// values end in
+// kbuf[rs_out_slot(bits)] and vbuf[rs_out_slot(bits)].  counts holds
+// rs_count_words(n) words: digit counts per tile and the 256 digit totals.
 int rs_out_slot(int bits) { return rs_passes(bits) & 1; }
 
 void radix_sort(int n, int bits, uint32_t* const kbuf[2], uint32_t* const vbuf[2], uint32_t* counts,
                 hipStream_t s) {
   if (n <= 0) return;
   const int nt = rs_tiles(n), passes = rs_passes(bits);
-  const long total = 256L * nt;
+  uint32_t* total = counts + 256L * nt;
   for (int ps = 0; ps < passes; ps++) {
     const int src = ps & 1, dst = src ^ 1;
     k_rs_hist<<<nt, RS_T, 0, s>>>(n, kbuf[src], 8 * ps, nt, counts);
-    k_rs_scan<<<1, RS_SCAN_T, 0, s>>>(total, counts);
+    k_rs_rowscan<<<256, RS_SCAN_T, 0, s>>>(nt, counts, total);
     k_rs_scatter<<<nt, RS_T, 0, s>>>(n, kbuf[src], ps == 0 ? nullptr : vbuf[src], kbuf[dst], vbuf[dst], 8 * ps, nt,
-                                     counts);
+                                     counts, total);
   }
 }
 
@@ -521,7 +527,7 @@ int Engine::sort_points(int n, const float* x, const float* y, const float* z, i
   }
   const int lm = cfg.lmax;
   double rbinv = (double)((float)(1 << lm) / cfg.rd);
-  if ((rc = srt_bad.ensure(4)) || (rc = rs_counts.ensure((size_t)256 * rs_tiles(n) * 4))) return rc;
+  if ((rc = srt_bad.ensure(4)) || (rc = rs_counts.ensure((size_t)rs_count_words(n) * 4))) return rc;
   const int slot = rs_out_slot(3 * lm);
   uint32_t* kb[2];
   uint32_t* vb[2];
@@ -597,7 +603,7 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   if (n == 0) return FMMX_OK;
   if ((rc = xyz4.ensure(nn * 16))) return rc;
   double rbinv = (double)((float)(1 << lmax) / rd);
-  if ((rc = rs_counts.ensure((size_t)256 * rs_tiles(n) * 4))) return rc;
+  if ((rc = rs_counts.ensure((size_t)rs_count_words(n) * 4))) return rc;
   const int slot = rs_out_slot(3 * lmax);
   uint32_t* kb[2];
   uint32_t* vb[2];
