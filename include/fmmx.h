@@ -243,6 +243,26 @@ int fmmx_set_helmholtz(fmmx_handle* h, double k);
 int fmmx_sort(fmmx_handle* h, int n, const float* x, const float* y, const float* z, int* keys, int* perm,
               int flags);
 
+/* fmmx_sort without a handle and without allocating: no reference
+ * counterpart; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The keys and stable sort of the n points (x, y, z) in the root cube
+ * (xmin, ymin, zmin, rd) at leaf level lmax (1..8), as fmmx_sort computes
+ * them (perm[k]: input index of the k-th point in key order, keys[k] its
+ * key), asynchronously on `stream` of `device`.  Every pointer is a device
+ * pointer and none may be NULL; *bad is set to 1 when a coordinate lies
+ * outside the closed root cube or is not finite, else 0 (read it after the
+ * stream has finished).  n is at most FMMX_SORT_NOALLOC_MAX: the
+ * temporaries (ping-pong keys and values, digit counts) live in one static
+ * device array of about 9 bytes per point of that maximum (18.9 MB), which
+ * each device holds until the process ends.  All calls on one device share
+ * that array: calls that overlap in time on one device race (serialize them
+ * on one stream).  The first call on a device looks the array up; make it
+ * (with n = 0 if need be) outside graph capture. */
+#define FMMX_SORT_NOALLOC_MAX (1 << 21)
+int fmmx_sort_noalloc(int device, void* stream, int n, const float* x, const float* y, const float* z, int lmax,
+                      float xmin, float ymin, float zmin, float rd, int* keys, int* perm, int* bad);
+
 /* Wait for the handle's stream. */
 int fmmx_synchronize(fmmx_handle* h);
 

@@ -72,6 +72,8 @@ _SYMBOLS = {
     "fmmx_set_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
     "fmmx_sort": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                             C.c_int]),
+    "fmmx_sort_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fmmx_synchronize": (C.c_int, [C.c_void_p]),
     "fmmx_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "fmmx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),

@@ -1301,6 +1301,11 @@ int fmmx_sort(fmmx_handle* h, int n, const float* x, const float* y, const float
   })
 }
 
+int fmmx_sort_noalloc(int device, void* stream, int n, const float* x, const float* y, const float* z, int lmax,
+                      float xmin, float ymin, float zmin, float rd, int* keys, int* perm, int* bad) {
+  FMMX_GUARD({ return fmmx::sort_noalloc(device, stream, n, x, y, z, lmax, xmin, ymin, zmin, rd, keys, perm, bad); })
+}
+
 int fmmx_synchronize(fmmx_handle* h) {
   if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
   if (hipStreamSynchronize(h->e.stream) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "stream synchronize failed");

@@ -271,4 +271,8 @@ struct Engine {
   double rb(int lev) const { return (double)rd / (double)(1 << lev); }
 };
 
+// fmmx_sort_noalloc (tree.hip)
+int sort_noalloc(int device, void* stream, int n, const float* x, const float* y, const float* z, int lmax,
+                 float xmin, float ymin, float zmin, float rd, int* keys, int* perm, int* bad);
+
 }  // namespace fmmx

@@ -501,6 +501,70 @@ void radix_sort(int n, int bits, uint32_t* const kbuf[2], uint32_t* const vbuf[2
   }
 }
 
+// ---------------------------------------------------------------------
+// fmmx_sort_noalloc: the keys and stable sort of fmmx_sort with every
+// temporary in `scratch`, a static device array of fixed size (each device
+// holds its own copy until the process ends): the ping-pong keys and values
+// of the radix passes, the digit counts per tile.  Host code never uses the
+// array's name as an address: hipGetSymbolAddress gives each device's copy,
+// once per device, kept in g_scratch_ptr.
+namespace {
+constexpr long NA_MAX = FMMX_SORT_NOALLOC_MAX;
+constexpr long NA_COUNTS = 256L * ((NA_MAX + RS_TILE - 1) / RS_TILE + 1);
+constexpr long NA_WORDS = 2 * NA_MAX + NA_COUNTS;
+static __device__ __align__(256) uint32_t scratch[NA_WORDS];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+int scratch_ptr(int device, uint32_t** out) {
+  if (device < 0 || device >= 64) return fail(FMMX_EINVAL, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FMMX_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = static_cast<uint32_t*>(g_scratch_ptr[device]);
+  return FMMX_OK;
+}
+}  // namespace
+
+int sort_noalloc(int device, void* stream, int n, const float* x, const float* y, const float* z, int lmax,
+                 float xmin, float ymin, float zmin, float rd, int* keys, int* perm, int* bad) {
+  if (n < 0 || n > NA_MAX)
+    return fail(FMMX_EINVAL, "n = " + std::to_string(n) + " outside [0, FMMX_SORT_NOALLOC_MAX = " +
+                                 std::to_string(NA_MAX) + "]");
+  if (lmax < 1 || lmax > MAXL) return fail(FMMX_EINVAL, "lmax out of range [1, 8]");
+  if (!(rd > 0) || !std::isfinite(rd) || !std::isfinite(xmin) || !std::isfinite(ymin) || !std::isfinite(zmin))
+    return fail(FMMX_EINVAL, "root cube not finite or rd <= 0");
+  if (n > 0 && (!x || !y || !z || !keys || !perm || !bad)) return fail(FMMX_EINVAL, "NULL pointer");
+  int prev = 0;
+  FMMX_HIP(hipGetDevice(&prev));
+  if (prev != device) FMMX_HIP(hipSetDevice(device));
+  uint32_t* sp = nullptr;
+  int rc = scratch_ptr(device, &sp);
+  if (!rc && n > 0) {
+    hipStream_t s = (hipStream_t)stream;
+    const int slot = rs_out_slot(3 * lmax);
+    uint32_t* kb[2];
+    uint32_t* vb[2];
+    kb[slot] = reinterpret_cast<uint32_t*>(keys);
+    kb[slot ^ 1] = sp;
+    vb[slot] = reinterpret_cast<uint32_t*>(perm);
+    vb[slot ^ 1] = sp + NA_MAX;
+    const double rbinv = (double)((float)(1 << lmax) / rd);
+    k_fill_int<<<1, 32, 0, s>>>(1, 0, bad);
+    k_keys<<<nblk(n), TPB, 0, s>>>(n, x, y, z, xmin, ymin, zmin, rd, rbinv, lmax, kb[0], nullptr, nullptr, bad);
+    radix_sort(n, 3 * lmax, kb, vb, sp + 2 * NA_MAX, s);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(FMMX_EDEVICE, std::string("fmmx_sort_noalloc launch: ") + hipGetErrorString(e));
+  }
+  if (prev != device) (void)hipSetDevice(prev);
+  return rc;
+}
+
 // Keys of one point set at the leaf level and their stable sort, as
 // sort/sorti.cxx:12-26 (boxn, then the counting sort of sort/csort.cxx) leaves
 // them in nfn/na (sorted keys) and nbi (sorted position -> caller index).
