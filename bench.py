@@ -25,7 +25,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"hbm": 8000.0, "fp32_mfma": 157.3, "fp64_mfma": 78.6}
+# MI355X published peaks: HBM GB/s; dense MFMA TFLOP/s (FP16 without sparsity);
+# the FP32 vector peak is the published (packed) figure
+PEAK = {"hbm": 8000.0, "fp32_mfma": 157.3, "fp64_mfma": 78.6, "fp16_mfma": 2500.0, "fp32_vector": 157.3,
+        "fp64_vector": 78.6}
 
 
 def parse():
@@ -46,12 +49,15 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profile(n, p, lmax, all_levels):
+def traffic_from_profile(n, p, lmax, all_levels, split):
     """HBM bytes per M2L launch (every level in one launch, or the leaf
     level's) measured with rocprofv3 PMC counters (FETCH_SIZE + WRITE_SIZE,
     scaled by a plain copy of known size), kept as a text summary under
     profiles/; None when no summary matches this workload."""
-    name = "k_m2l_all_levels_n1m_p10_pmc.txt" if all_levels else "k_m2l_leaf_n1m_p10_pmc.txt"
+    if split:
+        name = "k_m2l_split_all_levels_n1m_p10_pmc.txt"
+    else:
+        name = "k_m2l_all_levels_n1m_p10_pmc.txt" if all_levels else "k_m2l_leaf_n1m_p10_pmc.txt"
     path = os.path.join(ROOT, "profiles", name)
     if not (os.path.exists(path) and (n, p, lmax) == (1_000_000, 10, 5)):
         return None
@@ -298,47 +304,122 @@ def main():
         e64 = float(torch.linalg.norm(v64[torch.from_numpy(idx).cuda()] - dv) / torch.linalg.norm(dv))
         fp64 = {"ms_per_step": ms64, "value": a.n / (ms64 * 1e3), "ms_eval_only": ms64_eval, "err_v": e64,
                 "phases_ms": {k: round(v_, 4) for k, v_ in ph64.items()}}
+        # the ratio's denominator against its own roof: the FP64 leaf M2L
+        # (vector FMA) on the FP64 peak
+        fl64 = 2.0 * a.p ** 4 * st64["m2l_pairs_leaf"]
+        ach64 = fl64 / (ph64["ms_m2l_leaf"] * 1e-3) / 1e12
+        fp64["roofline"] = {"bound": "valu", "achieved": ach64, "peak": PEAK["fp64_vector"], "unit": "TFLOP/s",
+                            "frac": ach64 / PEAK["fp64_vector"], "kernel": f"k_m2l_vec<{a.p}, double>, leaf level",
+                            "ms_per_launch": ph64["ms_m2l_leaf"]}
         f64.close()
 
-    # dominant kernel and its roofline position.  The M2L launch (FP32 MFMA;
-    # one launch over every level, or the leaf level's) carries most of the
-    # matvec's flops: algorithmic work 2 p^4 flop per M2L pair (dense
-    # p^2 x p^2 real operator), pairs counted by the kernel itself; duration
-    # from HIP events recorded on the engine's stream around that launch alone.
+    # dominant kernel and its roofline position.  The M2L launch (one launch
+    # over every level, or the leaf level's) carries most of the matvec's
+    # flops: algorithmic work 2 p^4 flop per M2L pair (dense p^2 x p^2 real
+    # operator), pairs counted by the kernel itself; duration from HIP events
+    # recorded on the engine's stream around that launch alone.  The mixed
+    # engine's M2L runs on the FP16 matrix cores (error-compensated split:
+    # three FP16 products per FP32-accurate product) or on FP32 MFMA.
     p2 = a.p * a.p
+    split = f.tuning[lmax][1] >= 3
     launch_pairs = st["m2l_pairs"] if st["m2l_all_levels"] else st["m2l_pairs_leaf"]
+    peak_m2l = PEAK["fp16_mfma"] if split else PEAK["fp32_mfma"]
     if phases["ms_m2l"] >= phases["ms_p2p"]:
         flops = 2.0 * p2 * p2 * launch_pairs
         achieved = flops / (phases["ms_m2l_leaf"] * 1e-3) / 1e12
-        tr = traffic_from_profile(a.n, a.p, lmax, st["m2l_all_levels"])
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["fp32_mfma"], "traffic": tr,
-                "kernel": (f"k_m2l, levels 2-{lmax} in one launch (FP32 MFMA 16x16x4)"
-                           if st["m2l_all_levels"] else f"k_m2l, leaf level {lmax} (FP32 MFMA 16x16x4)"),
+        tr = traffic_from_profile(a.n, a.p, lmax, st["m2l_all_levels"], split)
+        if split and f.tuning[lmax][1] == 5:
+            # executed MFMA work per pair: 32 MT rows x 16 KH k-positions per target
+            mt, ks = (p2 + 31) // 32, (p2 + 1) // 2
+            kh = (3 * (ks + (ks & 1)) + 7) // 8
+            executed = 2.0 * (32 * mt) * (16 * kh)
+            kern = f"k_m2l_w (error-compensated FP16 split, v_mfma_f32_32x32x16_f16), levels 2-{lmax} in one launch"
+        elif split:
+            # executed MFMA work per pair: 16 MT rows x 32 KH k-positions per target
+            mt, ks = (p2 + 15) // 16, (p2 + 3) // 4
+            kh = (3 * (ks + (ks & 1)) + 7) // 8
+            executed = 2.0 * (16 * mt) * (32 * kh)
+            kern = f"k_m2l_hd (error-compensated FP16 split, v_mfma_f32_16x16x32_f16), levels 2-{lmax} in one launch"
+        else:
+            executed = None
+            kern = f"k_m2l (FP32 MFMA 16x16x4), levels 2-{lmax} in one launch"
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak_m2l, "unit": "TFLOP/s",
+                "frac": achieved / peak_m2l, "traffic": tr, "kernel": kern,
                 "ms_per_launch": phases["ms_m2l_leaf"],
-                "work": f"2*p^4 = {2 * p2 * p2} flop x {launch_pairs} M2L pairs per launch",
-                "all_levels_tflops": 2.0 * p2 * p2 * st["m2l_pairs"] / (phases["ms_m2l"] * 1e-3) / 1e12}
+                "work": f"2*p^4 = {2 * p2 * p2} flop x {launch_pairs} M2L pairs per launch (algorithmic)",
+                "executed_flop_per_algorithmic": (executed / (2 * p2 * p2)) if executed else None,
+                "frac_vs_fp32_mfma_peak": achieved / PEAK["fp32_mfma"]}
     else:
         flops = 20.0 * st["p2p_pairs"]
         achieved = flops / (phases["ms_p2p"] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["fp32_mfma"], "traffic": None, "kernel": "k_p2p",
+        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK["fp32_vector"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK["fp32_vector"], "traffic": None, "kernel": "k_p2p",
                 "ms_per_launch": phases["ms_p2p"],
-                "work": f"20 flop x {st['p2p_pairs']} near pairs (FP32 vector ALU; the FP32 peak is quoted)"}
+                "work": f"20 flop x {st['p2p_pairs']} near pairs (FP32 vector ALU)"}
+
+    # every phase against its own bound: t_ideal = max(flop / peak, bytes /
+    # HBM), fraction = t_ideal / t_measured (serialized per-phase device times)
+    nbi = {lv: f.nboxes("i", lv) for lv in range(2, lmax + 1)}
+    nbj = {lv: f.nboxes("j", lv) for lv in range(2, lmax + 1)}
+    ks = (p2 + 3) // 4
+    vs = 4 * ((ks + 3) // 4 * 4)
+    passes = (3 * lmax + 7) // 8
+    trans = 2.0 * p2 * p2
+    work = {
+        # keys (12 B in, 4 B out), per radix pass 4 B (counts) + 8 B in + 8 B
+        # out, coordinate gather 16 B in + 12 B out, per point
+        "ms_tree": (0.0, a.n * (16.0 + 20.0 * passes + 28.0), None),
+        "ms_p2p": (20.0 * st["p2p_pairs"], 0.0, PEAK["fp32_vector"]),
+        "ms_p2m": (0.0, a.n * 16.0 + nbj[lmax] * vs * 4.0, None),
+        "ms_m2m": (trans * sum(nbj[lv] for lv in range(3, lmax + 1)), 0.0, PEAK["fp32_mfma"]),
+        "ms_m2l": (trans * st["m2l_pairs"], 0.0, peak_m2l),  # M2L (+ L2L below)
+        "ms_l2p": (0.0, a.n * 44.0 + nbi[lmax] * p2 * 4.0, None),
+    }
+    per_phase, t_ideal_sum = {}, 0.0
+    for k_, (fl, by, pk) in work.items():
+        t_ideal = max(fl / (pk * 1e12) if pk else 0.0, by / (PEAK["hbm"] * 1e9)) * 1e3
+        if k_ == "ms_m2l":
+            t_ideal += trans * sum(nbi[lv] for lv in range(3, lmax + 1)) / (PEAK["fp32_mfma"] * 1e12) * 1e3
+        per_phase[k_[3:]] = {"ms": round(phases[k_], 4), "ms_ideal": round(t_ideal, 4),
+                             "frac": round(t_ideal / phases[k_], 4) if phases[k_] > 0 else None}
+        t_ideal_sum += t_ideal
+    roofline_phases = {"phases": per_phase, "ms_ideal_step": round(t_ideal_sum, 4),
+                       "frac_step": t_ideal_sum / ms,
+                       "model": "t_ideal = max(flop/peak, bytes/8 TB/s) per phase; P2P on the FP32 vector peak, "
+                                "M2M/L2L on FP32 MFMA, M2L on the peak of the matrix unit it runs on; bytes of "
+                                "tree, P2M, L2P are compulsory HBM traffic"}
 
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
 
+        def cpu_model():
+            try:
+                for line in open("/proc/cpuinfo"):
+                    if line.startswith("model name"):
+                        return line.split(":", 1)[1].strip()
+            except OSError:
+                pass
+            return "unknown"
+
+        def oracle_run(threads):
+            t0 = time.perf_counter()
+            o = O.OracleFMM(xh, xh, mp=a.p, lmax=lmax, conv=O.EXACT, box=(0, 0, 0, 1), nthreads=threads)
+            o.run(qh)
+            tc = time.perf_counter() - t0
+            o.close()
+            return tc
+
+        # the host's share for one GPU (16 threads), and one thread (the
+        # reference's CPU path has no threading)
         cores = min(16, os.cpu_count() or 1)
-        t0 = time.perf_counter()
-        o = O.OracleFMM(xh, xh, mp=a.p, lmax=lmax, conv=O.EXACT, box=(0, 0, 0, 1), nthreads=cores)
-        o.run(qh)
-        tc = time.perf_counter() - t0
-        o.close()
+        tc = oracle_run(cores)
+        t1 = oracle_run(1)
         cpu = {"value": a.n / tc / 1e6, "unit": "Mpoints/s", "cores": cores, "kind": "port",
                "sample": f"one full matvec of the same workload (N={a.n}, p={a.p}, lmax={lmax}), FP64, "
-                         f"{tc:.1f} s wall on {cores} threads"}
+                         f"{tc:.2f} s wall on {cores} threads",
+               "single_thread": {"value": a.n / t1 / 1e6, "unit": "Mpoints/s", "cores": 1, "s": t1},
+               "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
     out = {
         "metric": "FMM matvec Mpoints/s, N=1M p=10, err<=1e-5 vs FP64 direct: mixed vs FP64",
@@ -358,7 +439,9 @@ def main():
                    "n": a.n, "p": a.p, "lmax": lmax, "precision": "mixed", "convention": "exact"},
         "err_v": err_v,
         "err_g": err_g,
-        "valid": err_v <= 1e-5,
+        # the metric's bound is on the potential; the gradient carries the
+        # p = 10 truncation at about 2e-5 and is held to 3e-5
+        "valid": err_v <= 1e-5 and err_g <= 3e-5,
         "ms_eval_only": ms_eval,
         "value_eval_only": a.n / (ms_eval * 1e3),
         "phases_ms": {k: round(v_, 4) for k, v_ in phases.items()},
@@ -367,6 +450,7 @@ def main():
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms) if fp64 else None,
         "roofline": roof,
+        "roofline_phases": roofline_phases,
         "cpu_baseline": cpu,
         "m2l_tuning": {str(k): v for k, v in f.tuning.items()},
     }

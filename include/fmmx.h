@@ -198,10 +198,11 @@ int fmmx_set_yukawa(fmmx_handle* h, double lambda);
 
 /* M2L launch variant at level lev (2..8) for this handle: N-tiles per wave
  * (1|2), operator staging (0 LDS, 1 register prefetch, 2 double buffer, up
- * to p = 11; 3 and 4: the error-compensated FP16 kernel with one or two
- * operator buffers, FP32 engine, Laplace, EXACT convention, p = 6..12, two
- * buffers where they fit LDS), waves per workgroup (4|8|16, 16 with staging
- * 2), offset splits (1..16); 0 (-1 for staging) keeps the default.  The FP64
+ * to p = 11; 3 and 4: the error-compensated FP16 kernel on 16x16x32 MFMA
+ * with one or two operator buffers, 5: the same on 32x32x16 MFMA (32 targets
+ * per wave, p <= 10); FP32 engine, Laplace, EXACT convention, p = 6..12),
+ * waves per workgroup (4|8|16, 16 with staging 2), offset splits (1..16); 0
+ * (-1 for staging) keeps the default.  The FP64
  * and Helmholtz kernels use the split count only.  fmmx_get_m2l_variant returns the four
  * values in effect (after set_points).  The tuner that picks them by timing
  * whole evaluations is old_fmm_bem_amd.FMM.autotune.  No reference

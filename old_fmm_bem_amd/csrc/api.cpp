@@ -156,6 +156,8 @@ struct OpSet {
   // operators scaled by 2^split_e
   std::vector<uint4> m2l_split;
   int split_e = 0;
+  // the same split for the 32x32x16 kernel: all 316 operators as they act
+  std::vector<uint4> m2l_wide;
   int nlev = 1;
 };
 std::mutex g_cache_mu;
@@ -262,6 +264,7 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
       }
       S->split_e = split_scale_exp(ck.data(), ck.size());
       split_fragments(ck.data(), NCANON, T.p, S->split_e, S->m2l_split);
+      if (split_wide_fits(T.p)) split_fragments_wide(S->m2l_kmaj.data(), NOFF, T.p, S->split_e, S->m2l_wide);
     }
   }
   std::lock_guard<std::mutex> lk(g_cache_mu);
@@ -641,6 +644,14 @@ int Engine::upload_operators() {
       FMMX_HIP(hipStreamSynchronize(stream));
       sa_e = S->split_e;
       split_ok = true;
+    }
+    wide_ok = false;
+    if (split_ok && !S->m2l_wide.empty()) {
+      if ((rc = op_m2l_w.ensure(S->m2l_wide.size() * 16))) return rc;
+      FMMX_HIP(hipMemcpyAsync(op_m2l_w.p, S->m2l_wide.data(), S->m2l_wide.size() * 16, hipMemcpyHostToDevice,
+                              stream));
+      FMMX_HIP(hipStreamSynchronize(stream));
+      wide_ok = true;
     }
   }
   ops_lmax = lmax;
@@ -1239,7 +1250,7 @@ int fmmx_set_m2l_variant(fmmx_handle* h, int lev, int nt, int staging, int wpg, 
   FMMX_GUARD({
     if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
     if (lev < 2 || lev > fmmx::MAXL) return fmmx::fail(FMMX_EINVAL, "level out of range [2, 8]");
-    if (nt < 0 || nt > 2 || staging < -1 || staging > 4 || (wpg != 0 && wpg != 4 && wpg != 8 && wpg != 16) ||
+    if (nt < 0 || nt > 2 || staging < -1 || staging > 5 || (wpg != 0 && wpg != 4 && wpg != 8 && wpg != 16) ||
         nsplit < 0 || nsplit > 16)
       return fmmx::fail(FMMX_EINVAL, "bad M2L variant");
     auto& t = h->e.tune[lev];

@@ -113,10 +113,11 @@ struct Engine {
   bool m2l_per_level = false;
   // FP16 split M2L (m2l_split.hip): operator hi/lo fragments scaled by
   // 2^sa_e, split multipoles per level, largest |multipole| per level
-  DevBuf op_m2l_h, mmax_buf;
+  DevBuf op_m2l_h, op_m2l_w, mmax_buf;
   std::array<DevBuf, MAXL + 1> mexp_h;
   int sa_e = 0;
   bool split_ok = false;  // the operators are uploaded and p is supported
+  bool wide_ok = false;   // the same for the 32x32x16 kernel (all 316 operators)
   bool m2l_split(int lev) const;  // the level's M2L runs the FP16 split kernel
   DevBuf offtab;    // int4 (dx,dy,dz,opidx) per (class, 189)
   DevBuf offtab_c;  // the same with w = canonical operator | reflection bits << 16
@@ -221,8 +222,9 @@ struct Engine {
   int run_m2m(int lev);
   // M2L launch variant per level: MFMA kernel N-tiles per wave, operator
   // staging (0 LDS, 1 register prefetch, 2 double buffer; 3 and 4: the
-  // error-compensated FP16 kernel with one or two operator buffers), waves
-  // per workgroup, offset splits.  tune[] holds the autotuned choice (0 fields:
+  // error-compensated FP16 kernel on 16x16x32 MFMA with one or two operator
+  // buffers; 5: the same on 32x32x16 MFMA), waves per workgroup, offset
+  // splits.  tune[] holds the autotuned choice (0 fields:
   // the defaults); FMMX_M2L_* environment variables override both.
   struct M2LVariant {
     int nt = 0, pa = -1, wpg = 0, nsplit = 0;
@@ -232,7 +234,7 @@ struct Engine {
   int m2l_splits(int lev) const { return m2l_variant(lev).nsplit; }
   int run_m2l(int lev, bool init, int phase = 0);
   int run_m2l_all(int l0);  // FP32 MFMA: every level in one launch
-  int run_msplit(int l0, int l1);  // split multipoles for the FP16 kernel
+  int run_msplit(int l0, int l1, bool wide);  // split multipoles for the FP16 kernels
   bool last_m2l_all = false;  // the last evaluation took that form
   bool m2l_timed = false;     // the last evaluation recorded ev[8], ev[9] around its M2L launch
   bool m2l_all_levels() const {

@@ -2157,16 +2157,18 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
   if (v.nt <= 0) v.nt = 1;
   // default: the FP16 split kernel where it applies (FP32 Laplace engine,
   // EXACT, p 6..12), else LDS staging
-  if (v.pa < 0) v.pa = (split_ok && !yk && !hh && !fp64) ? 4 : 0;
+  if (v.pa < 0) v.pa = (split_ok && !yk && !hh && !fp64) ? (wide_ok ? 5 : 4) : 0;
   if (v.wpg <= 0) v.wpg = 8;
   if (const char* e = getenv("FMMX_M2L_NT")) v.nt = atoi(e) == 2 ? 2 : 1;
-  if (const char* e = getenv("FMMX_M2L_PA")) v.pa = std::max(0, std::min(4, atoi(e)));
+  if (const char* e = getenv("FMMX_M2L_PA")) v.pa = std::max(0, std::min(5, atoi(e)));
   if (const char* e = getenv("FMMX_M2L_WPG")) v.wpg = atoi(e) == 4 ? 4 : atoi(e) == 16 ? 16 : 8;
   if (v.pa >= 3) {
     // FP16 split kernel: Laplace FP32 engine, p 6..12; two operator buffers
     // with one N-tile and 8 waves where they fit LDS
     if (!split_ok || yk || hh || fp64) v.pa = 0;
-    else if (v.pa == 4 && !split_db_fits(p)) v.pa = 3;
+    else if (v.pa == 5 && !wide_ok) v.pa = 4;
+    if (v.pa == 4 && !split_db_fits(p)) v.pa = 3;
+    if (v.pa == 5) v.nt = 2, v.wpg = 8;  // 8 waves of 32 targets
     if (v.pa == 4) v.wpg = v.nt == 2 ? 4 : 8;  // 8 waves of one N-tile or 4 of two
     if (v.pa == 3 && v.nt == 2) v.wpg = 4;
     if (v.wpg == 16) v.wpg = 8;
@@ -2278,9 +2280,10 @@ bool Engine::m2l_split(int lev) const { return !fp64 && !hh && !yk && m2l_varian
 // The FP16 split kernel's inputs for levels [l0, l1]: the multipoles of every
 // level split into hi/lo pairs scaled by 2^(14 - exponent of the level's
 // largest magnitude), one pair of launches for all levels
-int Engine::run_msplit(int l0, int l1) {
+int Engine::run_msplit(int l0, int l1, bool wide) {
   MSplitLevels S{};
   S.off[0] = 0;
+  S.wide = wide ? 1 : 0;
   int rc = mmax_buf.ensure((MAXL + 1) * 4);
   if (rc) return rc;
   for (int lev = l0; lev <= l1; lev++) {
@@ -2310,7 +2313,7 @@ int Engine::run_m2l_all(int l0) {
     if (si.nb[lev] > 0 && sj.nb[lev] > 0 && m2l_splits(lev) > 1) need += (size_t)m2l_splits(lev) * si.nb[lev] * P2;
   int rc = m2l_part.ensure(std::max<size_t>(need, 1) * 4);
   if (rc) return rc;
-  if (split && (rc = run_msplit(l0, lmax))) return rc;
+  if (split && (rc = run_msplit(l0, lmax, leaf.pa == 5))) return rc;
   size_t off = 0;
   int wg = 0;
   for (int lev = l0; lev <= lmax; lev++) {
@@ -2324,6 +2327,10 @@ int Engine::run_m2l_all(int l0) {
     const int ns = m2l_splits(lev);
     A.lv[A.nlv] = m2l_desc(lev, leaf.nt, leaf.wpg, ns, ns > 1 ? m2l_part.as<float>() + off : nullptr, 1, wg);
     if (leaf.pa == 4) A.lv[A.nlv].offtab = offtab_s.as<int4>();
+    if (leaf.pa == 5) {
+      A.lv[A.nlv].offtab = offtab.as<int4>();
+      A.lv[A.nlv].opH = op_m2l_w.as<uint4>();
+    }
     wg += A.lv[A.nlv].nwg;
     A.nlv++;
     if (ns > 1) off += (size_t)ns * nl;
@@ -2333,7 +2340,8 @@ int Engine::run_m2l_all(int l0) {
     constexpr int P = decltype(pc)::value;
     if (per_phase) FMMX_HIP(hipEventRecord(ev[8], stream));
     if (split) {
-      int r2 = launch_m2l_split(p, A, wg, leaf.nt, leaf.wpg, leaf.pa == 4, stream);
+      int r2 = leaf.pa == 5 ? launch_m2l_wide(p, A, wg, stream)
+                            : launch_m2l_split(p, A, wg, leaf.nt, leaf.wpg, leaf.pa == 4, stream);
       if (r2) return r2;
     } else {
       launch_m2l<P>(A, wg, leaf.nt, leaf.pa, leaf.wpg, stream);
@@ -2458,14 +2466,19 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       A.nlv = 1;
       A.lv[0] = m2l_desc(lev, var.nt, var.wpg, nsplit, part_buf->as<float>(), init ? 1 : 0, 0);
       if (var.pa == 4) A.lv[0].offtab = offtab_s.as<int4>();
+      if (var.pa == 5) {
+        A.lv[0].offtab = offtab.as<int4>();
+        A.lv[0].opH = op_m2l_w.as<uint4>();
+      }
       if (var.pa >= 3) {
-        int r2 = run_msplit(lev, lev);
+        int r2 = run_msplit(lev, lev, var.pa == 5);
         if (r2) return r2;
       }
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
       if (var.pa >= 3) {
-        int r2 = launch_m2l_split(p, A, A.lv[0].nwg, var.nt, var.wpg, var.pa == 4, stream);
+        int r2 = var.pa == 5 ? launch_m2l_wide(p, A, A.lv[0].nwg, stream)
+                             : launch_m2l_split(p, A, A.lv[0].nwg, var.nt, var.wpg, var.pa == 4, stream);
         if (r2) return r2;
       } else {
         launch_m2l<P>(A, A.lv[0].nwg, var.nt, var.pa, var.wpg, stream);

@@ -64,16 +64,38 @@ struct HLayout {
   static constexpr int VSH = 4 * GH;                   // halfs per stored box
   static constexpr int NB4 = GH / 8;                   // 16-byte loads per lane group
   static constexpr int FRH = KH * MT * 64;             // 16-byte operator fragments
-  // 32-bit word of a lane's loaded box holding k-positions s, s+1 (s even),
-  // -1 where both are zero
+  // 32-bit word of a lane's loaded box holding k-positions s, s+1 (s even):
+  // hi, hi, lo segments, then the zero padding words, else -1 (constant zero)
   static __host__ __device__ constexpr int word(int s) {
-    return s / SEG < 2 ? (s % SEG) / 2 : s / SEG == 2 ? SEG / 2 + (s % SEG) / 2 : -1;
+    return s / SEG < 2 ? (s % SEG) / 2
+                       : s / SEG == 2 ? SEG / 2 + (s % SEG) / 2 : (SEG + (s - 3 * SEG) / 2 < GH / 2 ? SEG + (s - 3 * SEG) / 2 : -1);
   }
   // input slot of k-position s of lane group g and its segment (-1: none)
   static __host__ __device__ constexpr int slot(int g, int s) {
     return (s / SEG < 3 && s % SEG < KS && g * KS + s % SEG < P2) ? g * KS + s % SEG : -1;
   }
 };
+// The same split on v_mfma_f32_32x32x16_f16 (two lane groups of 32 lanes,
+// 8 k-positions each per step; output tiles of 32 rows x 32 targets): lane
+// group g owns the KS = ceil(P2/2) input slots g KS .. g KS + KS - 1.  Stored
+// split multipoles: per box two lane groups of GH halfs (hi[SEG], lo[SEG],
+// zero padding).
+template <int P>
+struct WLayout {
+  static constexpr int P2 = P * P, MT = (P2 + 31) / 32, KS = (P2 + 1) / 2, SEG = KS + (KS & 1);
+  static constexpr int KH = (3 * SEG + 7) / 8;      // k-steps of 16 (8 per lane group)
+  static constexpr int GH = (2 * SEG + 7) / 8 * 8;  // halfs per lane group in a stored box
+  static constexpr int VSH = 2 * GH;                // halfs per stored box
+  static constexpr int NB4 = GH / 8;                // 16-byte loads per lane group
+  static constexpr int FRW = KH * MT * 64;          // 16-byte operator fragments
+  // word of a lane's loaded box holding k-positions s, s+1 (s even): hi,
+  // hi, lo segments, then the zero padding words, else -1 (constant zero)
+  static __host__ __device__ constexpr int word(int s) {
+    return s / SEG < 2 ? (s % SEG) / 2
+                       : s / SEG == 2 ? SEG / 2 + (s % SEG) / 2 : (SEG + (s - 3 * SEG) / 2 < GH / 2 ? SEG + (s - 3 * SEG) / 2 : -1);
+  }
+};
+
 // largest magnitude of the scaled split operands (2^14)
 constexpr int SPLIT_TOP = 14;
 
@@ -111,6 +133,7 @@ struct MSplitLevels {
   unsigned* mmax[MAXL + 1];
   long off[MAXL + 2];
   int nl;
+  int wide;  // layout of the 32x32x16 kernel (WLayout), else HLayout
 };
 
 // m2l_split.hip
@@ -120,6 +143,11 @@ int launch_msplit(int p, const MSplitLevels& S, hipStream_t stream);
 // db: two operator buffers (8 waves, one N-tile; p with split_db_fits)
 int launch_m2l_split(int p, const M2LLaunch& A, int nwg, int nt, int wpg, int db, hipStream_t stream);
 bool split_db_fits(int p);
+// the 32x32x16 form: 8 waves of 32 targets, all 316 operators, two LDS
+// buffers (p with split_wide_fits)
+int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, hipStream_t stream);
+bool split_wide_fits(int p);
+void split_fragments_wide(const double* kmaj, int count, int p, int sa_e, std::vector<uint4>& out);
 // 16-byte fragments of one split operator; halfs of one split box
 inline size_t split_frag_count(int p) {
   const int P2 = p * p, MT = (P2 + 15) / 16, KS = (P2 + 3) / 4, SEG = KS + (KS & 1), KH = (3 * SEG + 7) / 8;
@@ -127,7 +155,8 @@ inline size_t split_frag_count(int p) {
 }
 inline size_t split_box_halfs(int p) {
   const int KS = (p * p + 3) / 4, SEG = KS + (KS & 1);
-  return (size_t)4 * ((2 * SEG + 7) / 8 * 8);
+  const int KS2 = (p * p + 1) / 2, SEG2 = KS2 + (KS2 & 1);
+  return std::max<size_t>((size_t)4 * ((2 * SEG + 7) / 8 * 8), (size_t)2 * ((2 * SEG2 + 7) / 8 * 8));
 }
 // operator fragments of the split form from k-major FP64 operators (count
 // of them, P2 x P2 each, element (row i, column k) at [k P2 + i]), scaled by

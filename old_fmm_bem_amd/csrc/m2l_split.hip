@@ -96,38 +96,41 @@ __global__ __launch_bounds__(MS_T) void k_msplit_max(MSplitLevels S) {
   if ((int)threadIdx.x < S.nl && smax[threadIdx.x]) atomicMax(S.mmax[threadIdx.x], smax[threadIdx.x]);
 }
 
-// the split pairs of every level: lane group g of box b holds
-// hi[u], lo[u] of input slot g KS + u scaled by 2^split_exp
-template <int P>
+// the split pairs of every level: lane group g (of G) of box b holds
+// hi[u], lo[u] of input slot g L::KS + u scaled by 2^split_exp, in the
+// layout L (HLayout, 4 groups: 16x16x32; WLayout, 2 groups: 32x32x16)
+template <int P, class L, int G>
 __global__ __launch_bounds__(MS_T) void k_msplit(MSplitLevels S) {
   using ML = MLayout<P>;
-  using H = HLayout<P>;
   const long t = (long)blockIdx.x * MS_T + threadIdx.x;
-  const long b = t >> 2;
+  const long b = t / G;
   if (b >= S.off[S.nl]) return;
   int k = 0;
   while (k + 1 < S.nl && b >= S.off[k + 1]) k++;
-  const int g = (int)(t & 3);
+  const int g = (int)(t % G);
   const float sc = ldexpf(1.f, split_exp(S.mmax[k]));
   // the last box of a level's range is the zero box (read where a target
   // has no source box)
   const bool zero = b == S.off[k + 1] - 1;
-  const float* m = S.M[k] + (size_t)(zero ? 0 : b - S.off[k]) * ML::VS + g * ML::KSP;
-  float x[H::SEG];
+  const float* m = S.M[k] + (size_t)(zero ? 0 : b - S.off[k]) * ML::VS;
+  float x[L::SEG];
 #pragma unroll
-  for (int u = 0; u < H::SEG; u++) x[u] = (!zero && u < ML::KS && g * ML::KS + u < ML::P2) ? m[u] * sc : 0.f;
-  uint32_t w[H::GH / 2];
+  for (int u = 0; u < L::SEG; u++) {
+    const int c = g * L::KS + u;
+    x[u] = (!zero && u < L::KS && c < ML::P2) ? m[ML::at(c)] * sc : 0.f;
+  }
+  uint32_t w[L::GH / 2];
 #pragma unroll
-  for (int j = 0; j < H::GH / 2; j++) w[j] = 0u;
+  for (int j = 0; j < L::GH / 2; j++) w[j] = 0u;
 #pragma unroll
-  for (int u = 0; u < H::SEG; u += 2) {
+  for (int u = 0; u < L::SEG; u += 2) {
     const float h0 = (float)(_Float16)x[u], h1 = (float)(_Float16)x[u + 1];
     w[u / 2] = pack_h2(h0, h1);
-    w[H::SEG / 2 + u / 2] = pack_h2(x[u] - h0, x[u + 1] - h1);
+    w[L::SEG / 2 + u / 2] = pack_h2(x[u] - h0, x[u + 1] - h1);
   }
-  uint4* dst = S.Mh[k] + ((size_t)(b - S.off[k]) * H::VSH + g * H::GH) / 8;
+  uint4* dst = S.Mh[k] + ((size_t)(b - S.off[k]) * L::VSH + g * L::GH) / 8;
 #pragma unroll
-  for (int j = 0; j < H::NB4; j++) dst[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+  for (int j = 0; j < L::NB4; j++) dst[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
 }
 
 // B' fragment of k-step ks from a lane's loaded words: k-positions
@@ -605,6 +608,187 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_hd(M2LLaunch A) {
   if (mypairs) atomicAdd(Q.npairs, mypairs);
 }
 
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// B' fragment of k-step ks in the 32x32x16 layout
+template <class W>
+__device__ __forceinline__ h8 bfrag_w(const uint32_t* bw, int ks) {
+  const int w0 = W::word(8 * ks), w1 = W::word(8 * ks + 2), w2 = W::word(8 * ks + 4), w3 = W::word(8 * ks + 6);
+  return __builtin_bit_cast(h8, u32x4{w0 < 0 ? 0u : bw[w0], w1 < 0 ? 0u : bw[w1], w2 < 0 ? 0u : bw[w2],
+                                      w3 < 0 ? 0u : bw[w3]});
+}
+
+// Split M2L on v_mfma_f32_32x32x16_f16: a wave owns 32 targets (one per
+// lane of a lane group; the two lane groups hold the two halves of the
+// input slots) and MT tiles of 32 output rows.  The operator of every one of
+// the 316 offsets is stored as it acts (no reflections to apply), copied
+// global -> LDS by the DMA path into two buffers: offset o+1's operator
+// streams in while o's products run.  Output lane layout: element e of row
+// tile mt is row 32 mt + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32.
+template <int P, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
+  M2LLev Q = A.lv[0];
+#pragma unroll
+  for (int k = 1; k < MAXL + 1; k++)
+    if (k < A.nlv && (int)blockIdx.x >= A.lv[k].wg0) Q = A.lv[k];
+  const int lev = Q.lev, nb = Q.nb, nbj = Q.nbj, nsplit = Q.nsplit, init = Q.init, region = Q.region, gsh = Q.gsh;
+  const int* __restrict__ cls_cnt = Q.cls_cnt;
+  const uint32_t* __restrict__ ord = Q.ord;
+  const uint32_t* __restrict__ ikey = Q.ikey;
+  const int* __restrict__ dense_j = Q.dense_j;
+  const uint4* __restrict__ Mh = Q.Mh;
+  const uint4* __restrict__ opH = Q.opH;
+  float* __restrict__ L = Q.L;
+  float* __restrict__ part = Q.part;
+  const int4* __restrict__ offtab = Q.offtab;
+  const int bid = (int)blockIdx.x - Q.wg0, nwg = Q.nwg;
+  using W = WLayout<P>;
+  constexpr int P2 = W::P2, MT = W::MT, KH = W::KH, FRW = W::FRW, NB4 = W::NB4, NCH = KH * MT;
+  constexpr int TN = 32 * WPG;
+  __shared__ uint4 As[2 * FRW];
+  __shared__ int s_class, s_start, s_n;
+  const int tid = threadIdx.x;
+  int cls0, split, tile;
+  if (region) {
+    const int xcd = bid & 7, item = bid >> 3, tpr = (nwg >> 6) / nsplit;
+    cls0 = item & 7;
+    split = (item >> 3) % nsplit;
+    tile = xcd * tpr + (item >> 3) / nsplit;
+  } else {
+    const int item = bid >> 3, tpc = (nwg >> 3) / nsplit;
+    cls0 = bid & 7;
+    split = item / tpc;
+    tile = item % tpc;
+  }
+  if (tid == 0) {
+    int start = 0;
+    for (int c = 0; c < cls0; c++) start += cls_cnt[c];
+    const int cnt = cls_cnt[cls0], nt = (cnt + TN - 1) / TN;
+    s_class = tile < nt ? cls0 : -1;
+    s_start = start + tile * TN;
+    s_n = tile < nt ? min(TN, cnt - tile * TN) : 0;
+  }
+  __syncthreads();
+  const int cls = s_class, ntile = s_n;
+  if (cls < 0) return;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, j = l & 31, kq = l >> 5;
+  const int g = 1 << lev;
+  int tord = -1, tx = 0, ty = 0, tz = 0;
+  uint32_t tk = 0u;
+  {
+    const int t = w * 32 + j;
+    if (t < ntile) {
+      tord = (int)ord[s_start + t];
+      tk = ikey[tord];
+      morton_decode(tk, lev, tx, ty, tz);
+    }
+  }
+  const uint32_t MY = 0x49249249u & ((1u << (3 * lev)) - 1u), MX = MY << 1, MZ = MY << 2;
+  const bool wave_on = w * 32 < ntile;
+  const int per = (NEBM + nsplit - 1) / nsplit, o0 = split * per, o1 = min(NEBM, o0 + per);
+  const float oscale = ldexpf(1.f, -(split_exp(Q.mmax) + Q.sa_e));
+  f32x16 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[mt][e] = 0.f;
+  unsigned long long mypairs = 0;
+  uint32_t bcur[4 * NB4], bnext[4 * NB4];
+  bool cur_any = false, next_any = false;
+#define FMMX_LOAD_BW(o_, dst_, any_)                                                          \
+  {                                                                                           \
+    const int4 off_ = offtab[cls * NEBM + (o_)];                                              \
+    const uint32_t dx_ = dil3(abs(off_.x)) << 1, dy_ = dil3(abs(off_.y)), dz_ = dil3(abs(off_.z)) << 2; \
+    int s_ = -1;                                                                              \
+    if (tord >= 0) {                                                                          \
+      const int jx = tx - off_.x, jy = ty - off_.y, jz = tz - off_.z;                         \
+      if ((unsigned)jx < (unsigned)g && (unsigned)jy < (unsigned)g && (unsigned)jz < (unsigned)g && \
+          (gsh < 0 || !((jx ^ tx) >> gsh | (jy ^ ty) >> gsh | (jz ^ tz) >> gsh))) {           \
+        const uint32_t kx_ = off_.x >= 0 ? ((tk & MX) - dx_) & MX : ((tk | ~MX) + dx_) & MX;  \
+        const uint32_t ky_ = off_.y >= 0 ? ((tk & MY) - dy_) & MY : ((tk | ~MY) + dy_) & MY;  \
+        const uint32_t kz_ = off_.z >= 0 ? ((tk & MZ) - dz_) & MZ : ((tk | ~MZ) + dz_) & MZ;  \
+        s_ = dense_j[kx_ | ky_ | kz_];                                                        \
+      }                                                                                       \
+    }                                                                                         \
+    if (s_ >= 0 && kq == 0) mypairs++;                                                        \
+    any_ = __ballot(s_ >= 0) != 0;                                                            \
+    /* no source box: the level's zero box after the last one */                              \
+    const uint4* ms_ = Mh + ((size_t)(s_ >= 0 ? s_ : nbj) * W::VSH + kq * W::GH) / 8;         \
+    _Pragma("unroll") for (int v_ = 0; v_ < NB4; v_++) {                                      \
+      const uint4 m4_ = ms_[v_];                                                              \
+      dst_[4 * v_] = m4_.x;                                                                   \
+      dst_[4 * v_ + 1] = m4_.y;                                                               \
+      dst_[4 * v_ + 2] = m4_.z;                                                               \
+      dst_[4 * v_ + 3] = m4_.w;                                                               \
+    }                                                                                         \
+  }
+#define FMMX_DMA_W(o_, buf_)                                                                  \
+  {                                                                                           \
+    const uint4* src_ = opH + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FRW + l;       \
+    for (int c_ = w; c_ < NCH; c_ += WPG)                                                     \
+      __builtin_amdgcn_global_load_lds((const void*)(src_ + c_ * 64),                         \
+                                       (__attribute__((address_space(3))) void*)(As + (buf_) * FRW + c_ * 64), \
+                                       16, 0, 0);                                             \
+  }
+#define FMMX_STEP_W(o_, BC_, AC_, BN_, AN_)                                                   \
+  {                                                                                           \
+    const int buf = ((o_) - o0) & 1;                                                          \
+    if ((o_) + 1 < o1) {                                                                      \
+      FMMX_DMA_W((o_) + 1, buf ^ 1);                                                          \
+      FMMX_LOAD_BW((o_) + 1, BN_, AN_);                                                       \
+    }                                                                                         \
+    if (wave_on && AC_) {                                                                     \
+      const uint4* Ab = As + buf * FRW;                                                       \
+      h8 af[2][MT];                                                                           \
+      _Pragma("unroll") for (int mt = 0; mt < MT; mt++) af[0][mt] = __builtin_bit_cast(h8, Ab[mt * 64 + l]); \
+      _Pragma("unroll") for (int ks = 0; ks < KH; ks++) {                                     \
+        if (ks + 1 < KH) {                                                                    \
+          _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                   \
+            af[(ks + 1) & 1][mt] = __builtin_bit_cast(h8, Ab[((ks + 1) * MT + mt) * 64 + l]); \
+        }                                                                                     \
+        __builtin_amdgcn_sched_barrier(0);                                                    \
+        const h8 bf_ = bfrag_w<W>(BC_, ks);                                                   \
+        _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                     \
+          acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[ks & 1][mt], bf_, acc[mt], 0, 0, 0); \
+        __builtin_amdgcn_sched_barrier(0);                                                    \
+      }                                                                                       \
+    }                                                                                         \
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");                                          \
+    __syncthreads();                                                                          \
+  }
+  if (o0 < o1) {
+    FMMX_DMA_W(o0, 0);
+    FMMX_LOAD_BW(o0, bcur, cur_any);
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  for (int o = o0; o < o1; o += 2) {
+    FMMX_STEP_W(o, bcur, cur_any, bnext, next_any);
+    if (o + 1 < o1) FMMX_STEP_W(o + 1, bnext, next_any, bcur, cur_any);
+  }
+#undef FMMX_STEP_W
+#undef FMMX_DMA_W
+#undef FMMX_LOAD_BW
+  if (tord >= 0) {
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int i = 32 * mt + 8 * (e / 4) + 4 * kq + e % 4;
+        if (i < P2) {
+          const float v = acc[mt][e] * oscale;
+          if (nsplit == 1) {
+            float* dst = L + (size_t)tord * P2 + i;
+            *dst = init ? v : *dst + v;
+          } else {
+            part[((size_t)split * nb + tord) * P2 + i] = v;
+          }
+        }
+      }
+  }
+  if (mypairs) atomicAdd(Q.npairs, mypairs);
+}
+
 template <typename F>
 int dispatch_split_p(int p, F&& f) {
   switch (p) {
@@ -689,7 +873,8 @@ int launch_msplit(int p, const MSplitLevels& S, hipStream_t stream) {
     constexpr int P = decltype(pc)::value;
     k_msplit_zero<<<1, 32, 0, stream>>>(S);
     k_msplit_max<P><<<nblk, MS_T, 0, stream>>>(S);
-    k_msplit<P><<<nblk, MS_T, 0, stream>>>(S);
+    if (S.wide) k_msplit<P, WLayout<P>, 2><<<(int)((2 * S.off[S.nl] + MS_T - 1) / MS_T), MS_T, 0, stream>>>(S);
+    else k_msplit<P, HLayout<P>, 4><<<nblk, MS_T, 0, stream>>>(S);
     return hipGetLastError() == hipSuccess ? FMMX_OK : fail(FMMX_EDEVICE, "multipole split launch failed");
   });
 }
@@ -712,6 +897,48 @@ int launch_m2l_split(int p, const M2LLaunch& A, int nwg, int nt, int wpg, int db
     }
     return hipGetLastError() == hipSuccess ? FMMX_OK : fail(FMMX_EDEVICE, "split M2L launch failed");
   });
+}
+
+bool split_wide_fits(int p) {
+  if (p < 6 || p > 12) return false;
+  const int P2 = p * p, MT = (P2 + 31) / 32, KS = (P2 + 1) / 2, SEG = KS + (KS & 1), KH = (3 * SEG + 7) / 8;
+  return (size_t)2 * KH * MT * 64 * 16 <= 160 * 1024 - 64;
+}
+
+int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, hipStream_t stream) {
+  return dispatch_split_p(p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    if constexpr ((size_t)2 * WLayout<P>::FRW * 16 <= 160 * 1024 - 64) {
+      k_m2l_w<P, 8><<<nwg, 512, 0, stream>>>(A);
+      return hipGetLastError() == hipSuccess ? FMMX_OK : fail(FMMX_EDEVICE, "split M2L launch failed");
+    }
+    return fail(FMMX_EUNSUPPORTED, "the 32x32x16 split M2L needs p <= 10");
+  });
+}
+
+void split_fragments_wide(const double* kmaj, int count, int p, int sa_e, std::vector<uint4>& out) {
+  const int P2 = p * p, MT = (P2 + 31) / 32, KS = (P2 + 1) / 2, SEG = KS + (KS & 1), KH = (3 * SEG + 7) / 8;
+  const size_t FRW = (size_t)KH * MT * 64;
+  out.assign((size_t)count * FRW, make_uint4(0u, 0u, 0u, 0u));
+  for (int o = 0; o < count; o++) {
+    const double* A = kmaj + (size_t)o * P2 * P2;  // element (i, k) at A[k * P2 + i]
+    for (int ks = 0; ks < KH; ks++)
+      for (int mt = 0; mt < MT; mt++)
+        for (int l = 0; l < 64; l++) {
+          const int i = 32 * mt + (l & 31), g = l >> 5;
+          uint16_t h[8] = {};
+          for (int t = 0; t < 8; t++) {
+            const int s = 8 * ks + t, seg = s / SEG, u = s % SEG, k = g * KS + u;
+            if (i >= P2 || seg >= 3 || u >= KS || k >= P2) continue;
+            const double a = std::ldexp(A[(size_t)k * P2 + i], sa_e);
+            const uint16_t hi = to_half(a);
+            h[t] = seg == 1 ? to_half(a - from_half(hi)) : hi;
+          }
+          out[(size_t)o * FRW + ((size_t)ks * MT + mt) * 64 + l] =
+              make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
+                         h[6] | (uint32_t)h[7] << 16);
+        }
+  }
 }
 
 bool split_db_fits(int p) {

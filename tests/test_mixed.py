@@ -89,7 +89,7 @@ def test_mixed_edge_cases():
     f.close()
 
 
-SPLIT_SHAPES = [(1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
+SPLIT_SHAPES = [(2, 5, 8), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
 
 
 def _with_shape(f, shape, lmax):
@@ -100,7 +100,8 @@ def _with_shape(f, shape, lmax):
 @pytest.mark.parametrize("p", [6, 8, 10, 12])
 def test_split_fp16_m2l_matches_fp32(p):
     """The error-compensated FP16 M2L (staging 3, 4: hi/lo operands, three
-    products per pair on v_mfma_f32_16x16x32_f16) against the FP32 MFMA M2L
+    products per pair on v_mfma_f32_16x16x32_f16; 5: on 32x32x16, p <= 10)
+    against the FP32 MFMA M2L
     and the FP64 engine on the same tree, every launch shape: the split form
     carries 2^-22 relative operand error against FP32's 2^-24, so its
     difference from the FP64 engine stays at FP32 rounding (bound 2e-6, the
@@ -121,7 +122,7 @@ def test_split_fp16_m2l_matches_fp32(p):
     v32, g32, _ = res["mixed", (1, 0, 8)]
     for s in SPLIT_SHAPES:
         v, g, eff = res["mixed", s]
-        assert eff[1] in (3, 4), eff  # the split kernel ran
+        assert eff[1] in (3, 4, 5), eff  # a split kernel ran
         ev, eg = rel_l2(v, vd), rel_l2(g, gd)
         e32 = rel_l2(v, v32)
         print(f"p={p} shape={s} -> {eff}: vs FP64 {ev:.2e} / {eg:.2e}, vs FP32 M2L {e32:.2e}", flush=True)
@@ -134,12 +135,13 @@ def test_split_fp16_m2l_scale_invariant():
     the per-level power-of-two scaling of the split multipoles gives the same
     relative result."""
     x, q = uniform_cube(30000, 77)
-    out = []
-    for s in (1.0, 2.0 ** -40, 2.0 ** 40):
-        f = F.FMM(p=10, lmax=4, precision="mixed", convention="exact", box=(0, 0, 0, 1))
-        f.set_points(x)
-        _with_shape(f, (1, 4, 8), 4)
-        v, _ = f.evaluate((q * np.float32(s)).astype(np.float32))
-        f.close()
-        out.append(v / s)
-    assert rel_l2(out[1], out[0]) < 1e-12 and rel_l2(out[2], out[0]) < 1e-12
+    for shape in ((2, 5, 8), (1, 4, 8)):
+        out = []
+        for s in (1.0, 2.0 ** -40, 2.0 ** 40):
+            f = F.FMM(p=10, lmax=4, precision="mixed", convention="exact", box=(0, 0, 0, 1))
+            f.set_points(x)
+            _with_shape(f, shape, 4)
+            v, _ = f.evaluate((q * np.float32(s)).astype(np.float32))
+            f.close()
+            out.append(v / s)
+        assert rel_l2(out[1], out[0]) < 1e-12 and rel_l2(out[2], out[0]) < 1e-12
