@@ -240,7 +240,7 @@ class FMM:
         if self.precision != FP64 and neq in (G, GNI, GNJ):
             # the error-compensated FP16 kernel (falls back to the FP32 shape
             # where it does not apply: Yukawa, REF convention, p outside 6..12)
-            shapes += [(2, 5, 8), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
+            shapes += [(2, 5, 8), (2, 5, 4), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
         for lev in range(lmax, 1, -1):
             if self.nboxes("i", lev) == 0:
                 continue

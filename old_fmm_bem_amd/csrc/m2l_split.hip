@@ -625,7 +625,7 @@ __device__ __forceinline__ h8 bfrag_w(const uint32_t* bw, int ks) {
 // global -> LDS by the DMA path into two buffers: offset o+1's operator
 // streams in while o's products run.  Output lane layout: element e of row
 // tile mt is row 32 mt + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32.
-template <int P, int WPG>
+template <int P, int WPG, int DB>
 __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
   M2LLev Q = A.lv[0];
 #pragma unroll
@@ -645,7 +645,10 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
   using W = WLayout<P>;
   constexpr int P2 = W::P2, MT = W::MT, KH = W::KH, FRW = W::FRW, NB4 = W::NB4, NCH = KH * MT;
   constexpr int TN = 32 * WPG;
-  __shared__ uint4 As[2 * FRW];
+  // two operator buffers as distinct objects, so that the compiler sees the
+  // fragment reads of one and the copy into the other as independent (no
+  // wait for the copy before the products)
+  __shared__ uint4 As0[FRW], As1[DB ? FRW : 1];
   __shared__ int s_class, s_start, s_n;
   const int tid = threadIdx.x;
   int cls0, split, tile;
@@ -727,24 +730,27 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
     const uint4* src_ = opH + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FRW + l;       \
     for (int c_ = w; c_ < NCH; c_ += WPG)                                                     \
       __builtin_amdgcn_global_load_lds((const void*)(src_ + c_ * 64),                         \
-                                       (__attribute__((address_space(3))) void*)(As + (buf_) * FRW + c_ * 64), \
-                                       16, 0, 0);                                             \
+                                       (__attribute__((address_space(3))) void*)((buf_) + c_ * 64), 16, 0, 0); \
   }
-#define FMMX_STEP_W(o_, BC_, AC_, BN_, AN_)                                                   \
+#define FMMX_STEP_W(o_, BC_, AC_, BN_, AN_, ACUR_, ANEXT_)                                     \
   {                                                                                           \
-    const int buf = ((o_) - o0) & 1;                                                          \
-    if ((o_) + 1 < o1) {                                                                      \
-      FMMX_DMA_W((o_) + 1, buf ^ 1);                                                          \
-      FMMX_LOAD_BW((o_) + 1, BN_, AN_);                                                       \
-    }                                                                                         \
+    /* the next multipoles are plain loads, in flight during the products; */                  \
+    /* the next operator's copy is issued after this step's last read of  */                   \
+    /* LDS (a copy into LDS orders every later LDS read behind it)        */                   \
+    if ((o_) + 1 < o1) FMMX_LOAD_BW((o_) + 1, BN_, AN_);                                      \
+    bool dma_ = (o_) + 1 < o1;                                                                \
     if (wave_on && AC_) {                                                                     \
-      const uint4* Ab = As + buf * FRW;                                                       \
+      const uint4* Ab = ACUR_;                                                                \
       h8 af[2][MT];                                                                           \
       _Pragma("unroll") for (int mt = 0; mt < MT; mt++) af[0][mt] = __builtin_bit_cast(h8, Ab[mt * 64 + l]); \
       _Pragma("unroll") for (int ks = 0; ks < KH; ks++) {                                     \
         if (ks + 1 < KH) {                                                                    \
           _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                   \
             af[(ks + 1) & 1][mt] = __builtin_bit_cast(h8, Ab[((ks + 1) * MT + mt) * 64 + l]); \
+        }                                                                                     \
+        if (ks + 2 == KH && dma_) {                                                           \
+          FMMX_DMA_W((o_) + 1, ANEXT_);                                                       \
+          dma_ = false;                                                                       \
         }                                                                                     \
         __builtin_amdgcn_sched_barrier(0);                                                    \
         const h8 bf_ = bfrag_w<W>(BC_, ks);                                                   \
@@ -753,18 +759,51 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
         __builtin_amdgcn_sched_barrier(0);                                                    \
       }                                                                                       \
     }                                                                                         \
+    if (dma_) FMMX_DMA_W((o_) + 1, ANEXT_);                                                   \
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");                                          \
     __syncthreads();                                                                          \
   }
-  if (o0 < o1) {
-    FMMX_DMA_W(o0, 0);
-    FMMX_LOAD_BW(o0, bcur, cur_any);
-  }
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  __syncthreads();
-  for (int o = o0; o < o1; o += 2) {
-    FMMX_STEP_W(o, bcur, cur_any, bnext, next_any);
-    if (o + 1 < o1) FMMX_STEP_W(o + 1, bnext, next_any, bcur, cur_any);
+  if constexpr (DB) {
+    if (o0 < o1) {
+      FMMX_DMA_W(o0, As0);
+      FMMX_LOAD_BW(o0, bcur, cur_any);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int o = o0; o < o1; o += 2) {
+      FMMX_STEP_W(o, bcur, cur_any, bnext, next_any, As0, As1);
+      if (o + 1 < o1) FMMX_STEP_W(o + 1, bnext, next_any, bcur, cur_any, As1, As0);
+    }
+  } else {
+    // one buffer, two workgroups per CU: one computes while the other waits
+    // for its operator copy and multipoles
+    for (int o = o0; o < o1; o++) {
+      __syncthreads();  // every wave is done with the previous operator
+      FMMX_DMA_W(o, As0);
+      FMMX_LOAD_BW(o, bcur, cur_any);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      if (wave_on && cur_any) {
+        h8 af[2][MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) af[0][mt] = __builtin_bit_cast(h8, As0[mt * 64 + l]);
+#pragma unroll
+        for (int ks = 0; ks < KH; ks++) {
+          if (ks + 1 < KH) {
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) af[(ks + 1) & 1][mt] = __builtin_bit_cast(h8, As0[((ks + 1) * MT + mt) * 64 + l]);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          const h8 bf = bfrag_w<W>(bcur, ks);
+#pragma unroll
+          for (int mt = 0; mt < MT; mt++)
+            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[ks & 1][mt], bf, acc[mt], 0, 0, 0);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    (void)bnext;
+    (void)next_any;
   }
 #undef FMMX_STEP_W
 #undef FMMX_DMA_W
@@ -905,11 +944,12 @@ bool split_wide_fits(int p) {
   return (size_t)2 * KH * MT * 64 * 16 <= 160 * 1024 - 64;
 }
 
-int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, hipStream_t stream) {
+int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, int wpg, hipStream_t stream) {
   return dispatch_split_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if constexpr ((size_t)2 * WLayout<P>::FRW * 16 <= 160 * 1024 - 64) {
-      k_m2l_w<P, 8><<<nwg, 512, 0, stream>>>(A);
+      if (wpg == 4) k_m2l_w<P, 4, 0><<<nwg, 256, 0, stream>>>(A);
+      else k_m2l_w<P, 8, 1><<<nwg, 512, 0, stream>>>(A);
       return hipGetLastError() == hipSuccess ? FMMX_OK : fail(FMMX_EDEVICE, "split M2L launch failed");
     }
     return fail(FMMX_EUNSUPPORTED, "the 32x32x16 split M2L needs p <= 10");

@@ -89,7 +89,7 @@ def test_mixed_edge_cases():
     f.close()
 
 
-SPLIT_SHAPES = [(2, 5, 8), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
+SPLIT_SHAPES = [(2, 5, 8), (2, 5, 4), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
 
 
 def _with_shape(f, shape, lmax):
