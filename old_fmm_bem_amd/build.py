@@ -26,8 +26,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I" + os.path.join(ROOT, "include"),
           "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 
-CORE = ["precompute.cpp", "yukawa.cpp", "helmholtz.cpp", "tree.hip", "kernels.hip", "m2l_split.hip", "api.cpp",
-        "adjoint.cpp"]
+CORE = ["precompute.cpp", "yukawa.cpp", "helmholtz.cpp", "split_host.cpp", "tree.hip", "kernels.hip", "m2l_split.hip",
+        "api.cpp", "adjoint.cpp"]
 SHIM = ["refshim.cpp"]
 
 
