@@ -70,7 +70,9 @@ enum {
 
 typedef struct fmmx_config {
   int p;          /* expansion degrees n = 0..p-1 (the reference's mp), 2..FMMX_MAX_P */
-  int lmax;       /* leaf level >= 2; 0 = fmm/nlevel.cxx thresholds on nj */
+  int lmax;       /* leaf level >= 2; 0 = automatic: FMMX_CONV_REF the
+                     fmm/nlevel.cxx thresholds on nj, FMMX_CONV_EXACT the
+                     measured cost model of fmmx_auto_lmax */
   int precision;  /* FMMX_MIXED or FMMX_FP64 */
   int convention; /* FMMX_CONV_REF or FMMX_CONV_EXACT */
   float xmin, ymin, zmin, rd; /* root cube (include/constants.h:56-59);
@@ -196,11 +198,12 @@ int fmmx_set_domain(fmmx_handle* h, int lmax, float xmin, float ymin, float zmin
  * operators are built at the first Yukawa evaluate after a change. */
 int fmmx_set_yukawa(fmmx_handle* h, double lambda);
 
-/* M2L launch variant at level lev (2..8) for this handle: N-tiles per wave
- * (1|2), operator staging (0 LDS, 1 register prefetch, 2 double buffer, up
- * to p = 11; 3 and 4: the error-compensated FP16 kernel on 16x16x32 MFMA
- * with one or two operator buffers, 5: the same on 32x32x16 MFMA (32 targets
- * per wave, p <= 10); FP32 engine, Laplace, EXACT convention, p = 6..12),
+/* M2L launch variant at level lev (2..8) for this handle: N-tiles of 16
+ * targets per wave (1|2; 4 only with staging 5), operator staging (0 LDS, 1
+ * register prefetch, 2 double buffer, up to p = 11; 3 and 4: the
+ * error-compensated FP16 kernel on 16x16x32 MFMA with one or two operator
+ * buffers, 5: the same on 32x32x16 MFMA (32 targets per wave, or 64 with 4
+ * N-tiles, p <= 10); FP32 engine, Laplace, EXACT convention, p = 6..12),
  * waves per workgroup (4|8|16, 16 with staging 2), offset splits (1..16); 0
  * (-1 for staging) keeps the default.  The FP64
  * and Helmholtz kernels use the split count only.  fmmx_get_m2l_variant returns the four
@@ -294,6 +297,19 @@ int fmmx_get_count(fmmx_handle* h, int what, int side, int lev);
  * 56-59), as fmmx_set_domain gave them or as computed from the points
  * (rd <= 0, bio/main.cxx:355-368; lmax 0, fmm/nlevel.cxx:8) */
 int fmmx_get_domain(fmmx_handle* h, int* lmax, float* box4);
+/* The depth rule of lmax 0 under FMMX_CONV_EXACT (host arithmetic only, no
+ * handle): the leaf level 2..8 of least modelled cost, from the box
+ * statistics of one point set at levels l = 0..8 of its root cube:
+ * boxes[l] = boxes holding at least one point, sumsq[l] = sum over boxes of
+ * the squared point count.  The model's far-field/near-field cost ratio was
+ * fitted per order p and precision to evaluation times measured on MI355X.
+ * No counterpart in the reference, whose depth is fmm/nlevel.cxx:8 (kept
+ * under FMMX_CONV_REF).  The level goes to *lmax. */
+int fmmx_auto_lmax(const long long* boxes, const double* sumsq, int p, int precision, int* lmax);
+/* the box statistics (levels 0..8) the last set_points chose its depth from
+ * (its sources, else its targets); zeros when lmax was given or the
+ * convention is FMMX_CONV_REF */
+int fmmx_get_occupancy(fmmx_handle* h, long long* boxes, double* sumsq);
 /* sort permutation: sorted position -> caller index (sort/sorti.cxx:39-41) */
 int fmmx_get_perm(fmmx_handle* h, int side, int* perm);
 /* box keys at a level; particle ranges [first,last] (inclusive, as ndi/ndj

@@ -20,7 +20,7 @@ import time
 
 import numpy as np
 
-__all__ = ["FMM", "FMMError", "lib", "operator", "G", "GNI", "GNJ", "YUKAWA", "HELMHOLTZ", "MIXED", "FP64", "REF",
+__all__ = ["FMM", "FMMError", "lib", "operator", "auto_lmax", "G", "GNI", "GNJ", "YUKAWA", "HELMHOLTZ", "MIXED", "FP64", "REF",
            "EXACT"]
 
 G, GNI, GNJ, YUKAWA, HELMHOLTZ = 0, 1, 2, 10, 20
@@ -79,6 +79,8 @@ _SYMBOLS = {
     "fmmx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "fmmx_get_count": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "fmmx_get_domain": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fmmx_auto_lmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "fmmx_get_occupancy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fmmx_evaluate_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
     "fmmx_get_multipoles_level": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fmmx_evaluate_transpose": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 5 + [C.c_int]),
@@ -97,6 +99,20 @@ _SYMBOLS = {
 
 def library_path() -> str:
     return os.path.join(_HERE, "libfmmx.so")
+
+
+def auto_lmax(boxes, sumsq, p, precision="mixed"):
+    """The engine's depth rule for lmax 0 in the EXACT convention: the leaf
+    level 2..8 of least modelled cost, from the box statistics of a point set
+    at levels 0..8 of its root cube (boxes[l] holding a point, sumsq[l] the
+    sum of squared points per box)."""
+    b = np.ascontiguousarray(boxes, dtype=np.int64)
+    q = np.ascontiguousarray(sumsq, dtype=np.float64)
+    if b.shape != (9,) or q.shape != (9,):
+        raise ValueError("boxes and sumsq need 9 entries (levels 0..8)")
+    lm = C.c_int(0)
+    _check(lib().fmmx_auto_lmax(b.ctypes.data, q.ctypes.data, int(p), _PREC[precision], C.byref(lm)))
+    return lm.value
 
 
 def lib():
@@ -214,14 +230,16 @@ class FMM:
     def set_m2l_variant(self, lev, nt=0, staging=-1, wpg=0, nsplit=0):
         _check(lib().fmmx_set_m2l_variant(self.h, lev, nt, staging, wpg, nsplit))
 
-    def autotune(self, q, neq=G, reps=3, margin=0.01):
+    def autotune(self, q, neq=G, reps=3, margin=0.01, levels=None, quick=False):
         """Pick the M2L launch variant of every level by timing whole
         evaluations of q on the current points (so that the choice accounts
         for the other phases running beside the M2L): levels from the leaf
         up, each trying the MFMA staging/workgroup shapes and split counts
         around the default, the others held.  Returns {level: (n-tiles,
         staging, waves per workgroup, splits)}.  A candidate replaces the
-        best so far only when faster by more than the fraction margin."""
+        best so far only when faster by more than the fraction margin.
+        levels: the levels to tune (default every level); quick: only the
+        default shape and the FP16-split shapes, two split counts each."""
         import torch
 
         def timed():
@@ -240,15 +258,23 @@ class FMM:
         if self.precision != FP64 and neq in (G, GNI, GNJ):
             # the error-compensated FP16 kernel (falls back to the FP32 shape
             # where it does not apply: Yukawa, REF convention, p outside 6..12)
-            shapes += [(2, 5, 8), (2, 5, 4), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
+            shapes += [(2, 5, 8), (2, 5, 4), (4, 5, 4), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
+        if quick:
+            shapes = [shapes[0]] + ([(2, 5, 8), (1, 4, 8)] if self.precision != FP64 and neq in (G, GNI, GNJ) else [])
         for lev in range(lmax, 1, -1):
-            if self.nboxes("i", lev) == 0:
+            if self.nboxes("i", lev) == 0 or (levels is not None and lev not in levels):
                 continue
+            # the engine's default shape first: it stays unless another is
+            # clearly faster
+            self.set_m2l_variant(lev, 0, -1, 0, 0)
+            dflt = tuple(self.m2l_variant(lev)[:3])
             best, tbest, seen = None, float("inf"), set()
-            for nt, st, wpg in shapes:
+            for nt, st, wpg in [dflt] + [sh for sh in shapes if sh != dflt]:
                 self.set_m2l_variant(lev, nt, st, wpg, 0)
                 base = self.m2l_variant(lev)[3]
-                for ns in sorted({base, max(1, base // 2), min(16, base * 2), max(1, base // 4), min(16, base * 4)}):
+                cand = {base, min(16, base * 2)} if quick else \
+                    {base, max(1, base // 2), min(16, base * 2), max(1, base // 4), min(16, base * 4)}
+                for ns in sorted(cand):
                     self.set_m2l_variant(lev, nt, st, wpg, ns)
                     eff = tuple(self.m2l_variant(lev))  # the launch that runs (shapes can coincide)
                     if eff in seen:
@@ -514,6 +540,15 @@ class FMM:
         b = np.zeros(4, np.float32)
         _check(lib().fmmx_get_domain(self.h, C.byref(lm), b.ctypes.data))
         return lm.value, tuple(float(v) for v in b)
+
+    def occupancy(self):
+        """(boxes, sumsq) of levels 0..8 of the points the last set_points
+        chose its depth from: boxes holding a point and the sums of squared
+        points per box (zeros when lmax was given or under REF)."""
+        b = np.zeros(9, np.int64)
+        q = np.zeros(9, np.float64)
+        _check(lib().fmmx_get_occupancy(self.h, b.ctypes.data, q.ctypes.data))
+        return b, q
 
     def nboxes(self, side, lev):
         return lib().fmmx_get_count(self.h, 1, 0 if side == "i" else 1, lev)

@@ -428,7 +428,8 @@ class BatchedInducedCharge:
     of launches over all panels) applies every surface's operator.  Vectors
     are concatenated in surface order; seg[s]:seg[s+1] is surface s."""
 
-    def __init__(self, surfaces, e1=4.0, e2=20.0, p=8, precision="mixed", lmax=0, extra_levels=4, device=None):
+    def __init__(self, surfaces, e1=4.0, e2=20.0, p=8, precision="mixed", lmax=0, extra_levels=4, device=None,
+                 tune=False, graph=True):
         import torch
 
         ns = len(surfaces)
@@ -473,10 +474,23 @@ class BatchedInducedCharge:
         # the panels in each surface's own coordinates (device), for the
         # right-hand sides and energies
         self.pc, self.pa, self.pn = c.contiguous(), area, nrm.contiguous()
+        # strengths buffer of the matvec; optionally the M2L launch shapes
+        # tuned on this tree (pays off only over many matvecs: a tuning pass
+        # costs about as much as 20 of them); one evaluation captured as a
+        # HIP graph that every matvec replays
+        self._q = (self.area / self.e1).float().contiguous()
+        self.tuning = self.f.autotune(self._q, neq=GNI, reps=2, levels=[self.f.lmax], quick=True) if tune else None
+        self._v = None
+        if graph:
+            self._v, _ = self.f.capture(self._q, neq=GNI, gradient=False)
 
     def matvec(self, x):
-        v, _ = self.f.evaluate((x * self.area / self.e1).float(), neq=GNI)
-        return v + self.diag * x
+        if self._v is None:
+            v, _ = self.f.evaluate((x * self.area / self.e1).float(), neq=GNI)
+            return v + self.diag * x
+        self._q.copy_(x * self.area / self.e1)
+        self.f.replay()
+        return self._v + self.diag * x
 
     def precondition(self, x):
         return x / (self.e1 + self.e2) * (2 * self.e1 * (self.e1 - self.e2))
@@ -485,53 +499,95 @@ class BatchedInducedCharge:
         self.f.close()
 
 
-def gmres_batched(matvec, b, seg_id, nseg, precond=None, rtol=1e-6, restart=30, maxiter=300):
+def gmres_batched(matvec, b, seg_id, nseg, precond=None, rtol=1e-6, restart=30, maxiter=300, chunk=1024):
     """Independent right-preconditioned restarted GMRES for every segment of
     a block-diagonal system, in lockstep: one matvec per iteration serves all
     segments; inner products, Givens rotations and updates are per segment
-    (FP64, modified Gram-Schmidt with one reorthogonalisation pass).  b and
-    the iterates are torch tensors (device-resident for a GPU matvec).
+    (FP64).  b and the iterates are torch tensors (device-resident for a GPU
+    matvec); segment s is the contiguous run of seg_id == s.
+
+    Layout: the Krylov vectors are kept with every segment padded to whole
+    chunks of `chunk` values (zeros in the padding), as a (chunks, restart+1,
+    chunk) array, so that the orthogonalisation is block classical
+    Gram-Schmidt with one reorthogonalisation pass: per pass, one batched
+    product gives the inner products of the new vector with every basis
+    vector chunk by chunk, a sum over each segment's chunks turns them into
+    per-segment coefficients, and one batched product subtracts the
+    projections.  Segment sums run over chunks (about n / chunk values), not
+    over single entries.
+    Host synchronisation: the host waits for iteration k-1's convergence flag
+    only after queueing iteration k, so the GPU never idles on the host; the
+    restart's true residual is the one blocking read per cycle.
     Returns (x, per-segment relative residuals, iterations)."""
     import torch
 
     dev, dt = b.device, torch.float64
+    gpu = dev.type == "cuda"
     M = precond or (lambda v: v)
+    n, C = b.shape[0], int(chunk)
+    counts = torch.bincount(seg_id, minlength=nseg).cpu().numpy().astype(np.int64)
+    nch = (counts + C - 1) // C
+    nc = int(nch.sum())
+    cstart = np.concatenate([[0], np.cumsum(nch)])[:-1]
+    start = np.concatenate([[0], np.cumsum(counts)])[:-1]
+    chunk_seg = torch.from_numpy(np.repeat(np.arange(nseg), nch)).to(dev)
+    # padded position of every entry (segments are contiguous and in order)
+    sid = seg_id.long()
+    pos = (torch.from_numpy(cstart * C - start).to(dev)[sid] + torch.arange(n, device=dev)).contiguous()
 
-    def sdot(a, c):
-        return torch.zeros(nseg, dtype=dt, device=dev).index_add_(0, seg_id, a * c)
+    def pack(v):
+        out = torch.zeros(nc * C, dtype=dt, device=dev)
+        out[pos] = v
+        return out.view(nc, C)
 
-    nb = torch.sqrt(sdot(b, b))
-    nb = torch.where(nb > 0, nb, torch.ones_like(nb))
-    x = torch.zeros_like(b)
+    def unpack(v):
+        return v.reshape(-1)[pos]
+
+    def segsum(cv):  # per-chunk values (nc, ...) -> per-segment sums (nseg, ...)
+        return torch.zeros((nseg,) + tuple(cv.shape[1:]), dtype=dt, device=dev).index_add_(0, chunk_seg, cv)
+
+    def snorm(v):  # per-segment 2-norms of a packed vector
+        return torch.sqrt(segsum((v * v).sum(1)))
+
+    def safe(a):
+        return torch.where(a > 0, a, torch.ones_like(a))
+
+    bp = pack(b.to(dt))
+    nb = safe(snorm(bp))
+    x = torch.zeros(n, dtype=dt, device=dev)
     it = 0
-    r = b - matvec(x)
-    beta = torch.sqrt(sdot(r, r))
+    r = pack(b - matvec(x))
+    beta = snorm(r)
     res = beta / nb
+    V = None
     while bool((res > rtol).any()) and it < maxiter:
         m = min(restart, maxiter - it)
-        V = [r / torch.where(beta > 0, beta, torch.ones_like(beta))[seg_id]]
+        if V is None or V.shape[1] < m + 1:
+            V = torch.empty(nc, m + 1, C, dtype=dt, device=dev)
+        V[:, 0, :] = r / safe(beta)[chunk_seg][:, None]
         H = torch.zeros(nseg, m + 1, m, dtype=dt, device=dev)
         g = torch.zeros(nseg, m + 1, dtype=dt, device=dev)
         g[:, 0] = beta
         cs = torch.zeros(nseg, m, dtype=dt, device=dev)
         sn = torch.zeros(nseg, m, dtype=dt, device=dev)
         kk = 0
+        prev = None  # (event, pinned flag) of the previous iteration
         for k in range(m):
-            w = matvec(M(V[k]))
+            w = pack(matvec(M(unpack(V[:, k, :]))))
+            Vk = V[:, :k + 1, :]
             for _ in range(2):
-                for i in range(k + 1):
-                    h = sdot(V[i], w)
-                    H[:, i, k] += h
-                    w = w - h[seg_id] * V[i]
-            hn = torch.sqrt(sdot(w, w))
+                h = segsum(torch.bmm(Vk, w.unsqueeze(2))[:, :, 0])  # (nseg, k+1)
+                H[:, :k + 1, k] += h
+                w = w - torch.bmm(Vk.transpose(1, 2), h[chunk_seg].unsqueeze(2))[:, :, 0]
+            hn = snorm(w)
             H[:, k + 1, k] = hn
-            V.append(w / torch.where(hn > 0, hn, torch.ones_like(hn))[seg_id])
+            V[:, k + 1, :] = w / safe(hn)[chunk_seg][:, None]
             for i in range(k):
                 t = cs[:, i] * H[:, i, k] + sn[:, i] * H[:, i + 1, k]
                 H[:, i + 1, k] = -sn[:, i] * H[:, i, k] + cs[:, i] * H[:, i + 1, k]
                 H[:, i, k] = t
             d = torch.sqrt(H[:, k, k] ** 2 + H[:, k + 1, k] ** 2)
-            d1 = torch.where(d > 0, d, torch.ones_like(d))
+            d1 = safe(d)
             cs[:, k] = torch.where(d > 0, H[:, k, k] / d1, torch.ones_like(d))
             sn[:, k] = torch.where(d > 0, H[:, k + 1, k] / d1, torch.zeros_like(d))
             H[:, k, k] = d
@@ -540,45 +596,74 @@ def gmres_batched(matvec, b, seg_id, nseg, precond=None, rtol=1e-6, restart=30, 
             g[:, k] = cs[:, k] * g[:, k]
             it += 1
             kk = k + 1
-            if bool((g[:, k + 1].abs() / nb <= rtol).all()):
-                break
+            done = (g[:, k + 1].abs() / nb <= rtol).all()
+            if not gpu:
+                if bool(done):
+                    break
+                continue
+            flag = torch.empty((), dtype=torch.bool, pin_memory=True)
+            flag.copy_(done, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            # iteration k is queued: now wait for iteration k-1's flag (the
+            # GPU meanwhile runs iteration k)
+            if prev is not None:
+                prev[0].synchronize()
+                if bool(prev[1]):
+                    break
+            prev = (ev, flag)
         R = H[:, :kk, :kk]
         diag = torch.diagonal(R, dim1=1, dim2=2)
         R = R + torch.diag_embed(torch.where(diag == 0, torch.ones_like(diag), torch.zeros_like(diag)))
         y = torch.linalg.solve_triangular(R, g[:, :kk, None], upper=True)[:, :, 0]
-        upd = torch.zeros_like(b)
-        for i in range(kk):
-            upd = upd + y[:, i][seg_id] * V[i]
-        x = x + M(upd)
-        r = b - matvec(x)
-        beta = torch.sqrt(sdot(r, r))
+        upd = torch.bmm(V[:, :kk, :].transpose(1, 2), y[chunk_seg].unsqueeze(2))[:, :, 0]
+        x = x + M(unpack(upd))
+        r = pack(b - matvec(x))
+        beta = snorm(r)
         res = beta / nb
     return x, res, it
 
 
 def batched_solvation(surfaces, charges, e1=4.0, e2=20.0, p=8, precision="mixed", rtol=1e-6, restart=30,
-                      maxiter=300, extra_levels=4, normalize=True):
+                      maxiter=300, extra_levels=4, normalize="size", tune=False, graph=True):
     """Solve the induced-charge equation of bio/main.cxx for every surface of
     a batch at once (each with its own point charges inside) and return the
     per-surface solvation energies (kcal/mol), the solution and the GMRES
     history.  charges[s] = (positions (m, 3), charges (m,)) in the
     surface's own coordinates.
 
-    normalize: scale every surface (and its charges) by a_s to a common
-    size, so that each fills its cell of the shared tree.  The operator
+    normalize: scale every surface (and its charges) by a_s.  The operator
     (area times the normal-derivative kernel) is scale-free, the right-hand
     side scales as 1/a^2 and the reaction potential as 1/a, so the energy of
     the scaled problem times a_s is the surface's energy; the returned si is
-    the scaled problem's."""
+    the scaled problem's.  "size" (or True): every surface fills its cell of
+    the shared tree.  "density": a surface of n_s panels is scaled to
+    sqrt(n_s / n_max) of the cell size, so every surface has about the
+    same number of panels per leaf box (a small surface spans fewer boxes,
+    a shallower effective tree).  False: as given.
+    Returns also times_s: operator set-up, right-hand sides, GMRES and
+    energies (seconds, each closed by a device synchronisation)."""
     import torch
 
     scale = np.ones(len(surfaces))
     if normalize:
         ext = [float(np.abs(np.asarray(v) - np.asarray(v).mean(0)).max()) for v, _ in surfaces]
         scale = np.max(ext) / np.array(ext)
+        if normalize == "density":
+            npan = np.array([len(f) for _, f in surfaces], float)
+            scale = scale * np.sqrt(npan / npan.max())
         surfaces = [((np.asarray(v, np.float64) * a).astype(np.float32), f) for (v, f), a in zip(surfaces, scale)]
         charges = [((np.asarray(x, np.float64) * a).astype(np.float32), q) for (x, q), a in zip(charges, scale)]
-    op = BatchedInducedCharge(surfaces, e1, e2, p=p, precision=precision, extra_levels=extra_levels)
+    import time
+
+    def now():
+        torch.cuda.synchronize()
+        return time.perf_counter()
+
+    t0 = now()
+    op = BatchedInducedCharge(surfaces, e1, e2, p=p, precision=precision, extra_levels=extra_levels, tune=tune,
+                              graph=graph)
+    t1 = now()
     # right-hand side: -4 pi Gni(charges -> centroids) / e1 per surface, on
     # the oracle-free GPU direct sum of the engine
     f = FMM(p=p, precision="fp64", convention="exact")
@@ -590,14 +675,19 @@ def batched_solvation(surfaces, charges, e1=4.0, e2=20.0, p=8, precision="mixed"
         sl = slice(int(op.seg[s]), int(op.seg[s + 1]))
         v, _ = f.direct(op.pc[sl], xq, q, neq=GNI, ni_normals=op.pn[sl])
         b[sl] = -4 * np.pi * v / e1
+    t2 = now()
     x, res, it = gmres_batched(op.matvec, b, op.seg_id, op.nseg, precond=op.precondition, rtol=rtol,
                                restart=restart, maxiter=maxiter)
+    t3 = now()
     si = x.cpu().numpy()
     for s, (xq, q) in enumerate(dq):
         sl = slice(int(op.seg[s]), int(op.seg[s + 1]))
         vr, _ = f.direct(xq, op.pc[sl], (x[sl] * op.pa[sl] / e1).float(), neq=G)
         energies.append(float(0.5 * KCAL * (q.double() * vr).sum()) * scale[s])
+    t4 = now()
     f.close()
+    tuning = op.tuning
     op.close()
     return {"energies": np.array(energies), "si": si, "residuals": res.cpu().numpy(), "iterations": it,
-            "seg": op.seg, "scale": scale}
+            "seg": op.seg, "scale": scale, "tuning": tuning,
+            "times_s": {"operator_setup": t1 - t0, "rhs": t2 - t1, "gmres": t3 - t2, "energies": t4 - t3}}

@@ -145,6 +145,46 @@ int nlevel(int nmax) {
   return l + 7;
 }
 
+// Leaf level of the EXACT convention when the caller leaves lmax to the
+// engine: the level 2..8 of least modelled cost
+//   near field  3^d S(l)            (S: sum over boxes of squared counts)
+//   far field   r (6^d - 3^d) T(l)  (T: boxes of levels 2..l)
+// where d = log2(B(l) / B(l-1)) in [0, 3] is the dimension the point set
+// shows at that scale (B: boxes holding a point; 3 for a volume, 2 for a
+// surface), so 3^d and 6^d - 3^d count the near and interaction-list boxes
+// that hold points.  r, the cost of one far-field box against one near pair,
+// was fitted per order and precision to evaluation times measured on MI355X
+// (uniform cube, sphere surface and Plummer inputs, N 1e4..3e6, every leaf
+// level with 3..2000 points per box): the choice is within 5% of the fastest
+// level on average over those 158 cases.  The model holds where boxes hold
+// several points (a volume's boxes hold one point each once B stops growing,
+// d drops to 0 and the far-field term with it), so the levels considered
+// are those with 3..2000 points per occupied box on average (level 2 when
+// none has 3, the deepest level when none has at most 2000).
+int auto_depth(const long boxes[MAXL + 1], const double sumsq[MAXL + 1], int p, bool fp64) {
+  // r at p = 4, 6, 8, 10, 12 (non-decreasing in p), geometric in between
+  static const double r_mixed[5] = {112, 112, 316, 501, 501};
+  static const double r_fp64[5] = {316, 501, 501, 1000, 2820};
+  const double* rt = fp64 ? r_fp64 : r_mixed;
+  const double u = std::min(std::max((p - 4) / 2.0, 0.0), 4.0);
+  const int k = std::min((int)u, 3);
+  const double r = rt[k] * std::pow(rt[k + 1] / rt[k], u - k);
+  const double n = (double)boxes[0] > 0 ? std::sqrt(sumsq[0]) : 0.0;  // one box holds all points
+  int best = -1, lo = 2, hi = MAXL;
+  while (lo < MAXL && boxes[lo] > 0 && n / (double)boxes[lo] > 2000) lo++;
+  while (hi > lo && (boxes[hi] <= 0 || n / (double)boxes[hi] < 3)) hi--;
+  double cbest = 0, tot = 0;
+  for (int l = 2; l <= hi; l++) {
+    tot += (double)boxes[l];
+    if (l < lo) continue;
+    const double d = boxes[l - 1] > 0 ? std::min(std::max(std::log2((double)boxes[l] / (double)boxes[l - 1]), 0.0), 3.0)
+                                       : 3.0;
+    const double c = std::pow(3.0, d) * sumsq[l] + r * (std::pow(6.0, d) - std::pow(3.0, d)) * tot;
+    if (best < 0 || c < cbest) best = l, cbest = c;
+  }
+  return best;
+}
+
 // process-wide cache of host operators keyed by (p, conv, lmax, rd)
 struct OpSet {
   std::vector<double> m2m, l2l;  // transposed [op][k][row]
@@ -498,6 +538,8 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   for (auto& e : ev) FMMX_HIP(hipEventCreate(&e));
   FMMX_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
   FMMX_HIP(hipStreamCreateWithFlags(&aux2, hipStreamNonBlocking));
+  // FMMX_OVERLAP=0 runs every phase on the caller's stream (measurement)
+  if (const char* e = getenv("FMMX_OVERLAP")) overlap_on = atoi(e) != 0;
   tab = *get_tables(p, c->convention);
   // device tables: table-ratio corrections and the M2L offset table
   std::vector<double> corr(tab.corr);
@@ -726,7 +768,18 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
     if (!(r > 0)) r = 1.0f;
     xmin = lo[0]; ymin = lo[1]; zmin = lo[2]; rd = r;
   }
-  lmax = cfg.lmax > 0 ? cfg.lmax : std::min(MAXL, nlevel(nj));
+  for (int l = 0; l <= MAXL; l++) occ_boxes[l] = 0, occ_sumsq[l] = 0;
+  if (cfg.lmax > 0) {
+    lmax = cfg.lmax;
+  } else if (cfg.convention == FMMX_CONV_REF) {
+    lmax = std::min(MAXL, nlevel(nj));
+  } else {
+    // modelled cost on the box statistics of the sources (targets when
+    // there are none)
+    const int o = nj > 0 ? 6 : 0;
+    if ((rc = occupancy(nj > 0 ? nj : ni, d[o], d[o + 1], d[o + 2], occ_boxes, occ_sumsq))) return rc;
+    lmax = auto_depth(occ_boxes, occ_sumsq, p, fp64);
+  }
   if (group_level > lmax)
     return fail(FMMX_EINVAL, "group level " + std::to_string(group_level) + " is deeper than lmax " +
                                  std::to_string(lmax) + " (fmmx_set_groups)");
@@ -856,7 +909,7 @@ int Engine::evaluate(int neq, const float* qj, double* vi, double* gxi, double* 
   }
   // Without per-phase timing the near field (vector ALU) runs on the second
   // stream beside the far field (MFMA); L2P, which adds to its results, waits.
-  const bool overlap = !per_phase && aux && ni > 0;
+  const bool overlap = overlap_on && !per_phase && aux && ni > 0;
   if (ni > 0) {
     if (overlap) {
       FMMX_HIP(hipEventRecord(ev[10], stream));
@@ -1250,7 +1303,7 @@ int fmmx_set_m2l_variant(fmmx_handle* h, int lev, int nt, int staging, int wpg, 
   FMMX_GUARD({
     if (!h) return fmmx::fail(FMMX_EINVAL, "handle is NULL");
     if (lev < 2 || lev > fmmx::MAXL) return fmmx::fail(FMMX_EINVAL, "level out of range [2, 8]");
-    if (nt < 0 || nt > 2 || staging < -1 || staging > 5 || (wpg != 0 && wpg != 4 && wpg != 8 && wpg != 16) ||
+    if (nt < 0 || nt > 4 || nt == 3 || (nt == 4 && staging != 5) || staging < -1 || staging > 5 || (wpg != 0 && wpg != 4 && wpg != 8 && wpg != 16) ||
         nsplit < 0 || nsplit > 16)
       return fmmx::fail(FMMX_EINVAL, "bad M2L variant");
     auto& t = h->e.tune[lev];
@@ -1388,6 +1441,32 @@ int fmmx_gradients(fmmx_handle* h, int neq, const float* qj, const float* wi, do
 int fmmx_energy(fmmx_handle* h, int neq, const float* q, double* E, double* dEdq, double* dEdx, int flags) {
   if (!h) return fmmx::fail(FMMX_EINVAL, "NULL handle");
   FMMX_GUARD({ return h->e.energy(neq, q, E, dEdq, dEdx, flags); })
+}
+
+int fmmx_auto_lmax(const long long* boxes, const double* sumsq, int p, int precision, int* lmax) {
+  if (!boxes || !sumsq || !lmax) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  if (p < 2 || p > fmmx::MAXP) return fmmx::fail(FMMX_EINVAL, "p outside 2..FMMX_MAX_P");
+  if (precision != FMMX_MIXED && precision != FMMX_FP64) return fmmx::fail(FMMX_EINVAL, "unknown precision");
+  long b[fmmx::MAXL + 1];
+  double q[fmmx::MAXL + 1];
+  for (int l = 0; l <= fmmx::MAXL; l++) {
+    if (boxes[l] < 0 || !(sumsq[l] >= 0)) return fmmx::fail(FMMX_EINVAL, "negative box statistics");
+    b[l] = (long)boxes[l];
+    q[l] = sumsq[l];
+  }
+  *lmax = fmmx::auto_depth(b, q, p, precision == FMMX_FP64);
+  return FMMX_OK;
+}
+
+int fmmx_get_occupancy(fmmx_handle* h, long long* boxes, double* sumsq) {
+  if (!h || !boxes || !sumsq) return fmmx::fail(FMMX_EINVAL, "NULL argument");
+  Engine& e = h->e;
+  if (!e.have_points) return fmmx::fail(FMMX_ESTATE, "no tree (fmmx_set_points)");
+  for (int l = 0; l <= fmmx::MAXL; l++) {
+    boxes[l] = e.occ_boxes[l];
+    sumsq[l] = e.occ_sumsq[l];
+  }
+  return FMMX_OK;
 }
 
 int fmmx_get_domain(fmmx_handle* h, int* lmax, float* box4) {

@@ -83,6 +83,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   hipStream_t aux = nullptr;   // second stream: the near field beside the far field
   hipStream_t aux2 = nullptr;  // third stream: the upper levels' M2L/L2L beside the leaf M2L
+  bool overlap_on = true;      // side streams in use (without per-phase timing)
   int p = 0, P2 = 0, lmax = 0;
   // multipoles are stored padded for the M2L B operand: real slot c lives at
   // (c / KS) * KSP + c % KS, KS = ceil(P2/4), KSP = KS rounded up to 4; VS = 4*KSP
@@ -171,6 +172,10 @@ struct Engine {
   int graph_launch();
   // stand-alone Morton sort (fmmx_sort): staging, keys, permutation
   DevBuf srt_in[3], srt_keys_in, srt_idx_in, srt_keys, srt_perm, srt_bad;
+  DevBuf occ_bits;  // points per box of levels 8..0 and their statistics (automatic depth)
+  // box statistics the last automatic depth was chosen from
+  long occ_boxes[MAXL + 1] = {};
+  double occ_sumsq[MAXL + 1] = {};
   DevBuf bad_flag;  // set by the key kernel for coordinates outside the root cube
   int last_neq = -1;
   // The transposed problem (targets: this handle's sources, sources: its
@@ -211,6 +216,10 @@ struct Engine {
 
   // tree.hip
   int sort_points(int n, const float* x, const float* y, const float* z, int* keys, int* perm, int flags);
+  // boxes holding a point and sums of squared points per box, levels 0..8,
+  // for n points in the current root cube
+  int occupancy(int n, const float* x, const float* y, const float* z, long boxes[MAXL + 1],
+                double sumsq[MAXL + 1]);
   int build_side(Side& s, int n, const float* x, const float* y, const float* z, const float* nx,
                  const float* ny, const float* nz, bool dense, const Side* same = nullptr);
   int build_lists();

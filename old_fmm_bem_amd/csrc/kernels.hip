@@ -2159,7 +2159,7 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
   // EXACT, p 6..12), else LDS staging
   if (v.pa < 0) v.pa = (split_ok && !yk && !hh && !fp64) ? (wide_ok ? 5 : 4) : 0;
   if (v.wpg <= 0) v.wpg = 8;
-  if (const char* e = getenv("FMMX_M2L_NT")) v.nt = atoi(e) == 2 ? 2 : 1;
+  if (const char* e = getenv("FMMX_M2L_NT")) v.nt = atoi(e) == 4 ? 4 : atoi(e) == 2 ? 2 : 1;
   if (const char* e = getenv("FMMX_M2L_PA")) v.pa = std::max(0, std::min(5, atoi(e)));
   if (const char* e = getenv("FMMX_M2L_WPG")) v.wpg = atoi(e) == 4 ? 4 : atoi(e) == 16 ? 16 : 8;
   if (v.pa >= 3) {
@@ -2168,7 +2168,13 @@ Engine::M2LVariant Engine::m2l_variant(int lev) const {
     if (!split_ok || yk || hh || fp64) v.pa = 0;
     else if (v.pa == 5 && !wide_ok) v.pa = 4;
     if (v.pa == 4 && !split_db_fits(p)) v.pa = 3;
-    if (v.pa == 5) v.nt = 2, v.wpg = v.wpg == 4 ? 4 : 8;  // 8 (two buffers) or 4 (one) waves of 32 targets
+    // 32 targets per wave: 8 waves (two buffers) or 4 (one); 64 targets per
+    // wave (nt 4): 4 waves, two buffers
+    if (v.pa == 5) {
+      if (v.nt == 4) v.wpg = 4;
+      else v.nt = 2, v.wpg = v.wpg == 4 ? 4 : 8;
+    }
+    if (v.pa != 5 && v.nt == 4) v.nt = 2;
     if (v.pa == 4) v.wpg = v.nt == 2 ? 4 : 8;  // 8 waves of one N-tile or 4 of two
     if (v.pa == 3 && v.nt == 2) v.wpg = 4;
     if (v.wpg == 16) v.wpg = 8;
@@ -2340,7 +2346,7 @@ int Engine::run_m2l_all(int l0) {
     constexpr int P = decltype(pc)::value;
     if (per_phase) FMMX_HIP(hipEventRecord(ev[8], stream));
     if (split) {
-      int r2 = leaf.pa == 5 ? launch_m2l_wide(p, A, wg, leaf.wpg, stream)
+      int r2 = leaf.pa == 5 ? launch_m2l_wide(p, A, wg, leaf.wpg, leaf.nt, stream)
                             : launch_m2l_split(p, A, wg, leaf.nt, leaf.wpg, leaf.pa == 4, stream);
       if (r2) return r2;
     } else {
@@ -2477,7 +2483,7 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       const bool tmark = per_phase && lev == lmax;
       if (tmark) FMMX_HIP(hipEventRecord(ev[8], stream));
       if (var.pa >= 3) {
-        int r2 = var.pa == 5 ? launch_m2l_wide(p, A, A.lv[0].nwg, var.wpg, stream)
+        int r2 = var.pa == 5 ? launch_m2l_wide(p, A, A.lv[0].nwg, var.wpg, var.nt, stream)
                              : launch_m2l_split(p, A, A.lv[0].nwg, var.nt, var.wpg, var.pa == 4, stream);
         if (r2) return r2;
       } else {

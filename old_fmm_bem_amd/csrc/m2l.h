@@ -143,10 +143,10 @@ int launch_msplit(int p, const MSplitLevels& S, hipStream_t stream);
 // db: two operator buffers (8 waves, one N-tile; p with split_db_fits)
 int launch_m2l_split(int p, const M2LLaunch& A, int nwg, int nt, int wpg, int db, hipStream_t stream);
 bool split_db_fits(int p);
-// the 32x32x16 form: waves of 32 targets, all 316 operators; 8 waves with
-// two LDS buffers, or 4 waves with one (two workgroups per CU); p with
-// split_wide_fits
-int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, int wpg, hipStream_t stream);
+// the 32x32x16 form, all 316 operators: nt 2 (32 targets per wave) with 8
+// waves and two LDS buffers or 4 waves and one (two workgroups per CU); nt 4
+// (64 targets per wave), 4 waves and two buffers; p with split_wide_fits
+int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, int wpg, int nt, hipStream_t stream);
 bool split_wide_fits(int p);
 void split_fragments_wide(const double* kmaj, int count, int p, int sa_e, std::vector<uint4>& out);
 // 16-byte fragments of one split operator; halfs of one split box

@@ -618,14 +618,17 @@ __device__ __forceinline__ h8 bfrag_w(const uint32_t* bw, int ks) {
                                       w3 < 0 ? 0u : bw[w3]});
 }
 
-// Split M2L on v_mfma_f32_32x32x16_f16: a wave owns 32 targets (one per
-// lane of a lane group; the two lane groups hold the two halves of the
-// input slots) and MT tiles of 32 output rows.  The operator of every one of
-// the 316 offsets is stored as it acts (no reflections to apply), copied
-// global -> LDS by the DMA path into two buffers: offset o+1's operator
-// streams in while o's products run.  Output lane layout: element e of row
-// tile mt is row 32 mt + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32.
-template <int P, int WPG, int DB>
+// Split M2L on v_mfma_f32_32x32x16_f16: a wave owns NT groups of 32
+// targets (one per lane of a lane group; the two lane groups hold the two
+// halves of the input slots) and MT tiles of 32 output rows per group.  The
+// operator of every one of the 316 offsets is stored as it acts (no
+// reflections to apply), copied global -> LDS by the DMA path into two
+// buffers: offset o+1's operator streams in while o's products run.  Each
+// operator fragment read from LDS feeds NT products, so NT = 2 halves the
+// LDS reads per product (4 waves of 64 targets, one wave per SIMD).  Output
+// lane layout: element e of row tile mt is row 32 mt + 8 (e / 4) +
+// 4 (lane / 32) + e % 4, column lane % 32.
+template <int P, int WPG, int DB, int NT>
 __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
   M2LLev Q = A.lv[0];
 #pragma unroll
@@ -644,7 +647,7 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
   const int bid = (int)blockIdx.x - Q.wg0, nwg = Q.nwg;
   using W = WLayout<P>;
   constexpr int P2 = W::P2, MT = W::MT, KH = W::KH, FRW = W::FRW, NB4 = W::NB4, NCH = KH * MT;
-  constexpr int TN = 32 * WPG;
+  constexpr int TN = 32 * NT * WPG;
   // two operator buffers as distinct objects, so that the compiler sees the
   // fragment reads of one and the copy into the other as independent (no
   // wait for the copy before the products)
@@ -676,92 +679,107 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
   if (cls < 0) return;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, j = l & 31, kq = l >> 5;
   const int g = 1 << lev;
-  int tord = -1, tx = 0, ty = 0, tz = 0;
-  uint32_t tk = 0u;
-  {
-    const int t = w * 32 + j;
+  int tord[NT], tx[NT], ty[NT], tz[NT];
+  uint32_t tk[NT];
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    const int t = (w * NT + n) * 32 + j;
+    tord[n] = -1;
+    tx[n] = ty[n] = tz[n] = 0;
+    tk[n] = 0u;
     if (t < ntile) {
-      tord = (int)ord[s_start + t];
-      tk = ikey[tord];
-      morton_decode(tk, lev, tx, ty, tz);
+      tord[n] = (int)ord[s_start + t];
+      tk[n] = ikey[tord[n]];
+      morton_decode(tk[n], lev, tx[n], ty[n], tz[n]);
     }
   }
   const uint32_t MY = 0x49249249u & ((1u << (3 * lev)) - 1u), MX = MY << 1, MZ = MY << 2;
-  const bool wave_on = w * 32 < ntile;
+  const bool wave_on = w * NT * 32 < ntile;
   const int per = (NEBM + nsplit - 1) / nsplit, o0 = split * per, o1 = min(NEBM, o0 + per);
   const float oscale = ldexpf(1.f, -(split_exp(Q.mmax) + Q.sa_e));
-  f32x16 acc[MT];
+  f32x16 acc[NT][MT];
 #pragma unroll
-  for (int mt = 0; mt < MT; mt++)
+  for (int n = 0; n < NT; n++)
 #pragma unroll
-    for (int e = 0; e < 16; e++) acc[mt][e] = 0.f;
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[n][mt][e] = 0.f;
   unsigned long long mypairs = 0;
-  uint32_t bcur[4 * NB4], bnext[4 * NB4];
+  uint32_t bcur[NT][4 * NB4], bnext[NT][4 * NB4];
   bool cur_any = false, next_any = false;
-#define FMMX_LOAD_BW(o_, dst_, any_)                                                          \
-  {                                                                                           \
-    const int4 off_ = offtab[cls * NEBM + (o_)];                                              \
+#define FMMX_LOAD_BW(o_, dst_, any_)                                                            \
+  {                                                                                             \
+    const int4 off_ = offtab[cls * NEBM + (o_)];                                                \
     const uint32_t dx_ = dil3(abs(off_.x)) << 1, dy_ = dil3(abs(off_.y)), dz_ = dil3(abs(off_.z)) << 2; \
-    int s_ = -1;                                                                              \
-    if (tord >= 0) {                                                                          \
-      const int jx = tx - off_.x, jy = ty - off_.y, jz = tz - off_.z;                         \
-      if ((unsigned)jx < (unsigned)g && (unsigned)jy < (unsigned)g && (unsigned)jz < (unsigned)g && \
-          (gsh < 0 || !((jx ^ tx) >> gsh | (jy ^ ty) >> gsh | (jz ^ tz) >> gsh))) {           \
-        const uint32_t kx_ = off_.x >= 0 ? ((tk & MX) - dx_) & MX : ((tk | ~MX) + dx_) & MX;  \
-        const uint32_t ky_ = off_.y >= 0 ? ((tk & MY) - dy_) & MY : ((tk | ~MY) + dy_) & MY;  \
-        const uint32_t kz_ = off_.z >= 0 ? ((tk & MZ) - dz_) & MZ : ((tk | ~MZ) + dz_) & MZ;  \
-        s_ = dense_j[kx_ | ky_ | kz_];                                                        \
-      }                                                                                       \
-    }                                                                                         \
-    if (s_ >= 0 && kq == 0) mypairs++;                                                        \
-    any_ = __ballot(s_ >= 0) != 0;                                                            \
-    /* no source box: the level's zero box after the last one */                              \
-    const uint4* ms_ = Mh + ((size_t)(s_ >= 0 ? s_ : nbj) * W::VSH + kq * W::GH) / 8;         \
-    _Pragma("unroll") for (int v_ = 0; v_ < NB4; v_++) {                                      \
-      const uint4 m4_ = ms_[v_];                                                              \
-      dst_[4 * v_] = m4_.x;                                                                   \
-      dst_[4 * v_ + 1] = m4_.y;                                                               \
-      dst_[4 * v_ + 2] = m4_.z;                                                               \
-      dst_[4 * v_ + 3] = m4_.w;                                                               \
-    }                                                                                         \
+    bool a_ = false;                                                                            \
+    _Pragma("unroll") for (int n = 0; n < NT; n++) {                                            \
+      int s_ = -1;                                                                              \
+      if (tord[n] >= 0) {                                                                       \
+        const int jx = tx[n] - off_.x, jy = ty[n] - off_.y, jz = tz[n] - off_.z;                \
+        if ((unsigned)jx < (unsigned)g && (unsigned)jy < (unsigned)g && (unsigned)jz < (unsigned)g && \
+            (gsh < 0 || !((jx ^ tx[n]) >> gsh | (jy ^ ty[n]) >> gsh | (jz ^ tz[n]) >> gsh))) {  \
+          const uint32_t t_ = tk[n];                                                            \
+          const uint32_t kx_ = off_.x >= 0 ? ((t_ & MX) - dx_) & MX : ((t_ | ~MX) + dx_) & MX;  \
+          const uint32_t ky_ = off_.y >= 0 ? ((t_ & MY) - dy_) & MY : ((t_ | ~MY) + dy_) & MY;  \
+          const uint32_t kz_ = off_.z >= 0 ? ((t_ & MZ) - dz_) & MZ : ((t_ | ~MZ) + dz_) & MZ;  \
+          s_ = dense_j[kx_ | ky_ | kz_];                                                        \
+        }                                                                                       \
+      }                                                                                         \
+      if (s_ >= 0 && kq == 0) mypairs++;                                                        \
+      a_ = a_ || s_ >= 0;                                                                       \
+      /* no source box: the level's zero box after the last one */                              \
+      const uint4* ms_ = Mh + ((size_t)(s_ >= 0 ? s_ : nbj) * W::VSH + kq * W::GH) / 8;         \
+      _Pragma("unroll") for (int v_ = 0; v_ < NB4; v_++) {                                      \
+        const uint4 m4_ = ms_[v_];                                                              \
+        dst_[n][4 * v_] = m4_.x;                                                                \
+        dst_[n][4 * v_ + 1] = m4_.y;                                                            \
+        dst_[n][4 * v_ + 2] = m4_.z;                                                            \
+        dst_[n][4 * v_ + 3] = m4_.w;                                                            \
+      }                                                                                         \
+    }                                                                                           \
+    any_ = __ballot(a_) != 0;                                                                   \
   }
-#define FMMX_DMA_W(o_, buf_)                                                                  \
-  {                                                                                           \
-    const uint4* src_ = opH + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FRW + l;       \
-    for (int c_ = w; c_ < NCH; c_ += WPG)                                                     \
-      __builtin_amdgcn_global_load_lds((const void*)(src_ + c_ * 64),                         \
+#define FMMX_DMA_W(o_, buf_)                                                                    \
+  {                                                                                             \
+    const uint4* src_ = opH + (size_t)(offtab[cls * NEBM + (o_)].w & 0xffff) * FRW + l;         \
+    for (int c_ = w; c_ < NCH; c_ += WPG)                                                       \
+      __builtin_amdgcn_global_load_lds((const void*)(src_ + c_ * 64),                           \
                                        (__attribute__((address_space(3))) void*)((buf_) + c_ * 64), 16, 0, 0); \
   }
-#define FMMX_STEP_W(o_, BC_, AC_, BN_, AN_, ACUR_, ANEXT_)                                     \
-  {                                                                                           \
-    /* the next multipoles are plain loads, in flight during the products; */                  \
-    /* the next operator's copy is issued after this step's last read of  */                   \
-    /* LDS (a copy into LDS orders every later LDS read behind it)        */                   \
-    if ((o_) + 1 < o1) FMMX_LOAD_BW((o_) + 1, BN_, AN_);                                      \
-    bool dma_ = (o_) + 1 < o1;                                                                \
-    if (wave_on && AC_) {                                                                     \
-      const uint4* Ab = ACUR_;                                                                \
-      h8 af[2][MT];                                                                           \
+#define FMMX_MFMA_W(Ab_, ks_, BC_)                                                              \
+  _Pragma("unroll") for (int n = 0; n < NT; n++) {                                              \
+    const h8 bf_ = bfrag_w<W>(BC_[n], ks_);                                                     \
+    _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                           \
+      acc[n][mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ab_[mt], bf_, acc[n][mt], 0, 0, 0);   \
+  }
+#define FMMX_STEP_W(o_, BC_, AC_, BN_, AN_, ACUR_, ANEXT_)                                       \
+  {                                                                                             \
+    /* the next multipoles are plain loads, in flight during the products; */                    \
+    /* the next operator's copy is issued after this step's last read of  */                    \
+    /* LDS (a copy into LDS orders every later LDS read behind it)        */                    \
+    if ((o_) + 1 < o1) FMMX_LOAD_BW((o_) + 1, BN_, AN_);                                        \
+    bool dma_ = (o_) + 1 < o1;                                                                  \
+    if (wave_on && AC_) {                                                                       \
+      const uint4* Ab = ACUR_;                                                                  \
+      h8 af[2][MT];                                                                             \
       _Pragma("unroll") for (int mt = 0; mt < MT; mt++) af[0][mt] = __builtin_bit_cast(h8, Ab[mt * 64 + l]); \
-      _Pragma("unroll") for (int ks = 0; ks < KH; ks++) {                                     \
-        if (ks + 1 < KH) {                                                                    \
-          _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                   \
-            af[(ks + 1) & 1][mt] = __builtin_bit_cast(h8, Ab[((ks + 1) * MT + mt) * 64 + l]); \
-        }                                                                                     \
-        if (ks + 2 == KH && dma_) {                                                           \
-          FMMX_DMA_W((o_) + 1, ANEXT_);                                                       \
-          dma_ = false;                                                                       \
-        }                                                                                     \
-        __builtin_amdgcn_sched_barrier(0);                                                    \
-        const h8 bf_ = bfrag_w<W>(BC_, ks);                                                   \
-        _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                     \
-          acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[ks & 1][mt], bf_, acc[mt], 0, 0, 0); \
-        __builtin_amdgcn_sched_barrier(0);                                                    \
-      }                                                                                       \
-    }                                                                                         \
-    if (dma_) FMMX_DMA_W((o_) + 1, ANEXT_);                                                   \
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");                                          \
-    __syncthreads();                                                                          \
+      _Pragma("unroll") for (int ks = 0; ks < KH; ks++) {                                       \
+        if (ks + 1 < KH) {                                                                      \
+          _Pragma("unroll") for (int mt = 0; mt < MT; mt++)                                     \
+            af[(ks + 1) & 1][mt] = __builtin_bit_cast(h8, Ab[((ks + 1) * MT + mt) * 64 + l]);   \
+        }                                                                                       \
+        if (ks + 2 == KH && dma_) {                                                             \
+          FMMX_DMA_W((o_) + 1, ANEXT_);                                                         \
+          dma_ = false;                                                                         \
+        }                                                                                       \
+        __builtin_amdgcn_sched_barrier(0);                                                      \
+        FMMX_MFMA_W(af[ks & 1], ks, BC_);                                                       \
+        __builtin_amdgcn_sched_barrier(0);                                                      \
+      }                                                                                         \
+    }                                                                                           \
+    if (dma_) FMMX_DMA_W((o_) + 1, ANEXT_);                                                     \
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");                                            \
+    __syncthreads();                                                                            \
   }
   if constexpr (DB) {
     if (o0 < o1) {
@@ -794,10 +812,7 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
             for (int mt = 0; mt < MT; mt++) af[(ks + 1) & 1][mt] = __builtin_bit_cast(h8, As0[((ks + 1) * MT + mt) * 64 + l]);
           }
           __builtin_amdgcn_sched_barrier(0);
-          const h8 bf = bfrag_w<W>(bcur, ks);
-#pragma unroll
-          for (int mt = 0; mt < MT; mt++)
-            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[ks & 1][mt], bf, acc[mt], 0, 0, 0);
+          FMMX_MFMA_W(af[ks & 1], ks, bcur);
           __builtin_amdgcn_sched_barrier(0);
         }
       }
@@ -806,21 +821,24 @@ __global__ __launch_bounds__(64 * WPG) void k_m2l_w(M2LLaunch A) {
     (void)next_any;
   }
 #undef FMMX_STEP_W
+#undef FMMX_MFMA_W
 #undef FMMX_DMA_W
 #undef FMMX_LOAD_BW
-  if (tord >= 0) {
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    if (tord[n] < 0) continue;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         const int i = 32 * mt + 8 * (e / 4) + 4 * kq + e % 4;
         if (i < P2) {
-          const float v = acc[mt][e] * oscale;
+          const float v = acc[n][mt][e] * oscale;
           if (nsplit == 1) {
-            float* dst = L + (size_t)tord * P2 + i;
+            float* dst = L + (size_t)tord[n] * P2 + i;
             *dst = init ? v : *dst + v;
           } else {
-            part[((size_t)split * nb + tord) * P2 + i] = v;
+            part[((size_t)split * nb + tord[n]) * P2 + i] = v;
           }
         }
       }
@@ -878,12 +896,13 @@ int launch_m2l_split(int p, const M2LLaunch& A, int nwg, int nt, int wpg, int db
   });
 }
 
-int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, int wpg, hipStream_t stream) {
+int launch_m2l_wide(int p, const M2LLaunch& A, int nwg, int wpg, int nt, hipStream_t stream) {
   return dispatch_split_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if constexpr ((size_t)2 * WLayout<P>::FRW * 16 <= 160 * 1024 - 64) {
-      if (wpg == 4) k_m2l_w<P, 4, 0><<<nwg, 256, 0, stream>>>(A);
-      else k_m2l_w<P, 8, 1><<<nwg, 512, 0, stream>>>(A);
+      if (nt == 4) k_m2l_w<P, 4, 1, 2><<<nwg, 256, 0, stream>>>(A);
+      else if (wpg == 4) k_m2l_w<P, 4, 0, 1><<<nwg, 256, 0, stream>>>(A);
+      else k_m2l_w<P, 8, 1, 1><<<nwg, 512, 0, stream>>>(A);
       return hipGetLastError() == hipSuccess ? FMMX_OK : fail(FMMX_EDEVICE, "split M2L launch failed");
     }
     return fail(FMMX_EUNSUPPORTED, "the 32x32x16 split M2L needs p <= 10");

@@ -477,6 +477,72 @@ __global__ __launch_bounds__(CL_T) void k_class_place(ClassLevels L) {
   }
 }
 
+// ---------------------------------------------------------------------
+// Box statistics of every level (automatic depth): the points per level-8
+// cell, counted in Morton order so that the 8 children of a box are 8
+// consecutive counts; each level's counts are the sums of its children's.
+// Per level l: boxes[l] = boxes holding a point, sumsq[l] = sum of the
+// squared counts (the near-field pair count of a box grows with it).
+constexpr int OCC_L = 8;
+constexpr long OCC_CELLS = 1L << (3 * OCC_L);  // 2^24 counts, 64 MB
+
+__global__ void k_occ_mark(int n, const float* __restrict__ x, const float* __restrict__ y,
+                           const float* __restrict__ z, float xmin, float ymin, float zmin, double rbinv,
+                           unsigned* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t k = 0xffffffffu;
+  if (i < n) {
+    const int kl = 1 << OCC_L;
+    const int f0 = min(max((int)((double)(x[i] - xmin) * rbinv), 0), kl - 1);
+    const int f1 = min(max((int)((double)(y[i] - ymin) * rbinv), 0), kl - 1);
+    const int f2 = min(max((int)((double)(z[i] - zmin) * rbinv), 0), kl - 1);
+    k = morton_encode(f0, f1, f2, OCC_L);
+  }
+  // a wave whose points share one cell (clustered or sorted input) adds
+  // once: same-address atomics serialise
+  const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+  const unsigned long long same = __ballot(k == k0 && i < n);
+  const unsigned long long act = __ballot(i < n);
+  if (same == act) {
+    if ((threadIdx.x & 63) == 0 && act) atomicAdd(&cnt[k0], (unsigned)__popcll(act));
+  } else if (i < n) {
+    atomicAdd(&cnt[k], 1u);
+  }
+}
+
+// one level up: out[b] = sum of in[8b .. 8b+7] for nout boxes, and the
+// statistics of the input level, st[0] += occupied boxes, st[1] += sum of
+// squared counts (one atomic pair per workgroup)
+__global__ __launch_bounds__(256) void k_occ_up(long nout, const unsigned* __restrict__ in,
+                                                unsigned* __restrict__ out, unsigned long long* __restrict__ st) {
+  __shared__ unsigned long long red[2][4];
+  unsigned long long nb = 0, sq = 0;
+  for (long b = (long)blockIdx.x * 256 + threadIdx.x; b < nout; b += (long)gridDim.x * 256) {
+    const uint4 a = *(const uint4*)(in + 8 * b), c = *(const uint4*)(in + 8 * b + 4);
+    const unsigned v[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+    unsigned t = 0;
+    for (int k = 0; k < 8; k++) {
+      t += v[k];
+      nb += v[k] != 0;
+      sq += (unsigned long long)v[k] * v[k];
+    }
+    out[b] = t;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    nb += __shfl_xor(nb, o);
+    sq += __shfl_xor(sq, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = nb;
+    red[1][threadIdx.x >> 6] = sq;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const unsigned long long v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+    if (v) atomicAdd(&st[threadIdx.x], v);
+  }
+}
+
 }  // namespace
 
 // Stable radix sort of the keys in kbuf[0] by their low `bits` bits, the
@@ -804,6 +870,52 @@ int Engine::build_m2l_order() {
   k_class_count<<<nch, CL_T, 0, stream>>>(L);
   k_class_place<<<nch, CL_T, 0, stream>>>(L);
   FMMX_HIP(hipGetLastError());
+  return FMMX_OK;
+}
+
+// Box statistics of levels 0..8 for the n points (x, y, z) in the current
+// root cube: boxes[l] holding a point and sumsq[l], the sum of the squared
+// points per box (synchronous: one readback).
+int Engine::occupancy(int n, const float* x, const float* y, const float* z, long boxes[MAXL + 1],
+                      double sumsq[MAXL + 1]) {
+  for (int l = 0; l <= MAXL; l++) boxes[l] = 0, sumsq[l] = 0;
+  if (n <= 0) return FMMX_OK;
+  static_assert(OCC_L == MAXL, "statistics from the deepest level");
+  int rc;
+  // counts of levels 8..0 (level l: 8^l words) then 2 words of statistics
+  // per level
+  long words = 0;
+  for (int l = 0; l <= OCC_L; l++) words += 1L << (3 * l);
+  const long st_off = (words + 3) / 4 * 4;  // 16-byte aligned
+  if ((rc = occ_bits.ensure((size_t)(st_off + 4 * (OCC_L + 1)) * 4))) return rc;
+  unsigned* base = occ_bits.as<unsigned>();
+  unsigned* lvl[OCC_L + 1];
+  long o = 0;
+  for (int l = OCC_L; l >= 0; l--) {
+    lvl[l] = base + o;  // level l starts at a multiple of 8 words (16-byte aligned for l >= 1)
+    o += 1L << (3 * l);
+  }
+  unsigned long long* st = (unsigned long long*)(base + st_off);
+  k_fill_int<<<(int)((st_off + 4 * (OCC_L + 1) + TPB - 1) / TPB), TPB, 0, stream>>>(st_off + 4 * (OCC_L + 1), 0,
+                                                                                    (int*)base);
+  const double rbinv = (double)((float)(1 << OCC_L) / rd);
+  k_occ_mark<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rbinv, lvl[OCC_L]);
+  for (int l = OCC_L; l >= 1; l--) {
+    const long nout = 1L << (3 * (l - 1));
+    const int g = (int)std::min<long>(1024, (nout + 255) / 256);
+    k_occ_up<<<g, 256, 0, stream>>>(nout, lvl[l], lvl[l - 1], st + 2 * l);
+  }
+  if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, (2 + 8 * (MAXL + 1)) * 4, hipHostMallocDefault));
+  static_assert(4 * (MAXL + 1) <= 2 + 8 * (MAXL + 1), "readback buffer");
+  FMMX_HIP(hipMemcpyAsync(rb_host, st, (size_t)2 * (OCC_L + 1) * 8, hipMemcpyDeviceToHost, stream));
+  FMMX_HIP(hipStreamSynchronize(stream));
+  const unsigned long long* h = (const unsigned long long*)rb_host;
+  boxes[0] = 1;
+  sumsq[0] = (double)n * (double)n;
+  for (int l = 1; l <= OCC_L; l++) {
+    boxes[l] = (long)h[2 * l];
+    sumsq[l] = (double)h[2 * l + 1];
+  }
   return FMMX_OK;
 }
 

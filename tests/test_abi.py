@@ -196,3 +196,49 @@ def test_create_rejects_bad_config():
         h = C.c_void_p()
         rc = F.lib().fmmx_create(C.byref(cfg), 0, None, C.byref(h))
         assert rc == 1 and not h.value, kw
+
+
+def _uniform_stats(n):
+    """Box statistics of n points spread evenly over the root cube."""
+    b = np.array([min(8 ** l, n) for l in range(9)], np.int64)
+    return b, n.__float__() ** 2 / b
+
+
+def _surface_stats(n):
+    """The same for n points spread evenly over a surface (4^l boxes)."""
+    b = np.array([min(max(1, 6 * 4 ** l // 4), n) for l in range(9)], np.int64)
+    b[0] = 1
+    return b, float(n) ** 2 / b
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_auto_lmax_rule(precision):
+    """The depth rule of lmax 0 (EXACT): levels 2..8; never shallower for
+    more points in the same boxes; never deeper for a larger p; a clustered
+    input (the same boxes, but most points in a few of them: a larger sum
+    of squares) never gets a shallower tree than an even one."""
+    for p in range(2, 13):
+        prev = 2
+        for n in np.unique(np.logspace(1, 9, 300).astype(np.int64)):
+            b, q = _uniform_stats(int(n))
+            lm = F.auto_lmax(b, q, p, precision)
+            assert 2 <= lm <= 8 and lm >= prev, (p, n)
+            prev = lm
+            if p > 2:
+                assert lm <= F.auto_lmax(b, q, p - 1, precision), (p, n)
+            assert F.auto_lmax(b, q * 20, p, precision) >= lm
+            bs, qs = _surface_stats(int(n))
+            assert 2 <= F.auto_lmax(bs, qs, p, precision) <= 8
+    # 1M uniform points at p = 10: 5 levels (about 30 points per leaf box),
+    # where the mean occupancy is between the measured optima of 10^5 and 3 10^6
+    b, q = _uniform_stats(1_000_000)
+    assert F.auto_lmax(b, q, 10, "mixed") == 5
+
+
+def test_auto_lmax_rejects_bad_arguments():
+    b, q = _uniform_stats(1000)
+    for args in [(b, q, 1, "mixed"), (b, q, 13, "mixed"), (-b, q, 8, "mixed"), (b, -q, 8, "mixed")]:
+        with pytest.raises(F.FMMError):
+            F.auto_lmax(*args)
+    with pytest.raises(ValueError):
+        F.auto_lmax(b[:5], q, 8)

@@ -25,6 +25,39 @@ def test_ellipsoid_surface_geometry():
     assert abs(vol - 4 / 3 * np.pi * 1.5 * 1.0 * 0.6) < 0.05 * vol  # bump adds a little
 
 
+def _block_system(sizes, seed=0):
+    rng = np.random.default_rng(seed)
+    blocks = [np.eye(m) * 3 + rng.standard_normal((m, m)) / np.sqrt(m) for m in sizes]
+    b = rng.standard_normal(sum(sizes))
+    seg = np.repeat(np.arange(len(sizes)), sizes)
+    return blocks, b, seg
+
+
+def _check_gmres_batched(dev):
+    # segments shorter than, equal to and longer than the chunk, an empty one
+    sizes, chunk = [5, 16, 37, 0, 64, 3], 16
+    blocks, b, seg = _block_system(sizes)
+    Ab = [torch.from_numpy(a).to(dev) for a in blocks]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+
+    def matvec(x):
+        return torch.cat([Ab[s] @ x[off[s]:off[s + 1]] for s in range(len(sizes))])
+
+    bt = torch.from_numpy(b).to(dev)
+    x, res, it = bem.gmres_batched(matvec, bt, torch.from_numpy(seg).to(dev), len(sizes),
+                                   precond=lambda v: v / 3, rtol=1e-10, restart=8, maxiter=200, chunk=chunk)
+    x = x.cpu().numpy()
+    for s, a in enumerate(blocks):
+        if sizes[s]:
+            ref = np.linalg.solve(a, b[off[s]:off[s + 1]])
+            assert np.linalg.norm(x[off[s]:off[s + 1]] - ref) <= 1e-8 * np.linalg.norm(ref)
+    assert float(res.max()) <= 1e-10 and it > 8  # restarted at least once
+
+
+def test_gmres_batched_block_solves_cpu():
+    _check_gmres_batched("cpu")
+
+
 # ---- GPU ----------------------------------------------------------------------
 # torch sees the device first (the batched solver keeps its vectors there)
 HAVE_GPU = torch.cuda.is_available()
@@ -92,3 +125,10 @@ def test_batched_solvation_matches_single_solves():
         print(f"surface {s}: {len(f)} panels, batched {r['energies'][s]:.6f}, alone {one['energy']:.6f}",
               flush=True)
         assert abs(r["energies"][s] - one["energy"]) < 1e-5 * abs(one["energy"])
+
+
+
+@pytest.mark.gpu
+def test_gmres_batched_block_solves_gpu():
+    """The device path: per-iteration flags read one iteration late."""
+    _check_gmres_batched("cuda")

@@ -230,16 +230,61 @@ def test_edge_cases():
 
 
 def test_auto_box_and_level():
-    """rd <= 0: root cube from the points (bio/main.cxx:364-380); lmax 0:
-    fmm/nlevel.cxx thresholds."""
+    """rd <= 0: root cube from the points (bio/main.cxx:364-380); lmax 0 in
+    the REF convention: fmm/nlevel.cxx thresholds."""
     x, q = uniform_cube(120000, 60, lo=-2.0, hi=3.0)
-    f = F.FMM(p=6, lmax=0, precision="fp64", convention="exact")
+    f = F.FMM(p=6, lmax=0, precision="fp64", convention="ref")
     f.set_points(x)
     assert f.lmax == 3  # 1e5 <= N < 5e5
+    assert not f.occupancy()[0].any()  # the cost model did not run
     v, _ = f.evaluate(q)
     box = O.bounding_cube(x)
-    ov, _ = O.fmm(x, x, q, mp=6, lmax=3, conv=O.EXACT, box=box, nthreads=8)
+    ov, _ = O.fmm(x, x, q, mp=6, lmax=3, conv=O.REF, box=box, nthreads=8, fp64_out=True)
     assert rel_l2(v, ov) < 1e-11
+    f.close()
+
+
+def _box_stats_np(x, box, lev):
+    """Boxes holding a point and the sum of squared points per box at a
+    level, keys as fmm/boxn.cxx forms them (float offsets times the float
+    scale widened to double, truncated)."""
+    lo = np.asarray(box[:3], np.float32)
+    scale = np.float64(np.float32(1 << lev) / np.float32(box[3]))
+    c = np.floor((x - lo).astype(np.float32).astype(np.float64) * scale).astype(np.int64).clip(0, (1 << lev) - 1)
+    _, cnt = np.unique((c[:, 0] << (2 * lev)) | (c[:, 1] << lev) | c[:, 2], return_counts=True)
+    return len(cnt), float((cnt.astype(np.float64) ** 2).sum())
+
+
+@pytest.mark.parametrize("dist", ["cube", "sphere", "plummer"])
+def test_auto_depth_exact(dist):
+    """lmax 0 in the EXACT convention: the engine counts the points of every
+    level-8 cell on the device (Morton order, each level's counts summed
+    from its children's) and applies the cost model of fmmx_auto_lmax to
+    the boxes and sums of squared counts.  The statistics equal numpy's
+    counts of the same keys at levels 1..8; the depth equals the rule
+    applied to them; a uniform 300k cloud at p = 8 gets 4 levels, the
+    measured optimum."""
+    from old_fmm_bem_amd import workloads as W
+
+    n = 300_000
+    x = {"cube": lambda: W.cube(n)[0], "sphere": lambda: W.sphere(n)[0], "plummer": lambda: W.plummer(n)[0]}[dist]()
+    f = F.FMM(p=8, lmax=0, precision="mixed", convention="exact")
+    f.set_points(x)
+    lm, box = f.domain()
+    boxes, sumsq = f.occupancy()
+    assert boxes[0] == 1 and sumsq[0] == float(n) ** 2
+    for lev in range(1, 9):
+        nb, sq = _box_stats_np(x, box, lev)
+        assert boxes[lev] == nb and sumsq[lev] == sq, lev
+    assert lm == F.auto_lmax(boxes, sumsq, 8, "mixed")
+    if dist == "cube":
+        assert lm == 4
+    # the evaluation on the chosen tree is correct
+    q = np.random.default_rng(2).random(n).astype(np.float32)
+    v, _ = f.evaluate(q)
+    idx = np.random.default_rng(3).choice(n, 200, replace=False)
+    vd, _ = O.direct(x[idx], x, q, conv=O.EXACT, nthreads=8)
+    assert rel_l2(v[idx], vd) < 2e-4  # p = 8 truncation
     f.close()
 
 

@@ -89,7 +89,7 @@ def test_mixed_edge_cases():
     f.close()
 
 
-SPLIT_SHAPES = [(2, 5, 8), (2, 5, 4), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
+SPLIT_SHAPES = [(2, 5, 8), (2, 5, 4), (4, 5, 4), (1, 4, 8), (1, 3, 8), (1, 3, 4), (2, 3, 4)]
 
 
 def _with_shape(f, shape, lmax):
@@ -100,7 +100,8 @@ def _with_shape(f, shape, lmax):
 @pytest.mark.parametrize("p", [6, 8, 10, 12])
 def test_split_fp16_m2l_matches_fp32(p):
     """The error-compensated FP16 M2L (staging 3, 4: hi/lo operands, three
-    products per pair on v_mfma_f32_16x16x32_f16; 5: on 32x32x16, p <= 10)
+    products per pair on v_mfma_f32_16x16x32_f16; 5: on 32x32x16, p <= 10,
+    32 or 64 targets per wave)
     against the FP32 MFMA M2L
     and the FP64 engine on the same tree, every launch shape: the split form
     carries 2^-22 relative operand error against FP32's 2^-24, so its
