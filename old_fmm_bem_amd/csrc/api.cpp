@@ -28,34 +28,27 @@ __global__ void k_gather_q(int n, const uint32_t* __restrict__ perm, const float
 }
 
 // device counters the host needs after the tree build, gathered into one
-// small buffer so that a single copy brings them back
+// small buffer so that a single copy brings them back:
+// [0, 9) target boxes per level, [9, 18) source boxes per level, [18] the
+// out-of-cube flag, [19 + 8 l + c) target boxes of level l in parity class c
 struct CountSrc {
-  const int* a[2 + MAXL + 1];  // box counts of both sides, class counts per level
-  const unsigned long long* pairs;
-  const int* bad;  // coordinates outside the root cube (stage 0)
+  const int* cnt[2];  // box counts of both sides
+  const int* cls;     // target boxes per level and class
+  const int* bad;     // coordinates outside the root cube
 };
-__global__ void k_pack_counts(CountSrc src, int lmax, int stage, int* __restrict__ out) {
+constexpr int RB_CLS = 2 * (MAXL + 1) + 1;
+static_assert(RB_CLS + 8 * (MAXL + 1) <= RB_INTS, "readback buffer");
+__global__ void k_pack_counts(CountSrc src, int lmax, int* __restrict__ out) {
   const int t = threadIdx.x;
-  if (stage == 0) {
-    // [0, 9): target boxes per level, [9, 18): source boxes per level
-    if (t < 2 * (MAXL + 1)) {
-      const int* c = src.a[t / (MAXL + 1)];
-      out[t] = c ? c[t % (MAXL + 1)] : 0;
-    } else if (t == 2 * (MAXL + 1)) {
-      out[t] = src.bad ? *src.bad : 0;
-    }
-  } else {
-    // [0, 2): near pair count (u64), then 8 class counts per level
-    if (t == 0) {
-      const unsigned long long v = src.pairs ? *src.pairs : 0ull;
-      out[0] = (int)(v & 0xffffffffu);
-      out[1] = (int)(v >> 32);
-    }
-    for (int e = t; e < 8 * (MAXL + 1); e += blockDim.x) {
-      const int l = e / 8;
-      const int* c = src.a[2 + l];
-      out[2 + e] = (l >= 2 && l <= lmax && c) ? c[e % 8] : 0;
-    }
+  if (t < 2 * (MAXL + 1)) {
+    const int* c = src.cnt[t / (MAXL + 1)];
+    out[t] = c ? c[t % (MAXL + 1)] : 0;
+  } else if (t == 2 * (MAXL + 1)) {
+    out[t] = src.bad ? *src.bad : 0;
+  }
+  for (int e = t; e < 8 * (MAXL + 1); e += blockDim.x) {
+    const int l = e / 8;
+    out[RB_CLS + e] = (l >= 2 && l <= lmax && src.cls) ? src.cls[e] : 0;
   }
 }
 
@@ -507,6 +500,7 @@ int Engine::graph_launch() {
 Engine::~Engine() {
   drop_graph();
   if (rb_host) (void)hipHostFree(rb_host);
+  if (stats_host) (void)hipHostFree(stats_host);
   if (gstream) (void)hipStreamDestroy(gstream);
   for (auto& e : gev)
     if (e) (void)hipEventDestroy(e);
@@ -717,6 +711,7 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
                        const float* nyj, const float* nzj, int flags) {
   if (ni < 0 || nj < 0) return fail(FMMX_EINVAL, "negative particle count");
   drop_graph();  // a captured evaluation refers to this tree's buffers
+  tree_stats_pending = false;
   if ((ni > 0 && (!xi || !yi || !zi)) || (nj > 0 && (!xj || !yj || !zj)))
     return fail(FMMX_EINVAL, "coordinate pointer is NULL");
   FMMX_HIP(hipSetDevice(device));
@@ -747,7 +742,7 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
     k_init_mm<<<1, 64, 0, stream>>>(mm.as<int>());
     if (ni > 0) k_minmax<<<nblk(ni), TPB, 0, stream>>>(ni, d[0], d[1], d[2], mm.as<int>());
     if (nj > 0) k_minmax<<<nblk(nj), TPB, 0, stream>>>(nj, d[6], d[7], d[8], mm.as<int>());
-    if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, (2 + 8 * (MAXL + 1)) * 4, hipHostMallocDefault));
+    if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, RB_INTS * 4, hipHostMallocDefault));
     int* h = rb_host;
     FMMX_HIP(hipMemcpyAsync(h, mm.p, 24, hipMemcpyDeviceToHost, stream));
     FMMX_HIP(hipStreamSynchronize(stream));
@@ -787,42 +782,55 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   if ((rc = bad_flag.ensure(4))) return rc;
   k_zero_i32<<<1, 32, 0, stream>>>(1, bad_flag.as<int>());
   // tree of both sides (keys, sort, gathers, per-level boxes)
-  if ((rc = build_side(si, ni, d[0], d[1], d[2], d[3], d[4], d[5], false))) return rc;
+  // targets = sources: one tree, whose dense tables the source side uses
+  if ((rc = build_side(si, ni, d[0], d[1], d[2], d[3], d[4], d[5], same))) return rc;
   if ((rc = build_side(sj, nj, d[6], d[7], d[8], d[9], d[10], d[11], true, same ? &si : nullptr))) return rc;
-  // one pinned copy per readback (small pageable copies cost tens of us each)
-  const int nrb = 2 + 8 * (MAXL + 1);
-  if ((rc = rb_dev.ensure(nrb * 4))) return rc;
-  if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, nrb * 4, hipHostMallocDefault));
+  // the one readback of the tree build: box counts of every level (buffer
+  // sizes and grids of the lists and phases), the target boxes per parity
+  // class (M2L launch shapes) and the out-of-cube flag; pinned, since small
+  // pageable copies cost tens of us each
+  if ((rc = rb_dev.ensure(RB_INTS * 4))) return rc;
+  if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, RB_INTS * 4, hipHostMallocDefault));
   CountSrc cs{};
-  cs.a[0] = ni > 0 ? si.cnt.as<int>() : nullptr;
-  cs.a[1] = nj > 0 ? sj.cnt.as<int>() : nullptr;
+  cs.cnt[0] = ni > 0 ? si.cnt.as<int>() : nullptr;
+  cs.cnt[1] = nj > 0 ? sj.cnt.as<int>() : nullptr;
+  cs.cls = ni > 0 ? si.clscnt.as<int>() : nullptr;
   cs.bad = bad_flag.as<int>();
-  k_pack_counts<<<1, 64, 0, stream>>>(cs, lmax, 0, rb_dev.as<int>());
-  FMMX_HIP(hipMemcpyAsync(rb_host, rb_dev.p, (2 * (MAXL + 1) + 1) * 4, hipMemcpyDeviceToHost, stream));
+  k_pack_counts<<<1, 64, 0, stream>>>(cs, lmax, rb_dev.as<int>());
+  FMMX_HIP(hipMemcpyAsync(rb_host, rb_dev.p, RB_INTS * 4, hipMemcpyDeviceToHost, stream));
   FMMX_HIP(hipStreamSynchronize(stream));
   if (rb_host[2 * (MAXL + 1)]) return fail(FMMX_EINVAL, "coordinates outside the root cube or not finite");
   for (int l = 2; l <= lmax; l++) {
     si.nb[l] = rb_host[l];
     sj.nb[l] = rb_host[MAXL + 1 + l];
   }
+  for (int l = 2; l <= MAXL; l++)
+    for (int c = 0; c < 8; c++) cls_cnt_h[l][c] = l <= lmax ? rb_host[RB_CLS + 8 * l + c] : 0;
   if ((rc = build_lists())) return rc;
   if ((rc = build_m2l_order())) return rc;
   if (keep_lists && (rc = build_far_lists())) return rc;
   FMMX_HIP(hipEventRecord(ev[1], stream));
-  CountSrc cc{};
-  for (int l = 2; l <= lmax; l++) cc.a[2 + l] = si.nb[l] > 0 ? cls_cnt[l].as<int>() : nullptr;
-  cc.pairs = si.nb[lmax] > 0 ? pair_ctr.as<unsigned long long>() : nullptr;
-  k_pack_counts<<<1, 64, 0, stream>>>(cc, lmax, 1, rb_dev.as<int>());
-  FMMX_HIP(hipMemcpyAsync(rb_host, rb_dev.p, nrb * 4, hipMemcpyDeviceToHost, stream));
-  FMMX_HIP(hipStreamSynchronize(stream));
-  for (int l = 2; l <= MAXL; l++)
-    for (int c = 0; c < 8; c++) cls_cnt_h[l][c] = l <= lmax ? rb_host[2 + 8 * l + c] : 0;
-  p2p_pairs = (int64_t)(((unsigned long long)(uint32_t)rb_host[1] << 32) | (uint32_t)rb_host[0]);
+  // statistics read when asked for (fmmx_get_stats), without a wait here:
+  // the near pair count and the tree time
+  if (!stats_host) FMMX_HIP(hipHostMalloc((void**)&stats_host, 64, hipHostMallocDefault));
+  stats_host[0] = 0;
+  if (si.nb[lmax] > 0) FMMX_HIP(hipMemcpyAsync(stats_host, pair_ctr.p, 8, hipMemcpyDeviceToHost, stream));
+  tree_stats_pending = true;
+  have_points = true;
+  last_neq = -1;
+  return FMMX_OK;
+}
+
+// The statistics of the last set_points (near pairs, tree time), once its
+// work has finished.
+int Engine::resolve_tree_stats() {
+  if (!tree_stats_pending) return FMMX_OK;
+  FMMX_HIP(hipEventSynchronize(ev[1]));
+  p2p_pairs = (int64_t)stats_host[0];
   float ms = 0;
   FMMX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   stats.ms_tree = ms;
-  have_points = true;
-  last_neq = -1;
+  tree_stats_pending = false;
   return FMMX_OK;
 }
 
@@ -1385,6 +1393,7 @@ int fmmx_set_timing(fmmx_handle* h, int per_phase) {
 int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st) {
   if (!h || !st) return fmmx::fail(FMMX_EINVAL, "NULL argument");
   Engine& e = h->e;
+  if (int rc = e.resolve_tree_stats()) return rc;
   if (hipEventSynchronize(e.ev[15]) != hipSuccess) return fmmx::fail(FMMX_EDEVICE, "event synchronize failed");
   fmmx_stats s = e.stats;
   float ms = 0;

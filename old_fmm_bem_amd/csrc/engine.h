@@ -11,6 +11,7 @@
 namespace fmmx {
 
 constexpr int MAXL = 8;  // deepest supported leaf level (keys of 3*8 bits)
+constexpr int RB_INTS = 128;  // ints of the pinned readback buffer
 
 #define FMMX_HIP(call)                                                                   \
   do {                                                                                   \
@@ -70,8 +71,9 @@ struct Side {
   bool has_normals = false;
   DevBuf keys_in, idx_in, keys, perm, tmp;
   DevBuf x, y, z, nx, ny, nz;
-  std::array<DevBuf, MAXL + 1> flag, pos, bkey, bfirst, parent, cfirst, dense;
+  std::array<DevBuf, MAXL + 1> bkey, bfirst, parent, cfirst, dense;
   DevBuf blast, cnt;
+  DevBuf clscnt;  // boxes per level and parity class, [8 l + class]
   std::array<int, MAXL + 1> nb{};  // host copy of box counts per level
 };
 
@@ -95,9 +97,8 @@ struct Engine {
   Tables tab;
 
   Side si, sj;
-  DevBuf scratch;    // rocPRIM temporary storage (level scans)
   DevBuf rs_counts;  // digit counts per tile of the radix sort
-  DevBuf lvl_f4, lvl_p4;  // box-start flags and numbers of four levels (tree build)
+  DevBuf box_tiles;       // boxes opened per tile of points and level (tree build)
   DevBuf xyz4;            // packed coordinates in caller order (tree build)
   // near list at the leaf level (targets): counts and rows of NNEAR
   DevBuf nij, neij, pair_ctr;
@@ -126,7 +127,10 @@ struct Engine {
   bool canon = false;  // FP32 EXACT: M2L from the 56 reflection-canonical operators
   DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)
   DevBuf rb_dev;            // counters gathered for one readback
-  int* rb_host = nullptr;   // pinned host copy of rb_dev
+  int* rb_host = nullptr;   // pinned host copy of rb_dev (RB_INTS ints)
+  unsigned long long* stats_host = nullptr;  // pinned: near pair count of the last set_points
+  bool tree_stats_pending = false;           // p2p_pairs and ms_tree not yet read
+  int resolve_tree_stats();
   int ops_lmax = -1;
   float ops_rd = -1;
   // caller-order staging for host pointers

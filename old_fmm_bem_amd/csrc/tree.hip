@@ -3,8 +3,6 @@
 // radix sort), SoA gathers (sort/sortvar.cxx), per-level box lists
 // (fmm/boxdatai.cxx, boxdataj.cxx, boxm2mi.cxx, boxm2mj.cxx), and the
 // interaction lists of fmm/ijbox.cxx in the reference's loop order.
-#include <rocprim/device/device_scan.hpp>
-
 #include <cmath>
 #include <mutex>
 
@@ -204,77 +202,190 @@ __global__ void k_gather3(int n, const uint32_t* __restrict__ perm, const float*
   b2[i] = a2[j];
 }
 
-// Box starts of up to four levels in one pass: component k of f[i] flags
-// whether particle i (key order) opens a box at level l0 + k (the leaf keys
-// shifted by 3 (lmax - level) give that level's keys, fmm/boxdatai.cxx:10-34);
-// one exclusive scan of the int4 flags numbers the boxes of all four levels.
-__device__ __forceinline__ int comp4(const int4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+// Boxes of every level in two passes over the sorted keys (fmm/boxdatai.cxx:
+// 10-34, boxm2mi.cxx:6-24: a box of level l starts where the leaf keys
+// shifted by 3 (lmax - l) change).  Keys are sorted, so a point that opens a
+// box at level l opens one at every deeper level too: per point the coarsest
+// such level lf(i) says everything.  Tiles of BX_TILE points: the first pass
+// counts the boxes each tile opens per level, the second sums the counts of
+// the earlier tiles, scans its own and writes per box: key, first particle,
+// parent (the level l-1 box holding that particle), first child of the
+// parent (children of a parent are consecutive, fmm/boxm2mj.cxx:267-275),
+// leaf last particle, the dense key -> box table (when given), and the
+// level totals with the first-child sentinels.
+constexpr int BX_T = 256, BX_PT = 8, BX_TILE = BX_T * BX_PT;
 
-__global__ void k_level_flags4(int n, const uint32_t* __restrict__ keys, int lmax, int l0, int nl,
-                               int4* __restrict__ f) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int v[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int shift = 3 * (lmax - (l0 + k));
-    v[k] = k < nl && (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1 : 0;
-  }
-  f[i] = make_int4(v[0], v[1], v[2], v[3]);
+struct BoxOut {
+  uint32_t* bkey[MAXL + 1];
+  int *bfirst[MAXL + 1], *parent[MAXL + 1], *cfirst[MAXL + 1], *dense[MAXL + 1];
+  int* blast;
+  int* cnt;
+  int* clscnt;  // boxes per level and parity class, [8 l + class] (nullable)
+};
+// per tile of points: [0, 7) boxes opened at levels 2..8, [8 + 8 (l - 2) + c)
+// of them in parity class c
+constexpr int BX_REC = 64;
+
+// parity class of a box, (x & 1) | (y & 1) << 1 | (z & 1) << 2, from its key
+__device__ __forceinline__ int box_class(uint32_t k) { return ((k >> 1) & 1) | ((k & 1) << 1) | (k & 4); }
+
+// coarsest level >= 2 at which point i opens a box (lmax + 1: none)
+__device__ __forceinline__ int open_level(const uint32_t* __restrict__ keys, int i, int lmax) {
+  if (i == 0) return 2;
+  const uint32_t d = keys[i] ^ keys[i - 1];
+  if (!d) return lmax + 1;
+  // highest differing bit -> level: bits 3 (lmax - l) .. 3 (lmax - l) + 2 belong to level l
+  const int hb = 31 - __clz(d);
+  return max(2, lmax - hb / 3);
 }
 
-struct LevelOut4 {
-  uint32_t* bkey[4];
-  int *bfirst[4], *flag[4], *pos[4], *cnt[4];
+// block-wide sums (red = true) or exclusive scans of the per-thread counts
+// v[0..6] (levels 2..8) over BX_T threads; tot receives the block totals
+__device__ __forceinline__ void bx_block(int v[7], int tot[7], bool red) {
+  __shared__ int ws[BX_T / 64][7];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int inc[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    inc[k] = v[k];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int u = __shfl_up(inc[k], d, 64);
+      if (lane >= d) inc[k] += u;
+    }
+  }
+  __syncthreads();  // ws may still be read by an earlier call
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 7; k++) ws[wv][k] = inc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < BX_T / 64; w++) {
+      if (w < wv) before += ws[w][k];
+      all += ws[w][k];
+    }
+    tot[k] = all;
+    if (!red) v[k] = before + inc[k] - v[k];
+  }
+}
+
+__global__ __launch_bounds__(BX_T) void k_box_count(int n, const uint32_t* __restrict__ keys, int lmax,
+                                                    int* __restrict__ tcnt) {
+  __shared__ int scl[7 * 8];
+  if (threadIdx.x < 7 * 8) scl[threadIdx.x] = 0;
+  __syncthreads();
+  int c[7] = {0, 0, 0, 0, 0, 0, 0}, tot[7];
+  const long i0 = (long)blockIdx.x * BX_TILE + (long)threadIdx.x * BX_PT;
+#pragma unroll
+  for (int k = 0; k < BX_PT; k++) {
+    const long i = i0 + k;
+    if (i < n) {
+      const int lf = open_level(keys, (int)i, lmax);
+      const uint32_t key = keys[i];
+#pragma unroll
+      for (int l = 2; l <= MAXL; l++)
+        if (l >= lf && l <= lmax) {
+          c[l - 2]++;
+          atomicAdd(&scl[(l - 2) * 8 + box_class(key >> (3 * (lmax - l)))], 1);
+        }
+    }
+  }
+  bx_block(c, tot, true);  // synchronises the block: scl is complete
+  int* rec = tcnt + (size_t)blockIdx.x * BX_REC;
+  if (threadIdx.x < 7) rec[threadIdx.x] = tot[threadIdx.x];
+  if (threadIdx.x < 7 * 8) rec[8 + threadIdx.x] = scl[threadIdx.x];
+}
+
+__global__ __launch_bounds__(BX_T) void k_box_place(int n, const uint32_t* __restrict__ keys, int lmax,
+                                                    const int* __restrict__ tcnt, BoxOut o) {
+  // the last tile's block also totals the parity classes of every tile
+  // (the counting pass has finished: no other block writes them)
+  if (o.clscnt && blockIdx.x == gridDim.x - 1) {
+    __shared__ int part[BX_T / BX_REC][BX_REC];
+    const int e = threadIdx.x % BX_REC, r = threadIdx.x / BX_REC;
+    int sum = 0;
+    for (int t = r; t < (int)gridDim.x; t += BX_T / BX_REC) sum += tcnt[(size_t)t * BX_REC + e];
+    part[r][e] = sum;
+    __syncthreads();
+    if (threadIdx.x >= 8 && threadIdx.x < 8 + 7 * 8) {
+      int v = 0;
+#pragma unroll
+      for (int q = 0; q < BX_T / BX_REC; q++) v += part[q][threadIdx.x];
+      const int l = 2 + (threadIdx.x - 8) / 8;
+      if (l <= lmax) o.clscnt[8 * l + (threadIdx.x - 8) % 8] = v;
+    }
+  }
+  // boxes opened by the earlier tiles, per level
+  int pre[7] = {0, 0, 0, 0, 0, 0, 0}, tot[7];
+  for (int t = threadIdx.x; t < (int)blockIdx.x; t += BX_T)
+#pragma unroll
+    for (int k = 0; k < 7; k++) pre[k] += tcnt[(size_t)t * BX_REC + k];
+  bx_block(pre, tot, true);
+  int run[7], c[7] = {0, 0, 0, 0, 0, 0, 0}, ctot[7];
+  const long i0 = (long)blockIdx.x * BX_TILE + (long)threadIdx.x * BX_PT;
+  int lfs[BX_PT];
+#pragma unroll
+  for (int k = 0; k < BX_PT; k++) {
+    const long i = i0 + k;
+    lfs[k] = i < n ? open_level(keys, (int)i, lmax) : MAXL + 2;
+#pragma unroll
+    for (int l = 2; l <= MAXL; l++) c[l - 2] += (l >= lfs[k] && l <= lmax) ? 1 : 0;
+  }
+  bx_block(c, ctot, false);
+#pragma unroll
+  for (int k = 0; k < 7; k++) run[k] = tot[k] + c[k];  // boxes before this thread's first point
+  // levels unrolled (run[] stays in registers); level l is live when
+  // lf <= l <= lmax
+#pragma unroll
+  for (int k = 0; k < BX_PT; k++) {
+    const long i = i0 + k;
+    if (i >= n) break;
+    const int lf = lfs[k];
+    const uint32_t key = keys[i];
+#pragma unroll
+    for (int l = 2; l <= MAXL; l++) {
+      if (l < lf || l > lmax) continue;
+      const int b = run[l - 2]++;
+      const uint32_t bk = key >> (3 * (lmax - l));
+      o.bkey[l][b] = bk;
+      o.bfirst[l][b] = (int)i;
+      if (o.dense[l]) o.dense[l][bk] = b;
+      if (l >= 3) {
+        const int par = run[l - 3] - 1;  // the level l-1 box holding point i
+        o.parent[l][b] = par;
+        if (l > lf) o.cfirst[l - 1][par] = b;  // point i opens the parent too: first child
+      }
+      if (l == lmax && b > 0) o.blast[b - 1] = (int)i - 1;
+    }
+    if (i == n - 1) {
+#pragma unroll
+      for (int l = 2; l <= MAXL; l++) {
+        if (l > lmax) continue;
+        o.cnt[l] = run[l - 2];
+        if (l >= 3) o.cfirst[l - 1][run[l - 3]] = run[l - 2];
+        if (l == lmax) o.blast[run[l - 2] - 1] = n - 1;
+      }
+    }
+  }
+}
+
+// dense key -> box tables of every level filled with -1 in one launch (the
+// box kernel then enters the boxes)
+struct DenseLevels {
+  int* dense[MAXL + 1];
+  long off[MAXL + 2];  // flattened entries of levels 2.. (8^l each)
+  int lmax;
 };
 
-__global__ void k_level_compact4(int n, const uint32_t* __restrict__ keys, int lmax, int l0, int nl,
-                                 const int4* __restrict__ f, const int4* __restrict__ p, LevelOut4 o) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int4 fi = f[i], pi = p[i];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    if (k >= nl) break;
-    const int fl = comp4(fi, k), ps = comp4(pi, k);
-    o.flag[k][i] = fl;
-    o.pos[k][i] = ps;
-    if (fl) {
-      o.bkey[k][ps] = keys[i] >> (3 * (lmax - (l0 + k)));
-      o.bfirst[k][ps] = i;
-    }
-    if (i == n - 1) *o.cnt[k] = ps + fl;
-  }
-}
-
-// parent ordinal of each box at level lev: the level lev-1 box holding its
-// first particle (inclusive count of level lev-1 box starts, minus one)
-__global__ void k_parent(int ub, const int* __restrict__ cnt, const int* __restrict__ bfirst,
-                         const int* __restrict__ pos_up, const int* __restrict__ flag_up,
-                         int* __restrict__ parent) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= ub || b >= *cnt) return;
-  const int i = bfirst[b];
-  parent[b] = pos_up[i] + flag_up[i] - 1;
-}
-
-// first child (at level lev) of every level lev-1 box; children of one
-// parent are consecutive (fmm/boxm2mj.cxx:267-275)
-__global__ void k_cfirst(int ub, const int* __restrict__ cnt_c, const int* __restrict__ cnt_p,
-                         const int* __restrict__ parent, int* __restrict__ cfirst) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  int nc = *cnt_c;
-  if (b >= ub || b >= nc) return;
-  if (b == 0 || parent[b] != parent[b - 1]) cfirst[parent[b]] = b;
-  if (b == nc - 1) cfirst[*cnt_p] = nc;
-}
-
-__global__ void k_blast(int ub, const int* __restrict__ cnt, const int* __restrict__ bfirst, int n,
-                        int* __restrict__ blast) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  int nb = *cnt;
-  if (b >= ub || b >= nb) return;
-  blast[b] = (b + 1 < nb ? bfirst[b + 1] : n) - 1;
+__global__ void k_dense_fill(DenseLevels D) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= D.off[D.lmax + 1]) return;
+  int l = 2;
+  while (t >= D.off[l + 1]) l++;
+  D.dense[l][t - D.off[l]] = -1;
 }
 
 __global__ void k_zero_u64(unsigned long long* a, int n) {
@@ -286,13 +397,6 @@ __global__ void k_fill_int(long n, int v, int* __restrict__ a) {
   if (i < n) a[i] = v;
 }
 
-__global__ void k_dense(int ub, const int* __restrict__ cnt, const uint32_t* __restrict__ bkey,
-                        int* __restrict__ dense) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= ub || b >= *cnt) return;
-  dense[bkey[b]] = b;
-}
-
 // fmm/ijbox.cxx:322-345 (neib 0): jx -> jy -> jz over the 3x3x3 block, empty
 // and out-of-grid boxes skipped.  The reference clips to the bounding box of
 // the source boxes (:312-321); positions outside it hold no source box, so
@@ -301,24 +405,28 @@ __global__ void k_near_list(int nbi, const uint32_t* __restrict__ bkey, int lev,
                             const int* __restrict__ ifirst, const int* __restrict__ ilast,
                             const int* __restrict__ jfirst, const int* __restrict__ jlast, int* __restrict__ nij,
                             int* __restrict__ neij, unsigned long long* __restrict__ pairs, int gsh) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbi) return;
-  int ix, iy, iz, g = 1 << lev, c = 0;
-  morton_decode(bkey[b], lev, ix, iy, iz);
-  long npair = 0, ni = ilast[b] - ifirst[b] + 1;
-  for (int jx = max(ix - 1, 0); jx <= min(ix + 1, g - 1); jx++)
-    for (int jy = max(iy - 1, 0); jy <= min(iy + 1, g - 1); jy++)
-      for (int jz = max(iz - 1, 0); jz <= min(iz + 1, g - 1); jz++) {
-        // groups: boxes in different cells of the group level do not interact
-        if (gsh >= 0 && ((jx ^ ix) >> gsh | (jy ^ iy) >> gsh | (jz ^ iz) >> gsh)) continue;
-        int jj = dense_j[morton_encode(jx, jy, jz, lev)];
-        if (jj >= 0) {
-          neij[b * NNEAR + c++] = jj;
-          npair += ni * (jlast[jj] - jfirst[jj] + 1);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  long npair = 0;
+  if (b < nbi) {
+    int ix, iy, iz, g = 1 << lev, c = 0;
+    morton_decode(bkey[b], lev, ix, iy, iz);
+    const long ni = ilast[b] - ifirst[b] + 1;
+    for (int jx = max(ix - 1, 0); jx <= min(ix + 1, g - 1); jx++)
+      for (int jy = max(iy - 1, 0); jy <= min(iy + 1, g - 1); jy++)
+        for (int jz = max(iz - 1, 0); jz <= min(iz + 1, g - 1); jz++) {
+          // groups: boxes in different cells of the group level do not interact
+          if (gsh >= 0 && ((jx ^ ix) >> gsh | (jy ^ iy) >> gsh | (jz ^ iz) >> gsh)) continue;
+          int jj = dense_j[morton_encode(jx, jy, jz, lev)];
+          if (jj >= 0) {
+            neij[b * NNEAR + c++] = jj;
+            npair += ni * (jlast[jj] - jfirst[jj] + 1);
+          }
         }
-      }
-  nij[b] = c;
-  atomicAdd(pairs, (unsigned long long)npair);
+    nij[b] = c;
+  }
+  // one atomic per wave (same-address atomics serialise)
+  for (int o = 32; o > 0; o >>= 1) npair += __shfl_xor(npair, o);
+  if ((threadIdx.x & 63) == 0 && npair) atomicAdd(pairs, (unsigned long long)npair);
 }
 
 // fmm/ijbox.cxx:346-365 (neib 1, used at level 2) and :366-401 (neib 2)
@@ -686,9 +794,22 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   s.has_normals = nx && ny && nz;
   size_t nn = (size_t)(n > 0 ? n : 1);
   int rc;
+  // dense key -> box tables of every level (sizes 8^l)
+  DenseLevels D{};
+  if (dense && !(same && n > 0)) {
+    D.lmax = lmax;
+    D.off[2] = 0;
+    for (int lev = 2; lev <= lmax; lev++) {
+      const long nd = 1L << (3 * lev);
+      if ((rc = s.dense[lev].ensure(nd * 4))) return rc;
+      D.dense[lev] = s.dense[lev].as<int>();
+      D.off[lev + 1] = D.off[lev] + nd;
+    }
+  }
   if (same && n > 0) {
     // the same points as `same` (targets = sources): its keys, order and
     // boxes are this side's too; only normals and dense tables are built
+    // (aliased when `same` built them)
     s.keys.alias(same->keys);
     s.perm.alias(same->perm);
     s.x.alias(same->x);
@@ -696,9 +817,8 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
     s.z.alias(same->z);
     s.blast.alias(same->blast);
     s.cnt.alias(same->cnt);
+    s.clscnt.alias(same->clscnt);
     for (int lev = 2; lev <= lmax; lev++) {
-      s.flag[lev].alias(same->flag[lev]);
-      s.pos[lev].alias(same->pos[lev]);
       s.bkey[lev].alias(same->bkey[lev]);
       s.bfirst[lev].alias(same->bfirst[lev]);
       s.parent[lev].alias(same->parent[lev]);
@@ -710,16 +830,9 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
       k_gather3<<<nblk(n), TPB, 0, stream>>>(n, s.perm.as<uint32_t>(), nx, ny, nz, s.nx.as<float>(),
                                              s.ny.as<float>(), s.nz.as<float>());
     }
-    if (dense) {
-      const int* cnt = s.cnt.as<int>();
-      for (int lev = 2; lev <= lmax; lev++) {
-        long nd = 1L << (3 * lev);
-        const int ub = (int)std::min<long>(n, nd);  // boxes at this level
-        if ((rc = s.dense[lev].ensure(nd * 4))) return rc;
-        k_fill_int<<<nblk(nd), TPB, 0, stream>>>(nd, -1, s.dense[lev].as<int>());
-        k_dense<<<nblk(ub), TPB, 0, stream>>>(ub, cnt + lev, s.bkey[lev].as<uint32_t>(), s.dense[lev].as<int>());
-      }
-    }
+    // `same` was built with its dense tables in this call (set_points)
+    if (dense)
+      for (int lev = 2; lev <= lmax; lev++) s.dense[lev].alias(same->dense[lev]);
     return FMMX_OK;
   }
   if ((rc = s.keys_in.ensure(nn * 4)) || (rc = s.idx_in.ensure(nn * 4)) || (rc = s.keys.ensure(nn * 4)) ||
@@ -731,7 +844,16 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
     return rc;
   s.nb.fill(0);
   if (n == 0) return FMMX_OK;
-  if ((rc = xyz4.ensure(nn * 16))) return rc;
+  for (int lev = lmax; lev >= 2; lev--) {
+    if ((rc = s.bkey[lev].ensure(nn * 4)) || (rc = s.bfirst[lev].ensure(nn * 4)) ||
+        (rc = s.parent[lev].ensure(nn * 4)) || (rc = s.cfirst[lev].ensure((nn + 1) * 4)))
+      return rc;
+  }
+  const int ntile = (int)((n + BX_TILE - 1) / BX_TILE);
+  if ((rc = xyz4.ensure(nn * 16)) || (rc = box_tiles.ensure((size_t)ntile * BX_REC * 4)) ||
+      (rc = s.clscnt.ensure((MAXL + 1) * 8 * 4)))
+    return rc;
+  if (dense) k_dense_fill<<<(int)((D.off[lmax + 1] + TPB - 1) / TPB), TPB, 0, stream>>>(D);
   double rbinv = (double)((float)(1 << lmax) / rd);
   if ((rc = rs_counts.ensure((size_t)rs_count_words(n) * 4))) return rc;
   const int slot = rs_out_slot(3 * lmax);
@@ -744,67 +866,27 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   k_keys<<<nblk(n), TPB, 0, stream>>>(n, x, y, z, xmin, ymin, zmin, rd, rbinv, lmax, kb[0], nullptr,
                                       xyz4.as<float4>(), bad_flag.as<int>());
   radix_sort(n, 3 * lmax, kb, vb, rs_counts.as<uint32_t>(), stream);
-  size_t sb = 0;
-  FMMX_HIP(rocprim::exclusive_scan(nullptr, sb, (int*)nullptr, (int*)nullptr, 0, (size_t)n, rocprim::plus<int>(),
-                                   stream));
-  if ((rc = scratch.ensure(sb + 256))) return rc;
   const uint32_t* perm = s.perm.as<uint32_t>();
   k_gather_xyz<<<nblk(n), TPB, 0, stream>>>(n, perm, xyz4.as<float4>(), s.x.as<float>(), s.y.as<float>(),
                                             s.z.as<float>());
   if (s.has_normals)
     k_gather3<<<nblk(n), TPB, 0, stream>>>(n, perm, nx, ny, nz, s.nx.as<float>(), s.ny.as<float>(),
                                            s.nz.as<float>());
-  int* cnt = s.cnt.as<int>();
-  for (int lev = lmax; lev >= 2; lev--) {
-    if ((rc = s.flag[lev].ensure(nn * 4)) || (rc = s.pos[lev].ensure(nn * 4)) ||
-        (rc = s.bkey[lev].ensure(nn * 4)) || (rc = s.bfirst[lev].ensure(nn * 4)) ||
-        (rc = s.parent[lev].ensure(nn * 4)) || (rc = s.cfirst[lev].ensure((nn + 1) * 4)))
-      return rc;
+  // boxes of every level: keys, first particles, parents, first children,
+  // leaf last particles, dense tables and counts (two passes)
+  BoxOut o{};
+  for (int lev = 2; lev <= lmax; lev++) {
+    o.bkey[lev] = s.bkey[lev].as<uint32_t>();
+    o.bfirst[lev] = s.bfirst[lev].as<int>();
+    o.parent[lev] = s.parent[lev].as<int>();
+    o.cfirst[lev] = s.cfirst[lev].as<int>();
+    o.dense[lev] = dense ? s.dense[lev].as<int>() : nullptr;
   }
-  // box starts, numbers, keys and first particles of up to four levels per pass
-  if ((rc = lvl_f4.ensure(nn * 16)) || (rc = lvl_p4.ensure(nn * 16))) return rc;
-  size_t b4 = 0;
-  FMMX_HIP(rocprim::exclusive_scan(nullptr, b4, lvl_f4.as<int4>(), lvl_p4.as<int4>(), make_int4(0, 0, 0, 0),
-                                   (size_t)n, rocprim::plus<int4>(), stream));
-  if (b4 > scratch.bytes && (rc = scratch.ensure(b4 + 256))) return rc;
-  for (int l0 = 2; l0 <= lmax; l0 += 4) {
-    const int nl = std::min(4, lmax - l0 + 1);
-    LevelOut4 o{};
-    for (int k = 0; k < nl; k++) {
-      o.bkey[k] = s.bkey[l0 + k].as<uint32_t>();
-      o.bfirst[k] = s.bfirst[l0 + k].as<int>();
-      o.flag[k] = s.flag[l0 + k].as<int>();
-      o.pos[k] = s.pos[l0 + k].as<int>();
-      o.cnt[k] = cnt + l0 + k;
-    }
-    k_level_flags4<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, l0, nl, lvl_f4.as<int4>());
-    size_t b2 = scratch.bytes;
-    FMMX_HIP(rocprim::exclusive_scan(scratch.p, b2, lvl_f4.as<int4>(), lvl_p4.as<int4>(), make_int4(0, 0, 0, 0),
-                                     (size_t)n, rocprim::plus<int4>(), stream));
-    k_level_compact4<<<nblk(n), TPB, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, l0, nl, lvl_f4.as<int4>(),
-                                                  lvl_p4.as<int4>(), o);
-  }
-  // box-level kernels run over at most min(n, 8^lev) boxes (the counts stay on
-  // the device)
-  auto nbox = [&](int lev) { return (int)std::min<long>(n, 1L << (3 * lev)); };
-  for (int lev = lmax; lev >= 3; lev--) {
-    const int ub = nbox(lev);
-    k_parent<<<nblk(ub), TPB, 0, stream>>>(ub, cnt + lev, s.bfirst[lev].as<int>(), s.pos[lev - 1].as<int>(),
-                                           s.flag[lev - 1].as<int>(), s.parent[lev].as<int>());
-    k_cfirst<<<nblk(ub), TPB, 0, stream>>>(ub, cnt + lev, cnt + lev - 1, s.parent[lev].as<int>(),
-                                           s.cfirst[lev - 1].as<int>());
-  }
-  k_blast<<<nblk(nbox(lmax)), TPB, 0, stream>>>(nbox(lmax), cnt + lmax, s.bfirst[lmax].as<int>(), n,
-                                                s.blast.as<int>());
-  if (dense) {
-    for (int lev = 2; lev <= lmax; lev++) {
-      long nd = 1L << (3 * lev);
-      if ((rc = s.dense[lev].ensure(nd * 4))) return rc;
-      k_fill_int<<<nblk(nd), TPB, 0, stream>>>(nd, -1, s.dense[lev].as<int>());
-      k_dense<<<nblk(nbox(lev)), TPB, 0, stream>>>(nbox(lev), cnt + lev, s.bkey[lev].as<uint32_t>(),
-                                                   s.dense[lev].as<int>());
-    }
-  }
+  o.blast = s.blast.as<int>();
+  o.cnt = s.cnt.as<int>();
+  o.clscnt = s.clscnt.as<int>();
+  k_box_count<<<ntile, BX_T, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, box_tiles.as<int>());
+  k_box_place<<<ntile, BX_T, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, box_tiles.as<int>(), o);
   return FMMX_OK;
 }
 
@@ -905,8 +987,8 @@ int Engine::occupancy(int n, const float* x, const float* y, const float* z, lon
     const int g = (int)std::min<long>(1024, (nout + 255) / 256);
     k_occ_up<<<g, 256, 0, stream>>>(nout, lvl[l], lvl[l - 1], st + 2 * l);
   }
-  if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, (2 + 8 * (MAXL + 1)) * 4, hipHostMallocDefault));
-  static_assert(4 * (MAXL + 1) <= 2 + 8 * (MAXL + 1), "readback buffer");
+  if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, RB_INTS * 4, hipHostMallocDefault));
+  static_assert(4 * (MAXL + 1) <= RB_INTS, "readback buffer");
   FMMX_HIP(hipMemcpyAsync(rb_host, st, (size_t)2 * (OCC_L + 1) * 8, hipMemcpyDeviceToHost, stream));
   FMMX_HIP(hipStreamSynchronize(stream));
   const unsigned long long* h = (const unsigned long long*)rb_host;
