@@ -102,6 +102,7 @@ struct Engine {
   DevBuf xyz4;            // packed coordinates in caller order (tree build)
   // near list at the leaf level (targets): counts and rows of NNEAR
   DevBuf nij, neij, pair_ctr;
+  DevBuf pair_part;  // near-pair counts per workgroup of the list build
   // far lists per level (only with FMMX_KEEP_LISTS): rows of NEBM
   std::array<DevBuf, MAXL + 1> fnij, fneij;
   // M2L target ordering by parity class and class counts per level

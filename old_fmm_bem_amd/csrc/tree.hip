@@ -225,6 +225,7 @@ struct BoxOut {
 // per tile of points: [0, 7) boxes opened at levels 2..8, [8 + 8 (l - 2) + c)
 // of them in parity class c
 constexpr int BX_REC = 64;
+constexpr int BX_CG = 16;  // tiles per block in the class totals
 
 // parity class of a box, (x & 1) | (y & 1) << 1 | (z & 1) << 2, from its key
 __device__ __forceinline__ int box_class(uint32_t k) { return ((k >> 1) & 1) | ((k & 1) << 1) | (k & 4); }
@@ -273,9 +274,11 @@ __device__ __forceinline__ void bx_block(int v[7], int tot[7], bool red) {
 }
 
 __global__ __launch_bounds__(BX_T) void k_box_count(int n, const uint32_t* __restrict__ keys, int lmax,
-                                                    int* __restrict__ tcnt) {
+                                                    int* __restrict__ tcnt, int* __restrict__ clscnt) {
   __shared__ int scl[7 * 8];
   if (threadIdx.x < 7 * 8) scl[threadIdx.x] = 0;
+  // the class totals, which the next kernel accumulates
+  if (clscnt && blockIdx.x == 0 && threadIdx.x < 8 * (MAXL + 1)) clscnt[threadIdx.x] = 0;
   __syncthreads();
   int c[7] = {0, 0, 0, 0, 0, 0, 0}, tot[7];
   const long i0 = (long)blockIdx.x * BX_TILE + (long)threadIdx.x * BX_PT;
@@ -301,22 +304,18 @@ __global__ __launch_bounds__(BX_T) void k_box_count(int n, const uint32_t* __res
 
 __global__ __launch_bounds__(BX_T) void k_box_place(int n, const uint32_t* __restrict__ keys, int lmax,
                                                     const int* __restrict__ tcnt, BoxOut o) {
-  // the last tile's block also totals the parity classes of every tile
-  // (the counting pass has finished: no other block writes them)
-  if (o.clscnt && blockIdx.x == gridDim.x - 1) {
-    __shared__ int part[BX_T / BX_REC][BX_REC];
-    const int e = threadIdx.x % BX_REC, r = threadIdx.x / BX_REC;
+  // class totals: block k adds the class counts of tiles BX_CG k ..
+  // BX_CG k + BX_CG - 1 (written by the counting pass) to the totals
+  if (o.clscnt && (long)blockIdx.x * BX_CG < gridDim.x) {
+    const int e = threadIdx.x % BX_REC, r = threadIdx.x / BX_REC;  // 4 rows of 64 columns
     int sum = 0;
-    for (int t = r; t < (int)gridDim.x; t += BX_T / BX_REC) sum += tcnt[(size_t)t * BX_REC + e];
-    part[r][e] = sum;
-    __syncthreads();
-    if (threadIdx.x >= 8 && threadIdx.x < 8 + 7 * 8) {
-      int v = 0;
 #pragma unroll
-      for (int q = 0; q < BX_T / BX_REC; q++) v += part[q][threadIdx.x];
-      const int l = 2 + (threadIdx.x - 8) / 8;
-      if (l <= lmax) o.clscnt[8 * l + (threadIdx.x - 8) % 8] = v;
+    for (int u = 0; u < BX_CG / (BX_T / BX_REC); u++) {
+      const long t = (long)blockIdx.x * BX_CG + u * (BX_T / BX_REC) + r;
+      if (t < gridDim.x) sum += tcnt[t * BX_REC + e];
     }
+    const int l = 2 + (e - 8) / 8;
+    if (e >= 8 && l <= lmax && sum) atomicAdd(&o.clscnt[8 * l + (e - 8) % 8], sum);
   }
   // boxes opened by the earlier tiles, per level
   int pre[7] = {0, 0, 0, 0, 0, 0, 0}, tot[7];
@@ -401,32 +400,53 @@ __global__ void k_fill_int(long n, int v, int* __restrict__ a) {
 // and out-of-grid boxes skipped.  The reference clips to the bounding box of
 // the source boxes (:312-321); positions outside it hold no source box, so
 // the lists are identical.  Also counts particle pairs for the statistics.
+// 32 lanes per target box, lane k < 27 for the neighbour position k in the
+// reference's loop order (k = 9 (dx + 1) + 3 (dy + 1) + dz + 1); the boxes
+// found are compacted in lane order.
 __global__ void k_near_list(int nbi, const uint32_t* __restrict__ bkey, int lev, const int* __restrict__ dense_j,
                             const int* __restrict__ ifirst, const int* __restrict__ ilast,
                             const int* __restrict__ jfirst, const int* __restrict__ jlast, int* __restrict__ nij,
                             int* __restrict__ neij, unsigned long long* __restrict__ pairs, int gsh) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  long npair = 0;
-  if (b < nbi) {
-    int ix, iy, iz, g = 1 << lev, c = 0;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = (int)(t >> 5), k = (int)(t & 31), half = (int)(threadIdx.x & 32);
+  int jj = -1;
+  long ni = 0;
+  if (b < nbi && k < 27) {
+    int ix, iy, iz;
+    const int g = 1 << lev;
     morton_decode(bkey[b], lev, ix, iy, iz);
-    const long ni = ilast[b] - ifirst[b] + 1;
-    for (int jx = max(ix - 1, 0); jx <= min(ix + 1, g - 1); jx++)
-      for (int jy = max(iy - 1, 0); jy <= min(iy + 1, g - 1); jy++)
-        for (int jz = max(iz - 1, 0); jz <= min(iz + 1, g - 1); jz++) {
-          // groups: boxes in different cells of the group level do not interact
-          if (gsh >= 0 && ((jx ^ ix) >> gsh | (jy ^ iy) >> gsh | (jz ^ iz) >> gsh)) continue;
-          int jj = dense_j[morton_encode(jx, jy, jz, lev)];
-          if (jj >= 0) {
-            neij[b * NNEAR + c++] = jj;
-            npair += ni * (jlast[jj] - jfirst[jj] + 1);
-          }
-        }
-    nij[b] = c;
+    const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+    const bool in = (unsigned)jx < (unsigned)g && (unsigned)jy < (unsigned)g && (unsigned)jz < (unsigned)g &&
+                    // groups: boxes in different cells of the group level do not interact
+                    !(gsh >= 0 && ((jx ^ ix) >> gsh | (jy ^ iy) >> gsh | (jz ^ iz) >> gsh));
+    if (in) jj = dense_j[morton_encode(jx, jy, jz, lev)];
+    if (jj >= 0) ni = ilast[b] - ifirst[b] + 1;
   }
-  // one atomic per wave (same-address atomics serialise)
+  const unsigned long long found = __ballot(jj >= 0);
+  const uint32_t mine = (uint32_t)(found >> half);  // this box's 32 lanes
+  const int pos = __popc(mine & ((1u << k) - 1u));
+  if (jj >= 0) neij[(size_t)b * NNEAR + pos] = jj;
+  if (k == 0 && b < nbi) nij[b] = __popc(mine);
+  // pair count: one partial sum per workgroup (pairs[blockIdx.x]), summed
+  // by k_sum_u64 (same-address atomics from every wave serialise)
+  __shared__ long wsum[4];
+  long npair = jj >= 0 ? ni * (jlast[jj] - jfirst[jj] + 1) : 0;
   for (int o = 32; o > 0; o >>= 1) npair += __shfl_xor(npair, o);
-  if ((threadIdx.x & 63) == 0 && npair) atomicAdd(pairs, (unsigned long long)npair);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = npair;
+  __syncthreads();
+  if (threadIdx.x == 0) pairs[blockIdx.x] = (unsigned long long)(wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+}
+
+// out[0] = sum of a[0 .. n) (one workgroup of 256)
+__global__ __launch_bounds__(256) void k_sum_u64(int n, const unsigned long long* __restrict__ a,
+                                                 unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long ws[4];
+  unsigned long long v = 0;
+  for (int i = threadIdx.x; i < n; i += 256) v += a[i];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
 // fmm/ijbox.cxx:346-365 (neib 1, used at level 2) and :366-401 (neib 2)
@@ -885,7 +905,7 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   o.blast = s.blast.as<int>();
   o.cnt = s.cnt.as<int>();
   o.clscnt = s.clscnt.as<int>();
-  k_box_count<<<ntile, BX_T, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, box_tiles.as<int>());
+  k_box_count<<<ntile, BX_T, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, box_tiles.as<int>(), o.clscnt);
   k_box_place<<<ntile, BX_T, 0, stream>>>(n, s.keys.as<uint32_t>(), lmax, box_tiles.as<int>(), o);
   return FMMX_OK;
 }
@@ -899,12 +919,15 @@ int Engine::build_lists() {
   // [0] near pairs, [2 + lev] M2L pairs at level lev
   if ((rc = ctr.ensure(16 * 8))) return rc;
   k_zero_u64<<<1, 32, 0, stream>>>(ctr.as<unsigned long long>(), 16);
-  if (nbi > 0 && sj.nb[lmax] > 0)
-    k_near_list<<<nblk(nbi), TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
+  if (nbi > 0 && sj.nb[lmax] > 0) {
+    const int nblk_near = nblk(32L * nbi);
+    if ((rc = pair_part.ensure((size_t)nblk_near * 8))) return rc;
+    k_near_list<<<nblk_near, TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
                                                si.bfirst[lmax].as<int>(), si.blast.as<int>(),
                                                sj.bfirst[lmax].as<int>(), sj.blast.as<int>(), nij.as<int>(),
-                                               neij.as<int>(), ctr.as<unsigned long long>(), group_shift(lmax));
-  else if (nbi > 0)
+                                               neij.as<int>(), pair_part.as<unsigned long long>(), group_shift(lmax));
+    k_sum_u64<<<1, 256, 0, stream>>>(nblk_near, pair_part.as<unsigned long long>(), ctr.as<unsigned long long>());
+  } else if (nbi > 0)
     k_fill_int<<<nblk(nbi), TPB, 0, stream>>>(nbi, 0, nij.as<int>());
   return FMMX_OK;
 }
