@@ -17,6 +17,7 @@
 //   direct sums kernel/G_dir.cxx, Gni_dir.cxx, Gnj_dir.cxx.
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "engine.h"
 #include "m2l.h"
@@ -410,6 +411,107 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
     const int hh = c2 / ML::VS, c = c2 % ML::VS;
     if (b0 + hh < nb && (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2))
       M[(size_t)(b0 + hh) * mstride + moff + c] = T(0);
+  }
+}
+
+// P2M of charges in the EXACT convention, FP32 (the mixed engine's G, Gni):
+// each lane's particle terms stay in registers and the per-leaf column sums
+// go through LDS in chunks of CH columns, so a workgroup holds 8-17 KB of
+// LDS instead of a whole 64 x P^2 table (four to eight times the
+// workgroups per CU of k_p2m).  Same leaves-per-wave forms as k_p2m.
+template <int P, int HALF>
+__global__ __launch_bounds__(64) void k_p2m_exact(int nb, const uint32_t* __restrict__ bkey,
+                                                  const int* __restrict__ bfirst, const int* __restrict__ blast,
+                                                  const float* __restrict__ x, const float* __restrict__ y,
+                                                  const float* __restrict__ z, const float* __restrict__ q, int lev,
+                                                  double xmin, double ymin, double zmin, double rb,
+                                                  float* __restrict__ M, int mstride, int moff) {
+  constexpr int NR = P * (P + 1) / 2, P2 = P * P, NL = HALF ? 2 : 1, STEP = 64 / NL;
+  constexpr int CH = 64 / NL, NQ = (P2 + CH - 1) / CH;
+  __shared__ float red[64][CH + 1];
+  const int lane = threadIdx.x, h = HALF ? lane >> 5 : 0, sub = HALF ? lane & 31 : lane;
+  const int b = HALF ? 2 * (int)blockIdx.x + h : (int)blockIdx.x;
+  const bool valid = b < nb;
+  if (!HALF && !valid) return;
+  double cx = 0, cy = 0, cz = 0;
+  if (valid) box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
+  const double rbinv = 1.0 / rb;
+  float acc[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; k++) acc[k] = 0.f;
+  // column sums: lane -> (leaf cl, column cc of the chunk)
+  const int cl = lane / CH, cc = lane % CH;
+  const int i0 = valid ? bfirst[b] : 0, i1 = valid ? blast[b] : -1;
+  constexpr HarmCoef hc = make_coef();
+  for (int base = 0;; base += STEP) {
+    if (__ballot(i0 + base <= i1) == 0) break;  // one wave per block: uniform
+    const int i = i0 + base + sub;
+    float v[P2];
+#pragma unroll
+    for (int c = 0; c < P2; c++) v[c] = 0.f;
+    if (i <= i1) {
+      double ux, uy, uz;
+      eff_point(FMMX_CONV_EXACT, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
+      const float tx = (float)ux, ty = (float)uy, tz = (float)uz, r2 = tx * tx + ty * ty + tz * tz, qi = q[i];
+      auto put = [&](int n, int m, float sr, float si) {
+        v[sl(n, m)] = qi * sr;
+        if (m > 0) v[NR + n * (n - 1) / 2 + m - 1] = -qi * si;
+      };
+      float dr = 1.f, di = 0.f;
+#pragma unroll
+      for (int m = 0; m < P; m++) {
+        if (m > 0) {
+          const float a = -(float)hc.a[m], nr = a * (tx * dr - ty * di), ni = a * (tx * di + ty * dr);
+          dr = nr;
+          di = ni;
+        }
+        put(m, m, dr, di);
+        if (m + 1 < P) {
+          const float bz = (float)hc.b[m] * tz;
+          float pr = dr, pi = di, sr = bz * dr, si = bz * di;
+          put(m + 1, m, sr, si);
+#pragma unroll
+          for (int n = m + 2; n < P; n++) {
+            const float cz = (float)hc.c[n - 1][m] * tz, d = (float)hc.d[n - 1][m] * r2;
+            const float nr = cz * sr - d * pr, ni = cz * si - d * pi;
+            pr = sr;
+            pi = si;
+            sr = nr;
+            si = ni;
+            put(n, m, sr, si);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+#pragma unroll
+      for (int c = 0; c < CH; c++) red[lane][c] = k * CH + c < P2 ? v[k * CH + c] : 0.f;
+      __syncthreads();
+      float s0 = 0.f, s1 = 0.f;
+#pragma unroll 8
+      for (int r = 0; r < STEP; r += 2) {
+        s0 += red[cl * STEP + r][cc];
+        s1 += red[cl * STEP + r + 1][cc];
+      }
+      acc[k] += s0 + s1;
+      __syncthreads();
+    }
+  }
+  using ML = MLayout<P>;
+  const int b0 = HALF ? 2 * (int)blockIdx.x : (int)blockIdx.x;
+  if (b0 + cl < nb) {
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const int col = k * CH + cc;
+      if (col < P2) M[(size_t)(b0 + cl) * mstride + moff + ML::at(col)] = acc[k];
+    }
+  }
+  // zero the padding slots (M2M applies its operator to whole stored vectors)
+  for (int c2 = lane; c2 < NL * ML::VS; c2 += 64) {
+    const int hh = c2 / ML::VS, c = c2 % ML::VS;
+    if (b0 + hh < nb && (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2))
+      M[(size_t)(b0 + hh) * mstride + moff + c] = 0.f;
   }
 }
 
@@ -1938,7 +2040,12 @@ int Engine::run_p2m(int neq) {
 #define FMMX_P2M_H(T_, YK_, H_, Q_, STRIDE_, OFF_)                                                 \
   do {                                                                                            \
     if constexpr (YK_ == 0) {                                                                     \
-      if (neq != FMMX_GNJ && cfg.convention == FMMX_CONV_EXACT)                                   \
+      if (neq != FMMX_GNJ && cfg.convention == FMMX_CONV_EXACT && std::is_same_v<T_, float>)      \
+        k_p2m_exact<P, H_><<<H_ ? (nb + 1) / 2 : nb, 64, 0, stream>>>(                            \
+            nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(), sj.blast.as<int>(),      \
+            sj.x.as<float>(), sj.y.as<float>(), sj.z.as<float>(), Q_, lmax, xmin, ymin, zmin,     \
+            rb(lmax), (float*)mexp[lmax].as<T_>(), STRIDE_, OFF_);                                \
+      else if (neq != FMMX_GNJ && cfg.convention == FMMX_CONV_EXACT)                              \
         FMMX_P2M_HC(T_, 0, H_, FMMX_CONV_EXACT, 0, Q_, STRIDE_, OFF_);                            \
       else                                                                                        \
         FMMX_P2M_HC(T_, 0, H_, -1, -1, Q_, STRIDE_, OFF_);                                        \

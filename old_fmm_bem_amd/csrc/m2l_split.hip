@@ -82,14 +82,21 @@ __global__ __launch_bounds__(MS_T) void k_msplit_max(MSplitLevels S) {
   const long b = t >> 2;
   int k = 0;
   while (k + 1 < S.nl && b >= S.off[k + 1]) k++;
+  float mx = 0.f;
   if (b < S.off[S.nl] && b != S.off[k + 1] - 1) {  // not the zero box
     const int g = (int)(t & 3);
     const float* m = S.M[k] + (size_t)(b - S.off[k]) * ML::VS + g * ML::KSP;
-    float mx = 0.f;
 #pragma unroll
     for (int u = 0; u < ML::KS; u++)
       if (g * ML::KS + u < ML::P2) mx = fmaxf(mx, fabsf(m[u]));
-    // non-negative floats order as their bit patterns
+  }
+  // non-negative floats order as their bit patterns; a wave within one
+  // level reduces first (one LDS atomic instead of 64 on one address)
+  const int k0 = __builtin_amdgcn_readfirstlane(k);
+  if (__ballot(k != k0) == 0) {
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0 && mx > 0.f) atomicMax(&smax[k0], __float_as_uint(mx));
+  } else if (mx > 0.f) {
     atomicMax(&smax[k], __float_as_uint(mx));
   }
   __syncthreads();
@@ -98,39 +105,39 @@ __global__ __launch_bounds__(MS_T) void k_msplit_max(MSplitLevels S) {
 
 // the split pairs of every level: lane group g (of G) of box b holds
 // hi[u], lo[u] of input slot g L::KS + u scaled by 2^split_exp, in the
-// layout L (HLayout, 4 groups: 16x16x32; WLayout, 2 groups: 32x32x16)
+// layout L (HLayout, 4 groups: 16x16x32; WLayout, 2 groups: 32x32x16).  One
+// thread per 32-bit word of the output (hi pairs, lo pairs, zero padding),
+// so that both the multipole reads and the writes are contiguous across a
+// wave.
 template <int P, class L, int G>
 __global__ __launch_bounds__(MS_T) void k_msplit(MSplitLevels S) {
   using ML = MLayout<P>;
-  const long t = (long)blockIdx.x * MS_T + threadIdx.x;
-  const long b = t / G;
+  constexpr int JW = L::GH / 2;  // words per lane group
+  // 32-bit index arithmetic (the division by JW is by a constant)
+  const unsigned t = blockIdx.x * MS_T + threadIdx.x;
+  const unsigned bg = t / JW;
+  const int j = (int)(t % JW);
+  const long b = bg / G;
+  const int g = (int)(bg % G);
   if (b >= S.off[S.nl]) return;
   int k = 0;
   while (k + 1 < S.nl && b >= S.off[k + 1]) k++;
-  const int g = (int)(t % G);
-  const float sc = ldexpf(1.f, split_exp(S.mmax[k]));
   // the last box of a level's range is the zero box (read where a target
   // has no source box)
   const bool zero = b == S.off[k + 1] - 1;
-  const float* m = S.M[k] + (size_t)(zero ? 0 : b - S.off[k]) * ML::VS;
-  float x[L::SEG];
-#pragma unroll
-  for (int u = 0; u < L::SEG; u++) {
-    const int c = g * L::KS + u;
-    x[u] = (!zero && u < L::KS && c < ML::P2) ? m[ML::at(c)] * sc : 0.f;
+  uint32_t w = 0u;
+  if (!zero && j < L::SEG) {
+    const int u = 2 * (j % (L::SEG / 2));
+    const float sc = ldexpf(1.f, split_exp(S.mmax[k]));
+    const float* m = S.M[k] + (size_t)(b - S.off[k]) * ML::VS;
+    const int c0 = g * L::KS + u, c1 = c0 + 1;
+    const float x0 = (u < L::KS && c0 < ML::P2) ? m[ML::at(c0)] * sc : 0.f;
+    const float x1 = (u + 1 < L::KS && c1 < ML::P2) ? m[ML::at(c1)] * sc : 0.f;
+    const float h0 = (float)(_Float16)x0, h1 = (float)(_Float16)x1;
+    w = j < L::SEG / 2 ? pack_h2(h0, h1) : pack_h2(x0 - h0, x1 - h1);
   }
-  uint32_t w[L::GH / 2];
-#pragma unroll
-  for (int j = 0; j < L::GH / 2; j++) w[j] = 0u;
-#pragma unroll
-  for (int u = 0; u < L::SEG; u += 2) {
-    const float h0 = (float)(_Float16)x[u], h1 = (float)(_Float16)x[u + 1];
-    w[u / 2] = pack_h2(h0, h1);
-    w[L::SEG / 2 + u / 2] = pack_h2(x[u] - h0, x[u + 1] - h1);
-  }
-  uint4* dst = S.Mh[k] + ((size_t)(b - S.off[k]) * L::VSH + g * L::GH) / 8;
-#pragma unroll
-  for (int j = 0; j < L::NB4; j++) dst[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(S.Mh[k]);
+  dst[((size_t)(b - S.off[k]) * L::VSH + g * L::GH) / 2 + j] = w;
 }
 
 // B' fragment of k-step ks from a lane's loaded words: k-positions
@@ -870,8 +877,15 @@ int launch_msplit(int p, const MSplitLevels& S, hipStream_t stream) {
     constexpr int P = decltype(pc)::value;
     k_msplit_zero<<<1, 32, 0, stream>>>(S);
     k_msplit_max<P><<<nblk, MS_T, 0, stream>>>(S);
-    if (S.wide) k_msplit<P, WLayout<P>, 2><<<(int)((2 * S.off[S.nl] + MS_T - 1) / MS_T), MS_T, 0, stream>>>(S);
-    else k_msplit<P, HLayout<P>, 4><<<nblk, MS_T, 0, stream>>>(S);
+    if (S.off[S.nl] * 4 * (HLayout<P>::GH / 2) >= (1L << 32))
+      return fail(FMMX_EUNSUPPORTED, "too many boxes for the multipole split");
+    if (S.wide) {
+      const long nw = S.off[S.nl] * 2 * (WLayout<P>::GH / 2);
+      k_msplit<P, WLayout<P>, 2><<<(int)((nw + MS_T - 1) / MS_T), MS_T, 0, stream>>>(S);
+    } else {
+      const long nw = S.off[S.nl] * 4 * (HLayout<P>::GH / 2);
+      k_msplit<P, HLayout<P>, 4><<<(int)((nw + MS_T - 1) / MS_T), MS_T, 0, stream>>>(S);
+    }
     return hipGetLastError() == hipSuccess ? FMMX_OK : fail(FMMX_EDEVICE, "multipole split launch failed");
   });
 }
