@@ -231,7 +231,9 @@ int fmmx_graph_launch(fmmx_handle* h);
  * Put each problem in its own box of the level (e.g. surface s of a batch in
  * cell s of the 2^level grid).  level 0 turns it off.  No reference
  * counterpart: for callers with many small, uneven problems (batched BEM
- * solves).  Takes effect at the next fmmx_set_points (required). */
+ * solves).  Takes effect at the next fmmx_set_points (required), which
+ * fails with FMMX_EINVAL when an explicit lmax is below the level; an
+ * automatic lmax (0) is raised to it. */
 int fmmx_set_groups(fmmx_handle* h, int level);
 
 /* Wavenumber k > 0 of FMMX_HELMHOLTZ.  No reference counterpart (the

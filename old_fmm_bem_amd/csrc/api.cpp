@@ -500,7 +500,6 @@ int Engine::graph_launch() {
 Engine::~Engine() {
   drop_graph();
   if (rb_host) (void)hipHostFree(rb_host);
-  if (stats_host) (void)hipHostFree(stats_host);
   if (gstream) (void)hipStreamDestroy(gstream);
   for (auto& e : gev)
     if (e) (void)hipEventDestroy(e);
@@ -775,6 +774,9 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
     if ((rc = occupancy(nj > 0 ? nj : ni, d[o], d[o + 1], d[o + 2], occ_boxes, occ_sumsq))) return rc;
     lmax = auto_depth(occ_boxes, occ_sumsq, p, fp64);
   }
+  // an automatic depth reaches at least the group level (independent
+  // problems need their own boxes)
+  if (cfg.lmax <= 0) lmax = std::max(lmax, std::min(group_level, MAXL));
   if (group_level > lmax)
     return fail(FMMX_EINVAL, "group level " + std::to_string(group_level) + " is deeper than lmax " +
                                  std::to_string(lmax) + " (fmmx_set_groups)");
@@ -812,9 +814,6 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   FMMX_HIP(hipEventRecord(ev[1], stream));
   // statistics read when asked for (fmmx_get_stats), without a wait here:
   // the near pair count and the tree time
-  if (!stats_host) FMMX_HIP(hipHostMalloc((void**)&stats_host, 64, hipHostMallocDefault));
-  stats_host[0] = 0;
-  if (si.nb[lmax] > 0) FMMX_HIP(hipMemcpyAsync(stats_host, pair_ctr.p, 8, hipMemcpyDeviceToHost, stream));
   tree_stats_pending = true;
   have_points = true;
   last_neq = -1;
@@ -826,7 +825,13 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
 int Engine::resolve_tree_stats() {
   if (!tree_stats_pending) return FMMX_OK;
   FMMX_HIP(hipEventSynchronize(ev[1]));
-  p2p_pairs = (int64_t)stats_host[0];
+  p2p_pairs = 0;
+  if (near_nblk > 0) {
+    std::vector<unsigned long long> part((size_t)near_nblk);
+    FMMX_HIP(hipMemcpyAsync(part.data(), pair_part.p, part.size() * 8, hipMemcpyDeviceToHost, stream));
+    FMMX_HIP(hipStreamSynchronize(stream));
+    for (unsigned long long v : part) p2p_pairs += (int64_t)v;
+  }
   float ms = 0;
   FMMX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   stats.ms_tree = ms;
@@ -1414,7 +1419,8 @@ int fmmx_get_stats(fmmx_handle* h, fmmx_stats* st) {
   }
   s.p2p_pairs = e.p2p_pairs;
   unsigned long long mp[16] = {};
-  if (e.pair_ctr.p && hipMemcpy(mp, e.pair_ctr.p, sizeof(mp), hipMemcpyDeviceToHost) != hipSuccess)
+  // the M2L pair counters belong to the last evaluation on this tree
+  if (e.last_neq >= 0 && e.pair_ctr.p && hipMemcpy(mp, e.pair_ctr.p, sizeof(mp), hipMemcpyDeviceToHost) != hipSuccess)
     return fmmx::fail(FMMX_EDEVICE, "hipMemcpy D2H failed");
   s.m2l_pairs = 0;
   for (int l = 2; l <= e.lmax; l++) s.m2l_pairs += (int64_t)mp[2 + l];

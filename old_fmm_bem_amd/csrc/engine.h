@@ -103,6 +103,7 @@ struct Engine {
   // near list at the leaf level (targets): counts and rows of NNEAR
   DevBuf nij, neij, pair_ctr;
   DevBuf pair_part;  // near-pair counts per workgroup of the list build
+  int near_nblk = 0;  // workgroups of the last near-list build (entries of pair_part)
   // far lists per level (only with FMMX_KEEP_LISTS): rows of NEBM
   std::array<DevBuf, MAXL + 1> fnij, fneij;
   // M2L target ordering by parity class and class counts per level
@@ -129,7 +130,6 @@ struct Engine {
   DevBuf corr_d;  // table ratio per packed (n,m) (1 in EXACT)
   DevBuf rb_dev;            // counters gathered for one readback
   int* rb_host = nullptr;   // pinned host copy of rb_dev (RB_INTS ints)
-  unsigned long long* stats_host = nullptr;  // pinned: near pair count of the last set_points
   bool tree_stats_pending = false;           // p2p_pairs and ms_tree not yet read
   int resolve_tree_stats();
   int ops_lmax = -1;

@@ -387,10 +387,6 @@ __global__ void k_dense_fill(DenseLevels D) {
   D.dense[l][t - D.off[l]] = -1;
 }
 
-__global__ void k_zero_u64(unsigned long long* a, int n) {
-  if ((int)threadIdx.x < n) a[threadIdx.x] = 0;
-}
-
 __global__ void k_fill_int(long n, int v, int* __restrict__ a) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -427,26 +423,14 @@ __global__ void k_near_list(int nbi, const uint32_t* __restrict__ bkey, int lev,
   const int pos = __popc(mine & ((1u << k) - 1u));
   if (jj >= 0) neij[(size_t)b * NNEAR + pos] = jj;
   if (k == 0 && b < nbi) nij[b] = __popc(mine);
-  // pair count: one partial sum per workgroup (pairs[blockIdx.x]), summed
-  // by k_sum_u64 (same-address atomics from every wave serialise)
+  // pair count: one partial sum per workgroup (pairs[blockIdx.x]; a single
+  // counter would take same-address atomics from every wave, which serialise)
   __shared__ long wsum[4];
   long npair = jj >= 0 ? ni * (jlast[jj] - jfirst[jj] + 1) : 0;
   for (int o = 32; o > 0; o >>= 1) npair += __shfl_xor(npair, o);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = npair;
   __syncthreads();
   if (threadIdx.x == 0) pairs[blockIdx.x] = (unsigned long long)(wsum[0] + wsum[1] + wsum[2] + wsum[3]);
-}
-
-// out[0] = sum of a[0 .. n) (one workgroup of 256)
-__global__ __launch_bounds__(256) void k_sum_u64(int n, const unsigned long long* __restrict__ a,
-                                                 unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long ws[4];
-  unsigned long long v = 0;
-  for (int i = threadIdx.x; i < n; i += 256) v += a[i];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) out[0] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
 // fmm/ijbox.cxx:346-365 (neib 1, used at level 2) and :366-401 (neib 2)
@@ -915,18 +899,18 @@ int Engine::build_lists() {
   int nbi = si.nb[lmax];
   size_t nb1 = (size_t)(nbi > 0 ? nbi : 1);
   if ((rc = nij.ensure(nb1 * 4)) || (rc = neij.ensure(nb1 * NNEAR * 4))) return rc;
-  DevBuf& ctr = pair_ctr;
-  // [0] near pairs, [2 + lev] M2L pairs at level lev
-  if ((rc = ctr.ensure(16 * 8))) return rc;
-  k_zero_u64<<<1, 32, 0, stream>>>(ctr.as<unsigned long long>(), 16);
+  // [2 + lev]: M2L pairs at level lev (zeroed by each evaluation)
+  if ((rc = pair_ctr.ensure(16 * 8))) return rc;
+  // the near-pair count stays as one partial sum per workgroup, summed on
+  // the host when statistics are asked for
+  near_nblk = 0;
   if (nbi > 0 && sj.nb[lmax] > 0) {
-    const int nblk_near = nblk(32L * nbi);
-    if ((rc = pair_part.ensure((size_t)nblk_near * 8))) return rc;
-    k_near_list<<<nblk_near, TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
+    near_nblk = nblk(32L * nbi);
+    if ((rc = pair_part.ensure((size_t)near_nblk * 8))) return rc;
+    k_near_list<<<near_nblk, TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
                                                si.bfirst[lmax].as<int>(), si.blast.as<int>(),
                                                sj.bfirst[lmax].as<int>(), sj.blast.as<int>(), nij.as<int>(),
                                                neij.as<int>(), pair_part.as<unsigned long long>(), group_shift(lmax));
-    k_sum_u64<<<1, 256, 0, stream>>>(nblk_near, pair_part.as<unsigned long long>(), ctr.as<unsigned long long>());
   } else if (nbi > 0)
     k_fill_int<<<nblk(nbi), TPB, 0, stream>>>(nbi, 0, nij.as<int>());
   return FMMX_OK;
