@@ -341,19 +341,32 @@ class FMM:
 
     def set_points(self, xi, xj=None, ni_normals=None, nj_normals=None, keep_lists=False):
         """Targets xi (n,3), sources xj (m,3) (default: the same points, which
-        then share one tree)."""
+        then share one tree).  Each array may also be a sequence of its three
+        columns (x, y, z), the engine's own layout, which is then used
+        without a copy when contiguous float32."""
         same = xj is None or xj is xi
         if xj is None:
             xj = xi
-        dev = _is_torch(xi) and xi.is_cuda
+
+        def columns(a):
+            if a is None:
+                return [None] * 3
+            if isinstance(a, (tuple, list)):
+                if len(a) != 3:
+                    raise ValueError("a sequence of columns needs 3 entries (x, y, z)")
+                return list(a)
+            return [a[:, k] for k in range(3)]
+
+        ci, cj = columns(xi), columns(xj)
+        dev = _is_torch(ci[0]) and ci[0].is_cuda
         self._device = dev
-        cols = [_Arr(xi[:, k], dev) for k in range(3)]
-        cols += [_Arr(None if ni_normals is None else ni_normals[:, k], dev) for k in range(3)]
+        cols = [_Arr(c, dev) for c in ci]
+        cols += [_Arr(c, dev) for c in columns(ni_normals)]
         # the same column buffers for both sides: the engine then builds one tree
-        cols += cols[:3] if same else [_Arr(xj[:, k], dev) for k in range(3)]
-        cols += [_Arr(None if nj_normals is None else nj_normals[:, k], dev) for k in range(3)]
+        cols += cols[:3] if same else [_Arr(c, dev) for c in cj]
+        cols += [_Arr(c, dev) for c in columns(nj_normals)]
         self._cols = cols
-        self.ni, self.nj = int(xi.shape[0]), int(xj.shape[0])
+        self.ni, self.nj = int(ci[0].shape[0]), int(cj[0].shape[0])
         flags = (1 if dev else 0) | (2 if keep_lists else 0)
         _check(lib().fmmx_set_points(self.h, self.ni, *[c.ptr for c in cols[:6]], self.nj,
                                      *[c.ptr for c in cols[6:]], flags))

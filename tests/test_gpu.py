@@ -335,6 +335,34 @@ def test_invalid_coordinates_and_groups():
     f.set_groups(2)
     f.set_points(x)
     f.close()
+    # an automatic depth is raised to the group level
+    a = F.FMM(p=4, lmax=0, precision="fp64", convention="exact")
+    a.set_groups(5)
+    a.set_points(x)
+    assert a.lmax >= 5
+    a.close()
+
+
+def test_column_inputs_match_rows():
+    """set_points with the coordinates as three columns (the engine's own
+    layout, no copy) gives the tree and results of the (n, 3) form, bit for
+    bit, for host and device arrays."""
+    import torch
+
+    x, q = uniform_cube(30000, 77)
+    f = F.FMM(p=8, lmax=4, precision="mixed", convention="exact", box=(0, 0, 0, 1))
+    f.set_points(x)
+    v0, g0 = f.evaluate(q)
+    f.set_points(tuple(np.ascontiguousarray(x[:, k]) for k in range(3)))
+    v1, g1 = f.evaluate(q)
+    assert np.array_equal(v0, v1) and np.array_equal(g0, g1)
+    xd = torch.from_numpy(x).cuda()
+    f.set_points(tuple(xd[:, k].contiguous() for k in range(3)))
+    v2, g2 = f.evaluate(torch.from_numpy(q).cuda())
+    assert np.array_equal(v0, v2.cpu().numpy()) and np.array_equal(g0, g2.cpu().numpy())
+    with pytest.raises(ValueError):
+        f.set_points((xd[:, 0], xd[:, 1]))
+    f.close()
 
 
 def test_device_pointers_match_host():
