@@ -787,10 +787,15 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   // targets = sources: one tree, whose dense tables the source side uses
   if ((rc = build_side(si, ni, d[0], d[1], d[2], d[3], d[4], d[5], same))) return rc;
   if ((rc = build_side(sj, nj, d[6], d[7], d[8], d[9], d[10], d[11], true, same ? &si : nullptr))) return rc;
+  // near lists and parity-class order on upper-bound grids (the counts are
+  // read on the device), queued ahead of the readback so that the GPU runs
+  // them while the host waits
+  if ((rc = build_lists())) return rc;
+  if ((rc = build_m2l_order())) return rc;
   // the one readback of the tree build: box counts of every level (buffer
-  // sizes and grids of the lists and phases), the target boxes per parity
-  // class (M2L launch shapes) and the out-of-cube flag; pinned, since small
-  // pageable copies cost tens of us each
+  // sizes and grids of the phases), the target boxes per parity class (M2L
+  // launch shapes) and the out-of-cube flag; pinned, since small pageable
+  // copies cost tens of us each
   if ((rc = rb_dev.ensure(RB_INTS * 4))) return rc;
   if (!rb_host) FMMX_HIP(hipHostMalloc((void**)&rb_host, RB_INTS * 4, hipHostMallocDefault));
   CountSrc cs{};
@@ -808,8 +813,6 @@ int Engine::set_points(int ni, const float* xi, const float* yi, const float* zi
   }
   for (int l = 2; l <= MAXL; l++)
     for (int c = 0; c < 8; c++) cls_cnt_h[l][c] = l <= lmax ? rb_host[RB_CLS + 8 * l + c] : 0;
-  if ((rc = build_lists())) return rc;
-  if ((rc = build_m2l_order())) return rc;
   if (keep_lists && (rc = build_far_lists())) return rc;
   FMMX_HIP(hipEventRecord(ev[1], stream));
   // statistics read when asked for (fmmx_get_stats), without a wait here:

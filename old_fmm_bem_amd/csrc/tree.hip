@@ -399,10 +399,13 @@ __global__ void k_fill_int(long n, int v, int* __restrict__ a) {
 // 32 lanes per target box, lane k < 27 for the neighbour position k in the
 // reference's loop order (k = 9 (dx + 1) + 3 (dy + 1) + dz + 1); the boxes
 // found are compacted in lane order.
-__global__ void k_near_list(int nbi, const uint32_t* __restrict__ bkey, int lev, const int* __restrict__ dense_j,
-                            const int* __restrict__ ifirst, const int* __restrict__ ilast,
-                            const int* __restrict__ jfirst, const int* __restrict__ jlast, int* __restrict__ nij,
-                            int* __restrict__ neij, unsigned long long* __restrict__ pairs, int gsh) {
+__global__ void k_near_list(const int* __restrict__ nbi_d, const uint32_t* __restrict__ bkey, int lev,
+                            const int* __restrict__ dense_j, const int* __restrict__ ifirst,
+                            const int* __restrict__ ilast, const int* __restrict__ jfirst,
+                            const int* __restrict__ jlast, int* __restrict__ nij, int* __restrict__ neij,
+                            unsigned long long* __restrict__ pairs, int gsh) {
+  // the grid covers an upper bound of the target boxes; their count is on the device
+  const int nbi = *nbi_d;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int b = (int)(t >> 5), k = (int)(t & 31), half = (int)(threadIdx.x & 32);
   int jj = -1;
@@ -477,7 +480,7 @@ struct ClassLevels {
   const uint32_t* key[MAXL + 1];
   uint32_t* ord[MAXL + 1];
   int* cnt[MAXL + 1];
-  int nb[MAXL + 1];
+  const int* nb;         // boxes per level (device; the chunks cover an upper bound)
   int chunk0[MAXL + 2];  // first chunk of each level
   int lmin, lmax;
   int* ccnt;  // 8 class counts per chunk
@@ -491,12 +494,12 @@ __device__ __forceinline__ void class_chunk(const ClassLevels& L, int& lev, int&
 
 // the classes of a thread's boxes (-1 past the end) and their packed counts
 __device__ __forceinline__ void class_load(const ClassLevels& L, int lev, int ch, int* c, uint32_t* w) {
-  const int b0 = ch * CL_CH + (int)threadIdx.x * CL_PER;
+  const int b0 = ch * CL_CH + (int)threadIdx.x * CL_PER, nb = L.nb[lev];
   w[0] = w[1] = w[2] = w[3] = 0;
 #pragma unroll
   for (int k = 0; k < CL_PER; k++) {
     c[k] = -1;
-    if (b0 + k < L.nb[lev]) {
+    if (b0 + k < nb) {
       int x, y, z;
       morton_decode(L.key[lev][b0 + k], lev, x, y, z);
       c[k] = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
@@ -894,25 +897,30 @@ int Engine::build_side(Side& s, int n, const float* x, const float* y, const flo
   return FMMX_OK;
 }
 
+// The near lists and the parity-class order are launched before the box
+// counts reach the host: grids and buffers cover min(N, 8^lev) boxes per
+// level and the kernels read the counts from the device.
 int Engine::build_lists() {
   int rc;
-  int nbi = si.nb[lmax];
-  size_t nb1 = (size_t)(nbi > 0 ? nbi : 1);
+  const int ni = si.n, nj = sj.n;
+  const int ub = (int)std::min<long>(ni, 1L << (3 * lmax));
+  size_t nb1 = (size_t)(ub > 0 ? ub : 1);
   if ((rc = nij.ensure(nb1 * 4)) || (rc = neij.ensure(nb1 * NNEAR * 4))) return rc;
   // [2 + lev]: M2L pairs at level lev (zeroed by each evaluation)
   if ((rc = pair_ctr.ensure(16 * 8))) return rc;
   // the near-pair count stays as one partial sum per workgroup, summed on
   // the host when statistics are asked for
   near_nblk = 0;
-  if (nbi > 0 && sj.nb[lmax] > 0) {
-    near_nblk = nblk(32L * nbi);
+  if (ub > 0 && nj > 0) {
+    near_nblk = nblk(32L * ub);
     if ((rc = pair_part.ensure((size_t)near_nblk * 8))) return rc;
-    k_near_list<<<near_nblk, TPB, 0, stream>>>(nbi, si.bkey[lmax].as<uint32_t>(), lmax, sj.dense[lmax].as<int>(),
-                                               si.bfirst[lmax].as<int>(), si.blast.as<int>(),
-                                               sj.bfirst[lmax].as<int>(), sj.blast.as<int>(), nij.as<int>(),
-                                               neij.as<int>(), pair_part.as<unsigned long long>(), group_shift(lmax));
-  } else if (nbi > 0)
-    k_fill_int<<<nblk(nbi), TPB, 0, stream>>>(nbi, 0, nij.as<int>());
+    k_near_list<<<near_nblk, TPB, 0, stream>>>(si.cnt.as<int>() + lmax, si.bkey[lmax].as<uint32_t>(), lmax,
+                                               sj.dense[lmax].as<int>(), si.bfirst[lmax].as<int>(),
+                                               si.blast.as<int>(), sj.bfirst[lmax].as<int>(), sj.blast.as<int>(),
+                                               nij.as<int>(), neij.as<int>(), pair_part.as<unsigned long long>(),
+                                               group_shift(lmax));
+  } else if (ub > 0)
+    k_fill_int<<<nblk(ub), TPB, 0, stream>>>(ub, 0, nij.as<int>());
   return FMMX_OK;
 }
 
@@ -939,18 +947,18 @@ int Engine::build_m2l_order() {
   ClassLevels L{};
   L.lmin = 2;
   L.lmax = lmax;
+  L.nb = si.n > 0 ? si.cnt.as<int>() : nullptr;
   int nch = 0;
   for (int lev = 2; lev <= lmax; lev++) {
-    const int nb = si.nb[lev];
-    size_t nb1 = (size_t)(nb > 0 ? nb : 1);
+    const int ub = (int)std::min<long>(si.n, 1L << (3 * lev));
+    size_t nb1 = (size_t)(ub > 0 ? ub : 1);
     if ((rc = cls_ord[lev].ensure(nb1 * 4)) || (rc = cls_cnt[lev].ensure(8 * 4))) return rc;
-    if (nb == 0) k_fill_int<<<1, 32, 0, stream>>>(8, 0, cls_cnt[lev].as<int>());
+    if (ub == 0) k_fill_int<<<1, 32, 0, stream>>>(8, 0, cls_cnt[lev].as<int>());
     L.key[lev] = si.bkey[lev].as<uint32_t>();
     L.ord[lev] = cls_ord[lev].as<uint32_t>();
     L.cnt[lev] = cls_cnt[lev].as<int>();
-    L.nb[lev] = nb;
     L.chunk0[lev] = nch;
-    nch += (nb + CL_CH - 1) / CL_CH;
+    nch += (ub + CL_CH - 1) / CL_CH;
   }
   L.chunk0[lmax + 1] = nch;
   if (nch == 0) return FMMX_OK;
