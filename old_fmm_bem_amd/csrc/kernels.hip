@@ -1812,23 +1812,60 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
       }
     }
     double ap = 0, ax = 0, ay = 0, az = 0;
-    for (int e = 0; e < nn; e++) {
-      const int jj = neij[b * NNEAR + e];
-      const int j0 = jfirst[jj], j1 = jlast[jj];
-      for (int jb = j0; jb <= j1; jb += BS) {
-        const int js = jb + tid;
+    const uint32_t kb = ikey[b];
+    for (int e = 0; e < nn;) {
+      // one staging round: near boxes e .. e + nbox - 1 packed into the BS
+      // slots when they fit together (fewer barriers and shorter tails for
+      // small leaves), else box e's sources BS at a time
+      int jj = neij[b * NNEAR + e];
+      int j0 = jfirst[jj], j1 = jlast[jj];
+      int nbox = 1, cnt0 = j1 - j0 + 1, tot = cnt0;
+      int s0[4] = {j0, 0, 0, 0}, c0[4] = {cnt0, 0, 0, 0};
+      bool self = jkey[jj] == kb, packing = cnt0 <= BS;
+#pragma unroll
+      for (int k = 1; k < 4; k++) {
+        if (packing && e + k < nn) {
+          const int jn = neij[b * NNEAR + e + k];
+          const int a0 = jfirst[jn], c = jlast[jn] - a0 + 1;
+          if (tot + c <= BS) {
+            s0[k] = a0;
+            c0[k] = c;
+            tot += c;
+            self = self || jkey[jn] == kb;
+            nbox = k + 1;
+          } else {
+            packing = false;
+          }
+        }
+      }
+      const int rounds = cnt0 <= BS ? 1 : (cnt0 + BS - 1) / BS;
+      for (int rr = 0; rr < rounds; rr++) {
+        int js = -1;
+        if (cnt0 <= BS) {
+          // slot tid -> (packed box, offset)
+          int t = tid;
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            if (js < 0 && k < nbox) {
+              if (t < c0[k]) js = s0[k] + t;
+              else t -= c0[k];
+            }
+          }
+        } else if (rr * BS + tid < cnt0) {
+          js = j0 + rr * BS + tid;
+        }
         __syncthreads();
-        if (js <= j1) {
+        if (js >= 0) {
           sp[tid] = make_float4(xj[js], yj[js], zj[js], q[js]);
           if (NEQ == FMMX_GNJ) sn[tid] = make_float4(nxj[js], nyj[js], nzj[js], 0.f);
           if (HH) sn[tid] = make_float4(nxj[js], 0.f, 0.f, 0.f);
         }
         __syncthreads();
-        const int cnt = min(BS, j1 - jb + 1);
+        const int cnt = cnt0 <= BS ? tot : min(BS, cnt0 - rr * BS);
         T p = T(0), fx = T(0), fy = T(0), fz = T(0);
         int s = part;
         // coincident points can only share a box key; other box pairs skip the test
-        if (CONV == FMMX_CONV_EXACT && !(jkey[jj] == ikey[b])) {
+        if (CONV == FMMX_CONV_EXACT && !self) {
           for (; s + 3 * TPT < cnt; s += 4 * TPT) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -1854,6 +1891,7 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
           az += (double)fz;
         }
       }
+      e += nbox;
     }
     if (TPT == 2) {
       // the two lanes of a target are tid and tid + 32 of the one wave

@@ -343,6 +343,41 @@ def test_invalid_coordinates_and_groups():
     a.close()
 
 
+def _near_pairs_np(x, box, lev):
+    """Particle pairs over the 3x3x3 box neighbourhoods of the leaf boxes."""
+    lo = np.asarray(box[:3], np.float32)
+    scale = np.float64(np.float32(1 << lev) / np.float32(box[3]))
+    c = np.floor((x - lo).astype(np.float32).astype(np.float64) * scale).astype(np.int64).clip(0, (1 << lev) - 1)
+    g = 1 << lev
+    cnt = np.zeros((g + 2, g + 2, g + 2), np.int64)
+    np.add.at(cnt, (c[:, 0] + 1, c[:, 1] + 1, c[:, 2] + 1), 1)
+    nb = sum(np.roll(np.roll(np.roll(cnt, dx, 0), dy, 1), dz, 2)
+             for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1))
+    return int((cnt * nb).sum())
+
+
+@pytest.mark.parametrize("dist", ["cube", "plummer"])
+def test_statistics_after_set_points(dist):
+    """The tree statistics are read when asked for: after set_points alone
+    the near-pair count equals the pairs of the 3x3x3 neighbourhoods counted
+    in numpy, no M2L pairs are reported yet, and after an evaluation the M2L
+    pair count is that evaluation's."""
+    from old_fmm_bem_amd import workloads as W
+
+    n = 100_000
+    x = W.cube(n)[0] if dist == "cube" else W.plummer(n)[0]
+    f = F.FMM(p=6, lmax=4, precision="mixed", convention="exact")
+    f.set_points(x)
+    lm, box = f.domain()
+    st = f.stats()
+    assert st["p2p_pairs"] == _near_pairs_np(x, box, lm)
+    assert st["m2l_pairs"] == 0
+    f.evaluate(np.ones(n, np.float32))
+    st2 = f.stats()
+    assert st2["p2p_pairs"] == st["p2p_pairs"] and st2["m2l_pairs"] > 0
+    f.close()
+
+
 def test_column_inputs_match_rows():
     """set_points with the coordinates as three columns (the engine's own
     layout, no copy) gives the tree and results of the (n, 3) form, bit for
