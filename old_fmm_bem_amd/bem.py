@@ -525,6 +525,8 @@ def gmres_batched(matvec, b, seg_id, nseg, precond=None, rtol=1e-6, restart=30, 
     gpu = dev.type == "cuda"
     M = precond or (lambda v: v)
     n, C = b.shape[0], int(chunk)
+    if n > 1 and not bool((seg_id[1:] >= seg_id[:-1]).all()):
+        raise ValueError("gmres_batched needs every segment contiguous and in order (seg_id non-decreasing)")
     counts = torch.bincount(seg_id, minlength=nseg).cpu().numpy().astype(np.int64)
     nch = (counts + C - 1) // C
     nc = int(nch.sum())

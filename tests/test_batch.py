@@ -58,6 +58,12 @@ def test_gmres_batched_block_solves_cpu():
     _check_gmres_batched("cpu")
 
 
+def test_gmres_batched_rejects_interleaved_segments():
+    seg = torch.tensor([0, 1, 0, 1])
+    with pytest.raises(ValueError):
+        bem.gmres_batched(lambda v: v, torch.ones(4, dtype=torch.float64), seg, 2)
+
+
 # ---- GPU ----------------------------------------------------------------------
 # torch sees the device first (the batched solver keeps its vectors there)
 HAVE_GPU = torch.cuda.is_available()
