@@ -69,7 +69,10 @@ enum {
 };
 
 typedef struct fmmx_config {
-  int p;          /* expansion degrees n = 0..p-1 (the reference's mp), 2..FMMX_MAX_P */
+  int p;          /* expansion degrees n = 0..p-1 (the reference's mp):
+                     2..12 in every precision and convention, and 18 (the
+                     reference's other compiled order, kernel/precalch.cxx:185)
+                     with FMMX_FP64 and FMMX_CONV_EXACT */
   int lmax;       /* leaf level >= 2; 0 = automatic: FMMX_CONV_REF the
                      fmm/nlevel.cxx thresholds on nj, FMMX_CONV_EXACT the
                      measured cost model of fmmx_auto_lmax */
@@ -80,7 +83,7 @@ typedef struct fmmx_config {
                                  bio/main.cxx:364-380 does */
 } fmmx_config;
 
-#define FMMX_MAX_P 12
+#define FMMX_MAX_P 18
 
 typedef struct fmmx_handle fmmx_handle;
 
@@ -304,7 +307,8 @@ int fmmx_get_domain(fmmx_handle* h, int* lmax, float* box4);
  * statistics of one point set at levels l = 0..8 of its root cube:
  * boxes[l] = boxes holding at least one point, sumsq[l] = sum over boxes of
  * the squared point count.  The model's far-field/near-field cost ratio was
- * fitted per order p and precision to evaluation times measured on MI355X.
+ * fitted per order p (2..12) and precision to evaluation times measured on
+ * MI355X; at p = 18 (FP64 only) it is the p = 12 ratio scaled by (p/12)^4.
  * No counterpart in the reference, whose depth is fmm/nlevel.cxx:8 (kept
  * under FMMX_CONV_REF).  The level goes to *lmax. */
 int fmmx_auto_lmax(const long long* boxes, const double* sumsq, int p, int precision, int* lmax);
@@ -344,6 +348,8 @@ int fmmx_get_locals(fmmx_handle* h, int lev, double* ax_reim);
  * multipole basis k_n(lambda r) Y_n^m and local basis i_n(lambda r) Y_n^m:
  * kind 3: M2M from child octant a0; 4: L2L to child octant a0; 5: M2L for
  * offset (a0,a1,a2).  The convention does not enter kinds 3-5.
+ * p = 2..FMMX_MAX_P (any order on the host; FMMX_CONV_REF up to 12, where its
+ * float-stored tables still hold the factorials).
  * Helmholtz (FMMX_HELMHOLTZ), where rb is k times the box size (of the child
  * for 6 and 7): (2 p^2) x (2 p^2) operators on split expansions, the real
  * slots of the Re part then of the Im part (multipole basis

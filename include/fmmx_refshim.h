@@ -41,7 +41,9 @@
  * called before the first shim call to take effect; the environment
  * variables FMMX_REF_P, FMMX_REF_PRECISION (fp64|mixed),
  * FMMX_REF_CONVENTION (ref|exact) and FMMX_REF_DEVICE set the same values
- * for unchanged binaries.  Returns FMMX_OK or FMMX_EINVAL. */
+ * for unchanged binaries.  p is 2..12, or 18 (a caller compiled with
+ * mp = 18) with FMMX_FP64 and FMMX_CONV_EXACT.  Returns FMMX_OK,
+ * FMMX_EINVAL or (p = 18 otherwise) FMMX_EUNSUPPORTED. */
 extern "C" int fmmx_ref_configure(int p, int precision, int convention, int device);
 /* The shim's handle (created on first use), e.g. for fmmx_get_stats. */
 extern "C" fmmx_handle* fmmx_ref_handle(void);

@@ -161,7 +161,10 @@ int auto_depth(const long boxes[MAXL + 1], const double sumsq[MAXL + 1], int p, 
   const double* rt = fp64 ? r_fp64 : r_mixed;
   const double u = std::min(std::max((p - 4) / 2.0, 0.0), 4.0);
   const int k = std::min((int)u, 3);
-  const double r = rt[k] * std::pow(rt[k + 1] / rt[k], u - k);
+  double r = rt[k] * std::pow(rt[k + 1] / rt[k], u - k);
+  // order 18 (FP64): not fitted; the far field's cost per box grows as p^4
+  // (M2L) while a near pair's does not
+  if (p > 12) r *= std::pow(p / 12.0, 4);
   const double n = (double)boxes[0] > 0 ? std::sqrt(sumsq[0]) : 0.0;  // one box holds all points
   int best = -1, lo = 2, hi = MAXL;
   while (lo < MAXL && boxes[lo] > 0 && n / (double)boxes[lo] > 2000) lo++;
@@ -244,18 +247,20 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
   const int KS = (P2 + 3) / 4, KSP = (KS + 3) / 4 * 4, VS = 4 * KSP;
   const size_t vq = (size_t)VS * VS;
   S->m2m.assign(8 * vq, 0.0);
-  const size_t fr = (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64;
+  // MFMA fragment order for the FP32 engine (orders up to P_ALL)
+  const bool frag = T.p <= P_ALL;
+  const size_t fr = frag ? (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64 : 0;
   S->m2m_frag.resize(8 * fr);
   S->l2l_frag.resize(8 * fr);
   for (int o = 0; o < 8; o++) {
     op_m2m(T, o, A.data());
-    fragment_into(A.data(), P2, &S->m2m_frag[o * fr]);
+    if (frag) fragment_into(A.data(), P2, &S->m2m_frag[o * fr]);
     // padded multipole layout, transposed [k][row]
     for (int r = 0; r < P2; r++)
       for (int k = 0; k < P2; k++)
         S->m2m[o * vq + (size_t)((k / KS) * KSP + k % KS) * VS + (r / KS) * KSP + r % KS] = A[(size_t)r * P2 + k];
     op_l2l(T, o, A.data());
-    fragment_into(A.data(), P2, &S->l2l_frag[o * fr]);
+    if (frag) fragment_into(A.data(), P2, &S->l2l_frag[o * fr]);
     transpose_into(A.data(), P2, &S->l2l[o * sq]);
   }
   S->nlev = per_level ? lmax - 1 : 1;
@@ -273,12 +278,12 @@ std::shared_ptr<OpSet> get_ops(const Tables& T, int lmax, float rd) {
         offset_vector(oi, d);
         double rbv = per_level ? (double)rd / (double)(1 << (li + 2)) : 1.0;
         op_m2l(T, d[0], d[1], d[2], rbv, B.data());
-        fragment_into(B.data(), P2, &S->m2l_frag[((size_t)li * NOFF + oi) * fr]);
+        if (frag) fragment_into(B.data(), P2, &S->m2l_frag[((size_t)li * NOFF + oi) * fr]);
         transpose_into(B.data(), P2, &S->m2l_kmaj[((size_t)li * NOFF + oi) * sq]);
       }
     });
   for (auto& x : th) x.join();
-  if (!per_level) {
+  if (!per_level && frag) {
     // EXACT: every offset is a reflection of one with non-negative components
     S->m2l_canon.resize((size_t)NCANON * fr);
     for (int ci = 0; ci < NCANON; ci++) {
@@ -326,7 +331,8 @@ std::shared_ptr<OpSet> get_yuk_ops(int p, double lam, float rd, int lmax) {
   S->m2m.assign((size_t)nt_lev * 8 * vq, 0.0);
   S->l2l.assign((size_t)nt_lev * 8 * sq, 0.0);
   S->nlev = lmax - 1;
-  const size_t fr = (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64;
+  const bool frag = p <= P_ALL;  // MFMA fragment order: the FP32 engine's orders
+  const size_t fr = frag ? (size_t)((P2 + 3) / 4) * ((P2 + 15) / 16) * 64 : 0;
   S->m2m_frag.assign((size_t)nt_lev * 8 * fr, 0.0);
   S->l2l_frag.assign((size_t)nt_lev * 8 * fr, 0.0);
   S->m2l_frag.resize((size_t)S->nlev * NOFF * fr);
@@ -347,14 +353,16 @@ std::shared_ptr<OpSet> get_yuk_ops(int p, double lam, float rd, int lmax) {
             for (int k = 0; k < P2; k++)
               M[(size_t)((k / KS) * KSP + k % KS) * VS + (r / KS) * KSP + r % KS] = A[(size_t)r * P2 + k];
           transpose_into(B.data(), P2, &S->l2l[((size_t)li * 8 + o) * sq]);
-          fragment_into(A.data(), P2, &S->m2m_frag[((size_t)li * 8 + o) * fr]);
-          fragment_into(B.data(), P2, &S->l2l_frag[((size_t)li * 8 + o) * fr]);
+          if (frag) {
+            fragment_into(A.data(), P2, &S->m2m_frag[((size_t)li * 8 + o) * fr]);
+            fragment_into(B.data(), P2, &S->l2l_frag[((size_t)li * 8 + o) * fr]);
+          }
         } else {
           const int jj = j - ntr, li = jj / NOFF, oi = jj % NOFF;
           int d[3];
           offset_vector(oi, d);
           yuk_op_m2l(p, lam * (double)rd / (double)(1 << (li + 2)), d[0], d[1], d[2], A.data());
-          fragment_into(A.data(), P2, &S->m2l_frag[((size_t)li * NOFF + oi) * fr]);
+          if (frag) fragment_into(A.data(), P2, &S->m2l_frag[((size_t)li * NOFF + oi) * fr]);
           transpose_into(A.data(), P2, &S->m2l_kmaj[((size_t)li * NOFF + oi) * sq]);
         }
       }
@@ -511,10 +519,14 @@ Engine::~Engine() {
 
 int Engine::init(const fmmx_config* c, int dev, void* strm) {
   if (!c) return fail(FMMX_EINVAL, "config is NULL");
-  if (c->p < 2 || c->p > MAXP) return fail(FMMX_EINVAL, "p out of range [2, FMMX_MAX_P]");
+  if (c->p < 2 || (c->p > P_ALL && c->p != P_HI)) return fail(FMMX_EINVAL, "p must be 2..12 or 18");
   if (c->lmax != 0 && (c->lmax < 2 || c->lmax > MAXL)) return fail(FMMX_EINVAL, "lmax out of range [2, 8]");
   if (c->precision != FMMX_MIXED && c->precision != FMMX_FP64) return fail(FMMX_EINVAL, "bad precision");
   if (c->convention != FMMX_CONV_REF && c->convention != FMMX_CONV_EXACT) return fail(FMMX_EINVAL, "bad convention");
+  // order 18: the vector-unit FP64 kernels only; the REF convention's
+  // float-stored tables underflow at the degrees its operators need (2p - 1)
+  if (c->p > P_ALL && (c->precision != FMMX_FP64 || c->convention != FMMX_CONV_EXACT))
+    return fail(FMMX_EUNSUPPORTED, "p = 18 needs FMMX_FP64 and FMMX_CONV_EXACT");
   if (c->rd > 0 && !(std::isfinite(c->xmin) && std::isfinite(c->ymin) && std::isfinite(c->zmin) && std::isfinite(c->rd)))
     return fail(FMMX_EINVAL, "root cube not finite");
   cfg = *c;
@@ -1203,6 +1215,8 @@ int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, doubl
   FMMX_GUARD({
     if (p < 2 || p > fmmx::MAXP || !A) return fmmx::fail(FMMX_EINVAL, "bad p or NULL output");
     if (convention != FMMX_CONV_REF && convention != FMMX_CONV_EXACT) return fmmx::fail(FMMX_EINVAL, "bad convention");
+    if (convention == FMMX_CONV_REF && p > fmmx::P_ALL)
+      return fmmx::fail(FMMX_EUNSUPPORTED, "the REF convention's float tables underflow above p = 12");
     auto T = fmmx::get_tables(p, convention);
     if (kind == 0 || kind == 1) {
       if (a0 < 0 || a0 > 7) return fmmx::fail(FMMX_EINVAL, "octant out of range");
@@ -1463,8 +1477,9 @@ int fmmx_energy(fmmx_handle* h, int neq, const float* q, double* E, double* dEdq
 
 int fmmx_auto_lmax(const long long* boxes, const double* sumsq, int p, int precision, int* lmax) {
   if (!boxes || !sumsq || !lmax) return fmmx::fail(FMMX_EINVAL, "NULL argument");
-  if (p < 2 || p > fmmx::MAXP) return fmmx::fail(FMMX_EINVAL, "p outside 2..FMMX_MAX_P");
+  if (p < 2 || (p > fmmx::P_ALL && p != fmmx::P_HI)) return fmmx::fail(FMMX_EINVAL, "p must be 2..12 or 18");
   if (precision != FMMX_MIXED && precision != FMMX_FP64) return fmmx::fail(FMMX_EINVAL, "unknown precision");
+  if (p > fmmx::P_ALL && precision != FMMX_FP64) return fmmx::fail(FMMX_EUNSUPPORTED, "p = 18 needs FMMX_FP64");
   long b[fmmx::MAXL + 1];
   double q[fmmx::MAXL + 1];
   for (int l = 0; l <= fmmx::MAXL; l++) {

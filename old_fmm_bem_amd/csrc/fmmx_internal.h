@@ -19,6 +19,9 @@ constexpr int NNEAR = 27;  // near list length bound (3x3x3)
 constexpr int NRBM = 512;  // include/constants.h:22, lattice directions
 constexpr int NOFF = 316;  // M2L offsets: [-3,3]^3 minus [-1,1]^3
 constexpr int MAXP = FMMX_MAX_P;
+// orders built in every precision (2..P_ALL) and the one order above them,
+// FP64 and the EXACT convention only (the reference's mp = 18)
+constexpr int P_ALL = 12, P_HI = 18;
 constexpr float EPSF = 1e-6f;                      // include/constants.h:34
 constexpr float PIF = 3.14159265358979323846f;     // include/constants.h:35
 

@@ -414,6 +414,93 @@ __global__ __launch_bounds__(64) void k_p2m(int nb, const uint32_t* __restrict__
   }
 }
 
+// P2M above order 12 (FP64, EXACT convention): a 64 x P^2 table of particle
+// terms (166 KB at P = 18) no longer fits LDS, so each lane keeps its
+// particle's harmonics in registers (spilled as the compiler needs) and the
+// table goes through LDS 64 columns at a time, each column summed by one lane
+// in fixed row order (deterministic).  YK 1: Yukawa weights as k_p2m;
+// GN 1: the dipole form of k_p2m (Gnj).
+template <int P, int YK, int GN>
+__global__ __launch_bounds__(64) void k_p2m_hi(int nb, const uint32_t* __restrict__ bkey,
+                                               const int* __restrict__ bfirst, const int* __restrict__ blast,
+                                               const float* __restrict__ x, const float* __restrict__ y,
+                                               const float* __restrict__ z, const float* __restrict__ nx,
+                                               const float* __restrict__ ny, const float* __restrict__ nz,
+                                               const float* __restrict__ q, int lev, double xmin, double ymin,
+                                               double zmin, double rb, double* __restrict__ M, double lrb) {
+  constexpr int NR = P * (P + 1) / 2, P2 = P * P, CH = 64, NC = (P2 + CH - 1) / CH;
+  __shared__ double red[64][CH + 1];
+  const int lane = threadIdx.x, b = blockIdx.x;
+  if (b >= nb) return;  // one wave per block: uniform
+  double cx, cy, cz;
+  box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
+  const double rbinv = 1.0 / rb;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) acc[c] = 0;
+  double ib0[P];
+  if (YK) ibar_dev<P - 1>(lrb, ib0);
+  const int i0 = bfirst[b], i1 = blast[b];
+  for (int base = i0; base <= i1; base += 64) {
+    const int i = base + lane;
+    const bool on = i <= i1;
+    double Sr[NR], Si[NR], w[P];
+    const double ux = on ? ((double)x[i] - cx) * rbinv : 0, uy = on ? ((double)y[i] - cy) * rbinv : 0,
+                 uz = on ? ((double)z[i] - cz) * rbinv : 0;
+    solid_harmonics<P, double>(ux, uy, uz, Sr, Si);
+    const double qi = on ? (double)q[i] : 0;
+    if (YK) {
+      double ibx[P];
+      ibar_dev<P - 1>(lrb * sqrt(ux * ux + uy * uy + uz * uz), ibx);
+#pragma unroll
+      for (int n = 0; n < P; n++) w[n] = qi * ibx[n] / ib0[n];
+    } else {
+#pragma unroll
+      for (int n = 0; n < P; n++) w[n] = GN ? rbinv : qi;
+    }
+    // dipole q*n: the float product vj*gxj of kernel/Gn_p2m.cxx:113, as k_p2m
+    const double qx = GN && on ? (double)(q[i] * nx[i]) : 0, qy = GN && on ? (double)(q[i] * ny[i]) : 0,
+                 qz = GN && on ? (double)(q[i] * nz[i]) : 0;
+    // column col of this particle's row: Re slots sl(n, m), then Im slots
+    auto term = [&](int n, int m, bool im) -> double {
+      if (GN) {
+        if (n == 0) return 0;
+        double pr, pi, mr, mi, zr, zi;
+        harm_grad<P, double>(Sr, Si, n, m, pr, pi, mr, mi, zr, zi);
+        if (!im) return w[n] * (0.5 * qx * (pr + mr) + 0.5 * qy * (pi - mi) + qz * zr);
+        return -w[n] * (0.5 * qx * (pi + mi) - 0.5 * qy * (pr - mr) + qz * zi);
+      }
+      return im ? -w[n] * Si[sl(n, m)] : w[n] * Sr[sl(n, m)];
+    };
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+#pragma unroll
+      for (int n = 0; n < P; n++)
+#pragma unroll
+        for (int m = 0; m <= n; m++) {
+          const int cr = sl(n, m), cim = NR + n * (n - 1) / 2 + m - 1;
+          if (cr >= c * CH && cr < (c + 1) * CH) red[lane][cr - c * CH] = term(n, m, false);
+          if (m > 0 && cim >= c * CH && cim < (c + 1) * CH) red[lane][cim - c * CH] = term(n, m, true);
+        }
+      __syncthreads();
+      if (c * CH + lane < P2) {
+        double s = 0;
+        for (int r = 0; r < 64; r++) s += red[r][lane];
+        acc[c] += s;
+      }
+      __syncthreads();
+    }
+  }
+  using ML = MLayout<P>;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int col = c * CH + lane;
+    if (col < P2) M[(size_t)b * ML::VS + ML::at(col)] = acc[c];
+  }
+  for (int c = lane; c < ML::VS; c += 64)
+    if (c % ML::KSP >= ML::KS || (c / ML::KSP) * ML::KS + c % ML::KSP >= P2) M[(size_t)b * ML::VS + c] = 0;
+}
+
 // P2M of charges in the EXACT convention, FP32 (the mixed engine's G, Gni):
 // each lane's particle terms stay in registers and the per-leaf column sums
 // go through LDS in chunks of CH columns, so a workgroup holds 8-17 KB of
@@ -2009,6 +2096,14 @@ int dispatch_hp(int p, F&& f) {
   return fail(FMMX_EUNSUPPORTED, "Helmholtz needs p in {4, 6, 8, 10, 12}");
 }
 
+// the order above P_ALL: FP64 and the EXACT convention (checked when the
+// handle is created), on the vector-unit kernels
+template <typename F>
+int dispatch_hi(int p, F&& f) {
+  if (p == P_HI) return f(std::integral_constant<int, P_HI>{});
+  return fail(FMMX_EUNSUPPORTED, "unsupported p");
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -2063,6 +2158,23 @@ int Engine::run_p2m(int neq) {
   const int nb = sj.nb[lmax];
   if (nb == 0) return FMMX_OK;
   const double* corr = corr_d.as<double>();
+  if (p > P_ALL) {
+    return dispatch_hi(p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+#define FMMX_P2M_HI(YK_, GN_)                                                                                      \
+  k_p2m_hi<P, YK_, GN_><<<nb, 64, 0, stream>>>(nb, sj.bkey[lmax].as<uint32_t>(), sj.bfirst[lmax].as<int>(),        \
+                                               sj.blast.as<int>(), sj.x.as<float>(), sj.y.as<float>(),             \
+                                               sj.z.as<float>(), sj.nx.as<float>(), sj.ny.as<float>(),             \
+                                               sj.nz.as<float>(), qs.as<float>(), lmax, xmin, ymin, zmin, rb(lmax), \
+                                               mexp[lmax].as<double>(), lam * rb(lmax))
+      if (yk) FMMX_P2M_HI(1, 0);
+      else if (neq == FMMX_GNJ) FMMX_P2M_HI(0, 1);
+      else FMMX_P2M_HI(0, 0);
+#undef FMMX_P2M_HI
+      FMMX_HIP(hipGetLastError());
+      return FMMX_OK;
+    });
+  }
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     const double lrb = (hh ? hk : lam) * rb(lmax);
@@ -2131,6 +2243,24 @@ int Engine::run_l2p(int neq, double* const* dst) {
   L2POut lout{};
   if (dst && !hh) lout = L2POut{si.perm.as<uint32_t>(), dst[0], dst[1], dst[2], dst[3]};
   const double* corr = corr_d.as<double>();
+  if (p > P_ALL) {
+    return dispatch_hi(p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      const double lrb = lam * rb(lmax), cl = yk ? rb(lmax) : c4();
+      const int grad = neq == FMMX_G || neq == FMMX_YUKAWA;
+#define FMMX_L2P_HI(YK_, C_)                                                                                    \
+  k_l2p<P, double, YK_, 0, C_><<<nb, 64, 0, stream>>>(                                                          \
+      nb, si.bkey[lmax].as<uint32_t>(), si.bfirst[lmax].as<int>(), si.blast.as<int>(), si.x.as<float>(),      \
+      si.y.as<float>(), si.z.as<float>(), si.nx.as<float>(), si.ny.as<float>(), si.nz.as<float>(),             \
+      lexp[lmax].as<double>(), neq == FMMX_GNI, grad, cfg.convention, lmax, xmin, ymin, zmin, rb(lmax), corr, \
+      cl, vs.as<double>(), gxs.as<double>(), gys.as<double>(), gzs.as<double>(), lrb, P * P, 0, lout)
+      if (yk) FMMX_L2P_HI(1, -1);
+      else FMMX_L2P_HI(0, FMMX_CONV_EXACT);
+#undef FMMX_L2P_HI
+      FMMX_HIP(hipGetLastError());
+      return FMMX_OK;
+    });
+  }
   return dispatch_p(p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     // Yukawa locals carry the kernel's constant: phi = sum L~ h_n S_n(u)
@@ -2223,7 +2353,7 @@ int Engine::run_m2m(int lev) {
       return FMMX_OK;
     });
   }
-  return dispatch_p(p, [&](auto pc) -> int {
+  auto m2m = [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value, V = MLayout<P>::VS;
     if (fp64) {
       k_translate<V, double, 0><<<grid, TR_NT, 0, stream>>>(
@@ -2231,7 +2361,7 @@ int Engine::run_m2m(int lev) {
           mexp[lev + 1].as<double>(), m2m_part.as<double>(), m2m_ops<double>(lev));
       k_m2l_reduce<double><<<(int)((n + 255) / 256), 256, 0, stream>>>(n, 8, m2m_part.as<double>(),
                                                                        mexp[lev].as<double>(), 1);
-    } else {
+    } else if constexpr (P <= P_ALL) {
       k_translate<V, float, 0><<<grid, TR_NT, 0, stream>>>(
           nbp, sj.cfirst[lev].as<int>(), sj.bkey[lev + 1].as<uint32_t>(), nullptr, nullptr, nullptr,
           mexp[lev + 1].as<float>(), m2m_part.as<float>(), m2m_ops<float>(lev));
@@ -2240,7 +2370,8 @@ int Engine::run_m2m(int lev) {
     }
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
-  });
+  };
+  return p > P_ALL ? dispatch_hi(p, m2m) : dispatch_p(p, m2m);
 }
 
 int Engine::run_l2l(int lev, bool add) {
@@ -2274,19 +2405,20 @@ int Engine::run_l2l(int lev, bool add) {
       return FMMX_OK;
     });
   }
-  return dispatch_p(p, [&](auto pc) -> int {
+  auto l2l = [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value, Q = P * P;
     if (fp64)
       k_translate<Q, double, 1><<<grid, TR_NT, 0, stream>>>(
           nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
           cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<double>(), lexp[lev].as<double>(), l2l_ops<double>(lev));
-    else
+    else if constexpr (P <= P_ALL)
       k_translate<Q, float, 1><<<grid, TR_NT, 0, stream>>>(
           nbc, nullptr, si.bkey[lev].as<uint32_t>(), si.parent[lev].as<int>(), cls_cnt[lev].as<int>(),
           cls_ord[lev].as<uint32_t>(), lexp[lev - 1].as<float>(), lexp[lev].as<float>(), l2l_ops<float>(lev));
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
-  });
+  };
+  return p > P_ALL ? dispatch_hi(p, l2l) : dispatch_p(p, l2l);
 }
 
 // Offset splits of the M2L launch at a level (partial sums when > 1).
@@ -2595,7 +2727,7 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       return FMMX_OK;
     });
   }
-  return dispatch_p(p, [&](auto pc) -> int {
+  auto m2l = [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if (fp64) {
       const bool tmark = per_phase && lev == lmax;
@@ -2611,7 +2743,7 @@ int Engine::run_m2l(int lev, bool init, int phase) {
       if (nsplit > 1 && phase == 0)
         k_m2l_reduce<double><<<(int)((nl + 255) / 256), 256, 0, stream>>>(nl, nsplit, part_buf->as<double>(),
                                                                           lexp[lev].as<double>(), init ? 1 : 0);
-    } else {
+    } else if constexpr (P <= P_ALL) {
       const M2LVariant var = m2l_variant(lev);
       M2LLaunch A{};
       A.nlv = 1;
@@ -2644,7 +2776,8 @@ int Engine::run_m2l(int lev, bool init, int phase) {
     }
     FMMX_HIP(hipGetLastError());
     return FMMX_OK;
-  });
+  };
+  return p > P_ALL ? dispatch_hi(p, m2l) : dispatch_p(p, m2l);
 }
 
 int Engine::run_direct(int neq, int ni, const float* xi, const float* yi, const float* zi, const float* nxi,

@@ -280,9 +280,11 @@ void plug_phase(const char* fn, int ph, int neq, int lev, bool grad, int slot) {
 
 extern "C" int fmmx_ref_configure(int p, int precision, int convention, int device) {
   Shim& s = S();
-  if (p < 2 || p > FMMX_MAX_P || (precision != FMMX_MIXED && precision != FMMX_FP64) ||
+  if (p < 2 || (p > 12 && p != FMMX_MAX_P) || (precision != FMMX_MIXED && precision != FMMX_FP64) ||
       (convention != FMMX_CONV_REF && convention != FMMX_CONV_EXACT) || device < 0)
     return FMMX_EINVAL;
+  // mp = 18 (include/constants.h:12): FP64 in the EXACT convention only
+  if (p > 12 && (precision != FMMX_FP64 || convention != FMMX_CONV_EXACT)) return FMMX_EUNSUPPORTED;
   if (s.h) {
     fmmx_destroy(s.h);
     s.h = nullptr;

@@ -64,12 +64,15 @@ namespace {
 using proj::ci;
 using proj::sl;
 using proj::harmonics_all;
-constexpr int NTH = 24, NPH = 36;  // quadrature on the projection sphere
-constexpr double RAD = 0.5;        // its radius in box units
+// quadrature on the projection sphere: 24 x 36 nodes up to p = 12, 2p x 3p
+// above (Gauss-Legendre exact to degree 2 NTH - 1 >= 2(p-1) + the field's
+// decay; the trapezoid rule aliases azimuthal orders NPH - p + 1 and up)
+constexpr double RAD = 0.5;  // its radius in box units
 
 template <class Field>
 void project(int p, int ncol, bool all_k, Field field, std::vector<cplx>& out) {
-  proj::project(p, ncol, all_k, NTH, NPH, RAD, field, out);
+  const int nth = p <= 12 ? 24 : 2 * p, nph = p <= 12 ? 36 : 3 * p;
+  proj::project(p, ncol, all_k, nth, nph, RAD, field, out);
 }
 
 // complex operator C (rows (j, k >= 0) at ci(j,k), columns (n, m) all m at

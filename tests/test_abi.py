@@ -190,12 +190,19 @@ def test_operator_rejects_bad_arguments():
 
 
 def test_create_rejects_bad_config():
-    for kw in [dict(p=1), dict(p=13), dict(lmax=9), dict(precision=7), dict(convention=5)]:
+    for kw in [dict(p=1), dict(p=13), dict(p=17), dict(p=19), dict(lmax=9), dict(precision=7), dict(convention=5)]:
         cfg = F._Config(kw.get("p", 10), kw.get("lmax", 3), kw.get("precision", 0), kw.get("convention", 0), 0, 0, 0,
                         0)
         h = C.c_void_p()
         rc = F.lib().fmmx_create(C.byref(cfg), 0, None, C.byref(h))
         assert rc == 1 and not h.value, kw
+    # order 18 exists for FP64 in the EXACT convention only (checked before any device call)
+    for prec, conv in [(0, 1), (0, 0), (1, 0)]:
+        cfg = F._Config(18, 3, prec, conv, 0, 0, 0, 0)
+        h = C.c_void_p()
+        rc = F.lib().fmmx_create(C.byref(cfg), 0, None, C.byref(h))
+        assert rc == 5 and not h.value, (prec, conv)
+        assert "18" in F.lib().fmmx_last_error().decode()
 
 
 def _uniform_stats(n):

@@ -70,6 +70,28 @@ def test_yukawa_m2l_operator_vs_direct(lam, off):
         assert abs(_l2p(12, lam, 1.0, L, x, np.zeros(3)) - ref) <= 1e-9 * abs(ref)
 
 
+@pytest.mark.parametrize("lam", [0.05, 5.0])
+def test_yukawa_m2l_operator_order_18(lam):
+    """The reference's second order (mp = 18, kernel/precalch.cxx:185-261):
+    the p = 18 operator reaches the truncation error of p = 18, far below
+    what p = 12 gives for the same sources and targets."""
+    rng = np.random.default_rng(9)
+    off = (2, 1, 0)
+    c = -np.array(off, float)
+    src = c + (rng.random((3, 3)) - 0.5) * 0.5
+    q = rng.random(3)
+    xs = (rng.random((3, 3)) - 0.5) * 0.5
+    err = {}
+    for p in (12, 18):
+        L = F.operator(p, "exact", 5, *off, rb=lam) @ _p2m(p, lam, 1.0, src, q, c)
+        e = 0.0
+        for x in xs:
+            ref = sum(qq * spherical_kn(0, lam * np.linalg.norm(x - y)) for y, qq in zip(src, q))
+            e = max(e, abs(_l2p(p, lam, 1.0, L, x, np.zeros(3)) - ref) / abs(ref))
+        err[p] = e
+    assert err[18] < 1e-10 and err[18] < 1e-2 * err[12], err
+
+
 @pytest.mark.parametrize("oct_", [0, 3, 5, 6])
 def test_yukawa_m2m_l2l_operators(oct_):
     """M2M(P2M about the child) ~ P2M about the parent (the Yukawa shift of a
