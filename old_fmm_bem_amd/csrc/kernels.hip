@@ -1858,6 +1858,9 @@ __device__ __forceinline__ void pair(float xi, float yi, float zi, float nxi, fl
 // TPT lanes per target (2: leaves of at most about 32 targets, one wave per
 // workgroup): the lanes of a target take alternating staged sources and add
 // their sums at the end, so a half-empty leaf does not idle half the wave.
+// With TPT 2 each pass over the leaf's targets picks its own lanes per target
+// (2, 4, 8 or 16) from the targets it has left, so a pass of a few targets
+// (a leaf of 33-40, or of 10) still keeps the wave's lanes busy.
 template <typename T, int NEQ, int CONV, int BS, int TPT = 1>
 __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ jkey,
                                             const int* __restrict__ ifirst, const int* __restrict__ ilast,
@@ -1880,11 +1883,19 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
   __shared__ float4 sp[BS];
   __shared__ float4 sn[SN ? BS : 1];
   static_assert(TPT == 1 || (TPT == 2 && BS == 64), "two lanes per target within one wave");
-  constexpr int TB = BS / TPT;  // targets per pass
-  const int b = blockIdx.x, tid = threadIdx.x, tt = tid % TB, part = tid / TB;
+  const int b = blockIdx.x, tid = threadIdx.x;
   if (b >= nbi) return;
   const int i0 = ifirst[b], i1 = ilast[b], nn = nij[b];
-  for (int base = i0; base <= i1; base += TB) {
+  for (int base = i0, tb = BS; base <= i1; base += tb) {
+    // lanes per target of this pass (uniform): TPT, or up to 16 (TPT 2) for
+    // the targets left
+    int tpt = TPT;
+    if (TPT == 2) {
+      const int m = i1 - base + 1;
+      tpt = m > 16 ? 2 : m > 8 ? 4 : m > 4 ? 8 : 16;
+    }
+    tb = BS / tpt;  // targets of this pass
+    const int tt = tid % tb, part = tid / tb;
     const int i = base + tt;
     const bool act = i <= i1;
     float px = 0, py = 0, pz = 0, qx = 0, qy = 0, qz = 0;
@@ -1950,26 +1961,39 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
         __syncthreads();
         const int cnt = cnt0 <= BS ? tot : min(BS, cnt0 - rr * BS);
         T p = T(0), fx = T(0), fy = T(0), fz = T(0);
-        int s = part;
-        // coincident points can only share a box key; other box pairs skip the test
-        if (CONV == FMMX_CONV_EXACT && !self) {
-          for (; s + 3 * TPT < cnt; s += 4 * TPT) {
+        // staged sources part, part + L, part + 2L, ... of the L lanes of a
+        // target (L a compile-time stride: LDS offsets stay immediates)
+        auto sweep = [&](auto lc) {
+          constexpr int L = decltype(lc)::value;
+          int s = part;
+          // coincident points can only share a box key; other box pairs skip the test
+          if (CONV == FMMX_CONV_EXACT && !self) {
+            for (; s + 3 * L < cnt; s += 4 * L) {
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-              pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u * TPT], SN ? sn[s + u * TPT] : sp[0], p,
-                                        fx, fy, fz, lam);
+              for (int u = 0; u < 4; u++)
+                pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s + u * L], SN ? sn[s + u * L] : sp[0], p, fx,
+                                          fy, fz, lam);
+            }
+            for (; s < cnt; s += L)
+              pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
+          } else {
+            for (; s + 3 * L < cnt; s += 4 * L) {
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u * L], SN ? sn[s + u * L] : sp[0], p, fx, fy,
+                                   fz, lam);
+            }
+            for (; s < cnt; s += L)
+              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
           }
-          for (; s < cnt; s += TPT)
-            pair<T, NEQ, CONV, false>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
+        };
+        if constexpr (TPT == 1) {
+          sweep(std::integral_constant<int, 1>{});
         } else {
-          for (; s + 3 * TPT < cnt; s += 4 * TPT) {
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-              pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s + u * TPT], SN ? sn[s + u * TPT] : sp[0], p, fx, fy,
-                                 fz, lam);
-          }
-          for (; s < cnt; s += TPT)
-            pair<T, NEQ, CONV>(px, py, pz, qx, qy, qz, sp[s], SN ? sn[s] : sp[0], p, fx, fy, fz, lam);
+          if (tpt == 2) sweep(std::integral_constant<int, 2>{});
+          else if (tpt == 4) sweep(std::integral_constant<int, 4>{});
+          else if (tpt == 8) sweep(std::integral_constant<int, 8>{});
+          else sweep(std::integral_constant<int, 16>{});
         }
         ap += (double)p;
         if (GRAD || HH) {
@@ -1981,12 +2005,15 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
       e += nbox;
     }
     if (TPT == 2) {
-      // the two lanes of a target are tid and tid + 32 of the one wave
-      ap += __shfl_xor(ap, 32, 64);
-      if (GRAD || HH) {
-        ax += __shfl_xor(ax, 32, 64);
-        ay += __shfl_xor(ay, 32, 64);
-        az += __shfl_xor(az, 32, 64);
+      // the lanes of a target are tt, tt + tb, tt + 2 tb, ... of the one
+      // wave: butterfly over the lane bits at and above tb (fixed order)
+      for (int d = tb; d < 64; d <<= 1) {
+        ap += __shfl_xor(ap, d, 64);
+        if (GRAD || HH) {
+          ax += __shfl_xor(ax, d, 64);
+          ay += __shfl_xor(ay, d, 64);
+          az += __shfl_xor(az, d, 64);
+        }
       }
     }
     if (act && part == 0) {
