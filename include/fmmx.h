@@ -348,8 +348,8 @@ int fmmx_get_locals(fmmx_handle* h, int lev, double* ax_reim);
  * multipole basis k_n(lambda r) Y_n^m and local basis i_n(lambda r) Y_n^m:
  * kind 3: M2M from child octant a0; 4: L2L to child octant a0; 5: M2L for
  * offset (a0,a1,a2).  The convention does not enter kinds 3-5.
- * p = 2..FMMX_MAX_P (any order on the host; FMMX_CONV_REF up to 12, where its
- * float-stored tables still hold the factorials).
+ * p = 2..FMMX_MAX_P (any order on the host; FMMX_CONV_REF up to 12: above it
+ * its float-stored anm table turns subnormal or zero).
  * Helmholtz (FMMX_HELMHOLTZ), where rb is k times the box size (of the child
  * for 6 and 7): (2 p^2) x (2 p^2) operators on split expansions, the real
  * slots of the Re part then of the Im part (multipole basis

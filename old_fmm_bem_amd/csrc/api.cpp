@@ -524,7 +524,8 @@ int Engine::init(const fmmx_config* c, int dev, void* strm) {
   if (c->precision != FMMX_MIXED && c->precision != FMMX_FP64) return fail(FMMX_EINVAL, "bad precision");
   if (c->convention != FMMX_CONV_REF && c->convention != FMMX_CONV_EXACT) return fail(FMMX_EINVAL, "bad convention");
   // order 18: the vector-unit FP64 kernels only; the REF convention's
-  // float-stored tables underflow at the degrees its operators need (2p - 1)
+  // float-stored anm table turns subnormal or zero at the degrees its
+  // operators need (up to 2p - 1 = 35)
   if (c->p > P_ALL && (c->precision != FMMX_FP64 || c->convention != FMMX_CONV_EXACT))
     return fail(FMMX_EUNSUPPORTED, "p = 18 needs FMMX_FP64 and FMMX_CONV_EXACT");
   if (c->rd > 0 && !(std::isfinite(c->xmin) && std::isfinite(c->ymin) && std::isfinite(c->zmin) && std::isfinite(c->rd)))
@@ -1216,7 +1217,7 @@ int fmmx_operator(int p, int convention, int kind, int a0, int a1, int a2, doubl
     if (p < 2 || p > fmmx::MAXP || !A) return fmmx::fail(FMMX_EINVAL, "bad p or NULL output");
     if (convention != FMMX_CONV_REF && convention != FMMX_CONV_EXACT) return fmmx::fail(FMMX_EINVAL, "bad convention");
     if (convention == FMMX_CONV_REF && p > fmmx::P_ALL)
-      return fmmx::fail(FMMX_EUNSUPPORTED, "the REF convention's float tables underflow above p = 12");
+      return fmmx::fail(FMMX_EUNSUPPORTED, "the REF convention's float tables turn subnormal above p = 12");
     auto T = fmmx::get_tables(p, convention);
     if (kind == 0 || kind == 1) {
       if (a0 < 0 || a0 > 7) return fmmx::fail(FMMX_EINVAL, "octant out of range");
