@@ -733,22 +733,19 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
       c[7] = zi;
     }
     __syncthreads();
-    if (!valid) return;
-    double cx, cy, cz;
-    box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
     const double rbinv = 1.0 / rb;
-    const int i0 = bfirst[b], i1 = blast[b];
     // a compile-time convention keeps the REF angle code (double acos,
     // atan, sincos) out of the EXACT kernel's register budget
     const int cv = CONV >= 0 ? CONV : conv;
-    for (int i = i0 + lane; i <= i1; i += STEP) {
+    // target i of the leaf whose table is Cs[hs], centre (cx, cy, cz)
+    auto target = [&](int i, int hs, double cx, double cy, double cz) {
       double ux, uy, uz;
       eff_point(cv, (double)x[i] - cx, (double)y[i] - cy, (double)z[i] - cz, rbinv, ux, uy, uz);
       const T tx = T(ux), ty = T(uy), tz = T(uz), r2 = tx * tx + ty * ty + tz * tz;
       T phi = T(0), gx = T(0), gy = T(0), gz = T(0);
       // the table holds zero gradient entries at degree p-1
       auto term = [&](T sr, T si, int s) {
-        const T* c = Cs[h][s];
+        const T* c = Cs[hs][s];
         phi += sr * c[0] - si * c[1];
         gx += sr * c[2] - si * c[3];
         gy += sr * c[4] + si * c[5];
@@ -783,6 +780,27 @@ __global__ __launch_bounds__(64) void k_l2p(int nb, const uint32_t* __restrict__
       }
       l2p_store(out, i, gn, grad, c4 * rbinv, c4 * rbinv * rbinv, (double)phi, (double)gx, (double)gy, (double)gz, nx, ny,
                 nz, vs, gxs, gys, gzs);
+    };
+    if constexpr (HALF != 0) {
+      // the targets of the wave's two leaves, one list over all 64 lanes: a
+      // leaf of 33-40 targets beside a smaller one still takes one pass
+      const int bA = 2 * (int)blockIdx.x, bB = bA + 1;  // bA < nb: the grid is (nb + 1) / 2
+      const int a0 = bfirst[bA], nA = max(0, blast[bA] - a0 + 1);
+      const bool vB = bB < nb;
+      const int b0 = vB ? bfirst[bB] : 0, nB = vB ? max(0, blast[bB] - b0 + 1) : 0;
+      double axc, ayc, azc, bxc = 0, byc = 0, bzc = 0;
+      box_center(bkey[bA], lev, xmin, ymin, zmin, rb, axc, ayc, azc);
+      if (vB) box_center(bkey[bB], lev, xmin, ymin, zmin, rb, bxc, byc, bzc);
+      for (int t = (int)threadIdx.x; t < nA + nB; t += 64) {
+        const bool hb = t >= nA;
+        target(hb ? b0 + (t - nA) : a0 + t, hb ? 1 : 0, hb ? bxc : axc, hb ? byc : ayc, hb ? bzc : azc);
+      }
+    } else {
+      if (!valid) return;
+      double cx, cy, cz;
+      box_center(bkey[b], lev, xmin, ymin, zmin, rb, cx, cy, cz);
+      const int i0 = bfirst[b], i1 = blast[b];
+      for (int i = i0 + lane; i <= i1; i += STEP) target(i, h, cx, cy, cz);
     }
     return;
   }
