@@ -2175,12 +2175,18 @@ int Engine::run_p2p(int neq) {
         gzs.as<double>(), kl);
     return FMMX_OK;
   };
+  // leaves of at most `small` targets on average: one wave per leaf with
+  // lanes per target chosen per pass (TPT 2; measured 36-47% less P2P time
+  // than one lane per target in 64/128-lane blocks at 46-61 targets per
+  // leaf); FMMX_P2P_SMALL sets the bound
+  double small = 96.0;
+  if (const char* e = getenv("FMMX_P2P_SMALL")) small = atof(e);
   auto by_bs = [&](auto t, auto nq, auto cv) -> int {
     using one = std::integral_constant<int, 1>;
+    if (mean <= small) return go(t, nq, cv, std::integral_constant<int, 64>{}, std::integral_constant<int, 2>{});
     if (mean > 96) return go(t, nq, cv, std::integral_constant<int, 256>{}, one{});
     if (mean > 40) return go(t, nq, cv, std::integral_constant<int, 128>{}, one{});
-    if (mean > 32) return go(t, nq, cv, std::integral_constant<int, 64>{}, one{});
-    return go(t, nq, cv, std::integral_constant<int, 64>{}, std::integral_constant<int, 2>{});
+    return go(t, nq, cv, std::integral_constant<int, 64>{}, one{});
   };
   auto by_conv = [&](auto t, auto nq) -> int {
     return ref ? by_bs(t, nq, std::integral_constant<int, FMMX_CONV_REF>{})
