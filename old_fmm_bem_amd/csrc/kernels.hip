@@ -1898,8 +1898,12 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
   // imaginary part of the potential
   constexpr bool HH = NEQ == FMMX_HELMHOLTZ, SN = NEQ == FMMX_GNJ || HH;
   const T lam = T(lamd);
-  __shared__ float4 sp[BS];
-  __shared__ float4 sn[SN ? BS : 1];
+  // staging slots (SB = SPT x BS, each thread stages SPT of them) and near
+  // boxes packed per round: the one-wave form stages up to 256 sources of up
+  // to 8 boxes, so a leaf's 27 near boxes take a few rounds instead of ~14
+  constexpr int SPT = TPT == 2 ? 4 : 1, SB = SPT * BS, KB = TPT == 2 ? 8 : 4;
+  __shared__ float4 sp[SB];
+  __shared__ float4 sn[SN ? SB : 1];
   static_assert(TPT == 1 || (TPT == 2 && BS == 64), "two lanes per target within one wave");
   const int b = blockIdx.x, tid = threadIdx.x;
   if (b >= nbi) return;
@@ -1930,54 +1934,63 @@ __global__ __launch_bounds__(BS) void k_p2p(int nbi, const uint32_t* __restrict_
     double ap = 0, ax = 0, ay = 0, az = 0;
     const uint32_t kb = ikey[b];
     for (int e = 0; e < nn;) {
-      // one staging round: near boxes e .. e + nbox - 1 packed into the BS
+      // one staging round: near boxes e .. e + nbox - 1 packed into the SB
       // slots when they fit together (fewer barriers and shorter tails for
-      // small leaves), else box e's sources BS at a time
+      // small leaves), else box e's sources SB at a time.  The target's own
+      // box keeps a round to itself: only its pairs need the coincident test
       int jj = neij[b * NNEAR + e];
       int j0 = jfirst[jj], j1 = jlast[jj];
       int nbox = 1, cnt0 = j1 - j0 + 1, tot = cnt0;
-      int s0[4] = {j0, 0, 0, 0}, c0[4] = {cnt0, 0, 0, 0};
-      bool self = jkey[jj] == kb, packing = cnt0 <= BS;
+      int s0[KB], c0[KB];
+      s0[0] = j0;
+      c0[0] = cnt0;
+      const bool self = jkey[jj] == kb;
+      bool packing = cnt0 <= SB && !self;
 #pragma unroll
-      for (int k = 1; k < 4; k++) {
+      for (int k = 1; k < KB; k++) {
+        s0[k] = 0;
+        c0[k] = 0;
         if (packing && e + k < nn) {
           const int jn = neij[b * NNEAR + e + k];
           const int a0 = jfirst[jn], c = jlast[jn] - a0 + 1;
-          if (tot + c <= BS) {
+          if (tot + c <= SB && jkey[jn] != kb) {
             s0[k] = a0;
             c0[k] = c;
             tot += c;
-            self = self || jkey[jn] == kb;
             nbox = k + 1;
           } else {
             packing = false;
           }
         }
       }
-      const int rounds = cnt0 <= BS ? 1 : (cnt0 + BS - 1) / BS;
+      const int rounds = cnt0 <= SB ? 1 : (cnt0 + SB - 1) / SB;
       for (int rr = 0; rr < rounds; rr++) {
-        int js = -1;
-        if (cnt0 <= BS) {
-          // slot tid -> (packed box, offset)
-          int t = tid;
+        __syncthreads();
 #pragma unroll
-          for (int k = 0; k < 4; k++) {
-            if (js < 0 && k < nbox) {
-              if (t < c0[k]) js = s0[k] + t;
-              else t -= c0[k];
+        for (int u = 0; u < SPT; u++) {
+          const int slot = tid + u * BS;
+          int js = -1;
+          if (cnt0 <= SB) {
+            // slot -> (packed box, offset)
+            int t = slot;
+#pragma unroll
+            for (int k = 0; k < KB; k++) {
+              if (js < 0 && k < nbox) {
+                if (t < c0[k]) js = s0[k] + t;
+                else t -= c0[k];
+              }
             }
+          } else if (rr * SB + slot < cnt0) {
+            js = j0 + rr * SB + slot;
           }
-        } else if (rr * BS + tid < cnt0) {
-          js = j0 + rr * BS + tid;
+          if (js >= 0) {
+            sp[slot] = make_float4(xj[js], yj[js], zj[js], q[js]);
+            if (NEQ == FMMX_GNJ) sn[slot] = make_float4(nxj[js], nyj[js], nzj[js], 0.f);
+            if (HH) sn[slot] = make_float4(nxj[js], 0.f, 0.f, 0.f);
+          }
         }
         __syncthreads();
-        if (js >= 0) {
-          sp[tid] = make_float4(xj[js], yj[js], zj[js], q[js]);
-          if (NEQ == FMMX_GNJ) sn[tid] = make_float4(nxj[js], nyj[js], nzj[js], 0.f);
-          if (HH) sn[tid] = make_float4(nxj[js], 0.f, 0.f, 0.f);
-        }
-        __syncthreads();
-        const int cnt = cnt0 <= BS ? tot : min(BS, cnt0 - rr * BS);
+        const int cnt = cnt0 <= SB ? tot : min(SB, cnt0 - rr * SB);
         T p = T(0), fx = T(0), fy = T(0), fz = T(0);
         // staged sources part, part + L, part + 2L, ... of the L lanes of a
         // target (L a compile-time stride: LDS offsets stay immediates)
